@@ -1,0 +1,211 @@
+"""End-to-end normal-estimation benchmark on one MI355X.
+
+One step = the whole hot path on one synthetic cloud already resident in HBM:
+cube bounds -> Morton encode -> radix sort -> duplicate removal -> decode ->
+Karras radix tree + boxes -> octree -> exact kNN (k=16) -> normals, i.e. the
+reference's RadixTree + Octree + NormalEstimation::estimateNormals<k>
+(src/main.cu:80-107) on a 8,388,608-point, 200-primitive scene.
+
+Prints ONE JSON line (rank 0, N=1).  `value` = input points / wall time of one
+step (device-resident input and outputs).  `roofline` refers to the dominant
+kernel (kNN), timed with HIP events on the stream it runs on; `cpu_baseline`
+times the CPU restatement of the reference pipeline (oracle/) on a bounded
+sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "points/sec end-to-end normals at 8M pts k=16; learned vs plain kNN at ≤1° p99"
+HBM_PEAK_GBS = 8000.0
+
+
+def knn_bytes_per_query(k: int) -> int:
+    """Algorithmic HBM bytes per query of the kNN kernel: its own point (16 B
+    float4), the traversal record of one internal node (64 B; the tree is read
+    once in aggregate, U-1 records for U queries) and the k output indices."""
+    return 16 + 64 + 4 * k
+
+
+def pipeline_bytes_per_point(key_bytes: int, passes: int, k: int, r: float) -> float:
+    """Byte model of the whole path per input point (137 + 8K + 2P(K+4) + 112r + 8k)."""
+    return 137 + 8 * key_bytes + 2 * passes * (key_bytes + 4) + 112 * r + 8 * k
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(workload_n: int, k: int) -> dict:
+    from oracle import oracle as O
+
+    import shapeextraction_amd as sx
+
+    n = int(os.environ.get("SX_CPU_SAMPLE", 1_048_576))
+    x, y, z = sx.gen_scene(2, n)
+    t0 = time.perf_counter()
+    U, _, stage = O.pipeline(x, y, z, 63, k, knn_mode=0)
+    dt = time.perf_counter() - t0
+    return {
+        "value": n / dt,
+        "unit": "points/s",
+        "cores": O.threads(),
+        "kind": "port",
+        "sample": f"{n}-point scene from the same generator and seed as the {workload_n}-point workload; "
+                  f"reference pipeline restated on the host (32-slot queue kNN, FP32 Jacobi), "
+                  f"{dt:.1f} s, stage ms {[round(float(s), 1) for s in stage]}",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--k", type=int, default=16)
+    ap.add_argument("--n", type=int, default=8_388_608)
+    ap.add_argument("--bucket", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        return 2
+
+    import shapeextraction_amd as sx
+
+    if sx.device_count() == 0:
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+    dev = sx.device_name(0)
+    log(f"device {dev}")
+    n, k = args.n, args.k
+    t = time.perf_counter()
+    x, y, z = sx.gen_scene(2, n)
+    log(f"generated {n} points in {time.perf_counter() - t:.1f} s")
+    st = sx.Stream()
+    dx, dy, dz = (sx.DeviceArray.from_numpy(a) for a in (x, y, z))
+    ix = sx.Index(n, 63)
+    ix.set_bucket(args.bucket)
+    lib = sx.load()
+
+    # first build to learn U (the only host read of a count; untimed)
+    ix.build(dx, dy, dz, stream=st.handle)
+    v = ix.view(stream=st.handle)
+    U, O_nodes = int(v.n_unique), int(v.n_oct)
+    log(f"U={U} octree nodes={O_nodes} r={O_nodes / U:.3f}")
+    nn = sx.DeviceArray((U, k), np.int32)
+    normals = sx.DeviceArray((U, 3), np.float32)
+    ev = [[sx.Event() for _ in range(4)] for _ in range(args.steps)]
+
+    def step(events=None):
+        if events:
+            events[0].record(st.handle)
+        sx._lib.call("sx_index_build", ix.handle, dx.ptr, dy.ptr, dz.ptr, n, st.handle)
+        if events:
+            events[1].record(st.handle)
+        sx._lib.call("sx_knn", ix.handle, 0, U, k, sx.SX_KNN_PLAIN, None, nn.ptr, st.handle)
+        if events:
+            events[2].record(st.handle)
+        sx._lib.call("sx_normals", ix.handle, nn.ptr, 0, U, k, sx.SX_PREC_FP32, 0.0, 0.0, 0.0, normals.ptr, None,
+                     st.handle)
+        if events:
+            events[3].record(st.handle)
+
+    for _ in range(args.warmup):
+        step()
+    st.sync()
+    sx.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step(ev[s])
+    st.sync()
+    sx.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    build_ms = np.array([e[0].elapsed_ms(e[1]) for e in ev])
+    knn_ms = np.array([e[1].elapsed_ms(e[2]) for e in ev])
+    nrm_ms = np.array([e[2].elapsed_ms(e[3]) for e in ev])
+    step_ms = np.array([e[0].elapsed_ms(e[3]) for e in ev])
+    # sanity: counts unchanged, outputs finite
+    v2 = ix.view(stream=st.handle)
+    assert int(v2.n_unique) == U and int(v2.n_oct) == O_nodes
+    nrm_host = normals.numpy()
+    assert np.isfinite(nrm_host).all()
+
+    value = n / (ms_per_step / 1000.0)
+    kb = knn_bytes_per_query(k) * U
+    knn_avg = float(knn_ms.mean())
+    achieved = kb / (knn_avg / 1000.0) / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / "knn_scene8m_traffic.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+    r = O_nodes / U
+    pipe_bytes = pipeline_bytes_per_point(8, 8, k, r) * n
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"scene8m: {n} float32 points, 200 random primitives (planes/spheres/cylinders, "
+                        f"sizes 1-20) in [0,100]^3, noise 0.01, 1% outliers, seed 2; 63-bit Morton codes; "
+                        f"k={k}; FP32 normals",
+            "knn": "plain",
+            "bucket": args.bucket,
+            "n_unique": U,
+            "n_octree_nodes": O_nodes,
+            "device": dev,
+        },
+        "stage_ms": {
+            "index_build": float(np.median(build_ms)),
+            "knn": float(np.median(knn_ms)),
+            "normals": float(np.median(nrm_ms)),
+            "step_event_median": float(np.median(step_ms)),
+        },
+        "roofline": {
+            "kernel": "knn_kernel<16>",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": kb,
+        },
+        "pipeline_roofline": {
+            "bytes_per_point": pipeline_bytes_per_point(8, 8, k, r),
+            "achieved_GBs": pipe_bytes / (ms_per_step / 1000.0) / 1e9,
+            "frac": pipe_bytes / (ms_per_step / 1000.0) / 1e9 / HBM_PEAK_GBS,
+        },
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        line["cpu_baseline"] = cpu_baseline(n, k)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

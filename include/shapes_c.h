@@ -1,0 +1,256 @@
+/*
+ * shapes_c.h - C ABI of the MI355X (gfx950) normal-estimation pipeline.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, returns an
+ * int status (0 = SX_OK) and never throws.  sx_last_error() returns a
+ * thread-local message for the last failing call on the calling thread.
+ *
+ * Conventions
+ *   - Pointers named d_* or documented as "device" are device (HBM) pointers.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     Compute entry points are asynchronous on that stream and never
+ *     synchronise, unless documented otherwise.
+ *   - Stage entry points that need scratch memory take (void* ws, size_t*
+ *     ws_bytes): call once with ws == NULL to receive the size, then again
+ *     with a device buffer of at least that size (the two-phase pattern cub
+ *     uses at src/RadixTree.cu:318-331 of the reference).
+ *   - Data-dependent counts (unique points, octree nodes) stay on the device
+ *     (int32 at a caller-visible address) so that a chain of calls never waits
+ *     for the host; kernels size their grids from a host capacity and read the
+ *     true count from device memory.
+ *
+ * Reference files cited below are those of Wehrdo/ShapeExtraction.
+ */
+#ifndef SHAPES_C_H_
+#define SHAPES_C_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SX_ABI_VERSION 1
+
+/* Status codes. */
+#define SX_OK 0
+#define SX_EINVAL 1   /* invalid argument (null pointer, bad size, unsupported k) */
+#define SX_ENOMEM 2   /* device allocation failed or workspace too small          */
+#define SX_EHIP 3     /* HIP runtime error                                          */
+#define SX_ERANGE 4   /* a data-dependent size exceeded the capacity given          */
+#define SX_EDATA 5    /* invalid input data (non-finite coordinates)                */
+#define SX_ESTATE 6   /* object used before it was built                            */
+
+/* Karras root rule.  The reference gives the root l = N-1 with N = n_unique-1
+ * (src/RadixTree.cu:128-131 with :353-362), so its root spans leaves
+ * [0, n_unique-2].  TEXTBOOK spans [0, n_unique-1].  The two agree except when
+ * the last code alone forms the root's right subtree or that subtree is exactly
+ * the last two leaves. */
+#define SX_ROOT_TEXTBOOK 0
+#define SX_ROOT_REFERENCE 1
+
+/* kNN modes. */
+#define SX_KNN_PLAIN 0    /* exact nearest-first descent from the root            */
+#define SX_KNN_RADIUS 1   /* exact descent seeded with a per-query radius          */
+
+/* Normal precision modes. */
+#define SX_PREC_FP32 0    /* compensated FP32 covariance + closed-form FP32 eigen,
+                             FP64 Jacobi for near-repeated smallest eigenvalues   */
+#define SX_PREC_FP64 1    /* FP64 covariance + FP64 Jacobi for every query         */
+
+/* Octree node; byte-identical to OT::OTNode (src/Octree.hpp:19-35): 56 bytes. */
+typedef struct sx_otnode {
+    int32_t children[8];
+    float corner[3];
+    float cell_size;
+    int32_t child_node_mask;
+    int32_t child_leaf_mask;
+} sx_otnode;
+
+/* ------------------------------------------------------------------------ */
+/* Library and device runtime (plumbing for harnesses in any language).      */
+/* ------------------------------------------------------------------------ */
+const char* sx_last_error(void);
+int sx_abi_version(void);
+int sx_device_count(int* n);
+int sx_set_device(int device);
+int sx_device_name(int device, char* buf, size_t buf_len);
+int sx_malloc(void** d_ptr, size_t bytes);
+int sx_free(void* d_ptr);
+int sx_host_malloc(void** h_ptr, size_t bytes);   /* pinned host memory */
+int sx_host_free(void* h_ptr);
+int sx_memcpy_htod(void* d_dst, const void* h_src, size_t bytes, void* stream);
+int sx_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes, void* stream);
+int sx_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes, void* stream);
+int sx_fill_u8(void* d_dst, uint8_t value, size_t bytes, void* stream);  /* kernel fill, graph-safe */
+int sx_stream_create(void** stream);
+int sx_stream_destroy(void* stream);
+int sx_stream_sync(void* stream);
+int sx_device_sync(void);
+int sx_event_create(void** event);
+int sx_event_destroy(void* event);
+int sx_event_record(void* event, void* stream);
+int sx_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic workloads (host code; deterministic counter-based RNG).          */
+/* ------------------------------------------------------------------------ */
+uint64_t sx_fnv64(const void* data, size_t bytes);
+int sx_gen_sphere(uint64_t seed, int64_t n, float noise_sigma, float* x, float* y, float* z);
+int sx_gen_scene(uint64_t seed, int64_t n, int n_prims, float box, float noise_sigma, float outlier_frac,
+                 float* x, float* y, float* z);
+int sx_gen_gmm_scene(uint64_t seed, int64_t n, int n_prims, int n_components, float box, float noise_sigma,
+                     float max_density_ratio, float* x, float* y, float* z, float* base_fraction_out);
+int sx_gen_batch_sizes(uint64_t seed, int n_clouds, int64_t lo, int64_t hi, int64_t* sizes);
+int sx_gen_degenerate(uint64_t seed, int64_t n, int n_per_class_structs, float offset, float box,
+                      float* x, float* y, float* z, uint8_t* cls, float* dir);
+
+/* ------------------------------------------------------------------------ */
+/* Stage entry points (one per reference kernel / library call).              */
+/* ------------------------------------------------------------------------ */
+
+/* Cube bounds of the cloud: d_bounds = {min over x,y,z; max over x,y,z;
+ * max-min}.  Replaces the six cub DeviceReduce::Min/Max calls and the host
+ * min/max of RadixTree::encodePoints (src/RadixTree.cu:208-241).  Non-finite
+ * coordinates set *d_bad (device int32, may be NULL) to their count. */
+int sx_bbox(const float* d_x, const float* d_y, const float* d_z, int64_t n, float* d_bounds, int32_t* d_bad,
+            void* ws, size_t* ws_bytes, void* stream);
+
+/* Morton codes of the points under the cube d_bounds; code_bits is 63 (the
+ * reference's CODE_LEN, src/MortonUtils.hpp:10) or 30 (10 bits per axis).
+ * Replaces pointToCode + makeCodes (src/MortonUtils.cu:7-19,
+ * src/RadixTree.cu:21-39).  A zero-extent cube gives code 0. */
+int sx_morton_encode(const float* d_x, const float* d_y, const float* d_z, int64_t n, const float* d_bounds,
+                     int code_bits, uint64_t* d_codes, void* stream);
+
+/* Decoded points (12-byte AoS x,y,z) of codes[0..count), count = *d_count if
+ * d_count != NULL else n_max.  Replaces codeToPoint + decodePoints
+ * (src/MortonUtils.cu:21-28, src/Octree.cu:14-24). */
+int sx_morton_decode(const uint64_t* d_codes, const int32_t* d_count, int64_t n_max, const float* d_bounds,
+                     int code_bits, float* d_points, void* stream);
+
+/* Stable ascending LSD radix sort of (key, value) pairs over key bits
+ * [0, key_bits).  d_vals_in == NULL sorts the values 0..n-1 (the sort
+ * permutation).  Replaces cub DeviceRadixSort::SortKeys
+ * (src/RadixTree.cu:316-335), which sorts keys only. */
+int sx_sort_pairs(const uint64_t* d_keys_in, const uint32_t* d_vals_in, int64_t n, int key_bits,
+                  uint64_t* d_keys_out, uint32_t* d_vals_out, void* ws, size_t* ws_bytes, void* stream);
+
+/* Duplicate removal over sorted codes: d_unique[0..U) are the distinct codes,
+ * d_rep[u] the value (input index) of the first element of run u,
+ * d_unique_of_sorted[i] the run of sorted element i (nullable), *d_count = U.
+ * Replaces findDups + InclusiveSum + moveDups (src/RadixTree.cu:71-97,258-303). */
+int sx_unique(const uint64_t* d_sorted, const uint32_t* d_vals, int64_t n, uint64_t* d_unique, uint32_t* d_rep,
+              int32_t* d_unique_of_sorted, int32_t* d_count, void* ws, size_t* ws_bytes, void* stream);
+
+/* Karras binary radix tree over the U = *d_count sorted unique codes (arrays
+ * sized >= U, capacity n_max).  Output arrays follow RT::Nodes
+ * (src/RadixTree.hpp:17-34): prefix_n, left_child, has_leaf_left/right,
+ * parent (parent[0] = -1).  Replaces constructTree (src/RadixTree.cu:99-194).
+ * d_node_end (nullable) receives the far end j of each internal node's leaf
+ * range [min(i,j), max(i,j)]; d_leaf_parent (nullable) receives for every
+ * leaf u the internal node holding it, times 2, plus 1 for a right child. */
+int sx_build_radix_tree(const uint64_t* d_codes, const int32_t* d_count, int64_t n_max, int code_bits,
+                        int root_mode, uint8_t* d_prefix_n, int32_t* d_left_child, uint8_t* d_has_leaf_left,
+                        uint8_t* d_has_leaf_right, int32_t* d_parent, int32_t* d_node_end,
+                        int32_t* d_leaf_parent, void* stream);
+
+/* Octree from the radix tree: edge counts, own scan, node chains and leaf
+ * links, with OT::OTNode layout and node numbering identical to the
+ * reference (src/Octree.cu:38-50,65-161,213-261).  oct_capacity bounds the
+ * node array; *d_oct_count receives the node count (SX_ERANGE is reported by
+ * sx_check_counts when it exceeds the capacity; nodes beyond it are not
+ * written). */
+int sx_build_octree(const uint64_t* d_codes, const int32_t* d_count, int64_t n_max, int code_bits,
+                    const float* d_bounds, const uint8_t* d_prefix_n, const int32_t* d_left_child,
+                    const uint8_t* d_has_leaf_left, const uint8_t* d_has_leaf_right, const int32_t* d_parent,
+                    sx_otnode* d_nodes, int64_t oct_capacity, int32_t* d_oct_count, void* ws, size_t* ws_bytes,
+                    void* stream);
+
+/* Reads device counts (synchronises `stream`) and validates them against
+ * capacities; any pointer may be NULL. */
+int sx_check_counts(const int32_t* d_count, int64_t count_capacity, const int32_t* d_oct_count,
+                    int64_t oct_capacity, const int32_t* d_bad, int64_t* count_out, int64_t* oct_count_out,
+                    void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Index object: one cloud's full spatial index, built without host syncs.   */
+/* Replaces the RT::RadixTree and OT::Octree constructors                    */
+/* (src/RadixTree.cu:305-382, src/Octree.cu:213-319).                        */
+/* ------------------------------------------------------------------------ */
+typedef struct sx_index sx_index;
+
+typedef struct sx_index_view {
+    int64_t n_input;     /* points given to the last build                  */
+    int64_t n_unique;    /* RadixTree::n_pts after duplicate removal        */
+    int64_t n_oct;       /* Octree::n_nodes                                 */
+    float min_coord;     /* RadixTree::min_coord                            */
+    float max_coord;     /* RadixTree::max_coord                            */
+    int code_bits;
+    /* device arrays */
+    const uint64_t* codes;              /* [n_unique] RT::Nodes::mortonCode          */
+    const uint8_t* has_leaf_left;       /* [n_unique]                                */
+    const uint8_t* has_leaf_right;      /* [n_unique]                                */
+    const uint8_t* prefix_n;            /* [n_unique]                                */
+    const int32_t* left_child;          /* [n_unique]                                */
+    const int32_t* parent;              /* [n_unique]                                */
+    const sx_otnode* oct_nodes;         /* [n_oct]    Octree::u_nodes                */
+    const float* points;                /* [3*n_unique] Octree::u_points (12-B AoS)  */
+    const uint32_t* sorted_perm;        /* [n_input] stable sort permutation         */
+    const uint32_t* rep;                /* [n_unique] first input index of each code */
+    const int32_t* unique_of_input;     /* [n_input] unique id of each input point   */
+    const int32_t* d_counts;            /* device {n_unique, n_oct, n_bad}           */
+    const float* d_bounds;              /* device {min, max, range}                  */
+} sx_index_view;
+
+/* capacity: maximum input points; oct_factor: octree node capacity as a
+ * multiple of capacity (<= 0 selects 3.0). */
+int sx_index_create(sx_index** out, int64_t capacity, int code_bits, int root_mode, float oct_factor);
+int sx_index_destroy(sx_index* index);
+/* Builds the index from device SoA coordinates; asynchronous on `stream`. */
+int sx_index_build(sx_index* index, const float* d_x, const float* d_y, const float* d_z, int64_t n,
+                   void* stream);
+/* Synchronises `stream`, validates counts and fills *view. */
+int sx_index_get_view(sx_index* index, sx_index_view* view, void* stream);
+/* Bucket size of the kNN descent: a child holding at most `bucket` points is
+ * scanned linearly instead of descended (default 16; tuning knob, no effect
+ * on results).  No reference counterpart (the reference descends to single
+ * points, src/Octree.hpp:160-178). */
+int sx_index_set_bucket(sx_index* index, int bucket);
+/* Number of radius-seeded queries that had to be searched again unseeded
+ * since the last reset (synchronises `stream`). */
+int sx_index_knn_reruns(sx_index* index, int64_t* reruns, int reset, void* stream);
+
+/* Exact k nearest neighbours of the index points [q_start, q_start+n_q)
+ * among all index points, k in {8, 16, 24, 32}.  Row q of d_out holds k
+ * indices ascending by (squared distance, index); the query itself is
+ * included; rows are padded with their first entry when fewer than k points
+ * exist (src/Octree.hpp:181-191).  mode SX_KNN_RADIUS seeds each query with
+ * the squared radius d_radius2[q] (device, may under- or over-estimate; the
+ * result stays exact).  Replaces Octree::deviceKnnSearch<k> and
+ * knnSearchKernel<k> (src/Octree.hpp:136-212). */
+int sx_knn(sx_index* index, int64_t q_start, int64_t n_q, int k, int mode, const float* d_radius2,
+           int32_t* d_out, void* stream);
+
+/* Normals of the index points [q_start, q_start+n_q) from their k-neighbour
+ * rows d_nn (as written by sx_knn): eigenvector of the smallest covariance
+ * eigenvalue, unit length, flipped so that (viewpoint - p) . n >= 0.  Output
+ * is 12-B AoS.  Replaces findCovariancesKernel + cusolverDnSsyevjBatched +
+ * normalsFromEigenVectors (src/NormalEstimation.hpp:96-192,235-315).
+ * d_flags (nullable, uint8 per query) receives 1 where the FP64 path ran. */
+int sx_normals(sx_index* index, const int32_t* d_nn, int64_t q_start, int64_t n_q, int k, int precision,
+               float vx, float vy, float vz, float* d_normals, uint8_t* d_flags, void* stream);
+
+/* kNN + normals for [q_start, q_start+n_q), viewpoint (0,0,0): the body of
+ * NormalEstimation::estimateNormals<k>(octree, start, n)
+ * (src/NormalEstimation.hpp:196-335).  d_nn may be NULL (then an internal
+ * buffer holds the neighbour rows). */
+int sx_estimate_normals(sx_index* index, int64_t q_start, int64_t n_q, int k, int knn_mode,
+                        const float* d_radius2, int precision, int32_t* d_nn, float* d_normals, void* stream);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif /* SHAPES_C_H_ */

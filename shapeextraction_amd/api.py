@@ -1,0 +1,325 @@
+"""Python host layer over the C ABI: stage calls, the index object, workloads.
+
+The names follow the reference's pipeline (RadixTree / Octree /
+NormalEstimation, src/main.cu:80-107); the C++ drop-in facade under
+``shapeextraction_amd/cpp`` is the reference-compatible API, this module is
+the harness used by tests and the benchmark.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import (SX_KNN_PLAIN, SX_KNN_RADIUS, SX_PREC_FP32, SX_PREC_FP64, SX_ROOT_REFERENCE,
+                   SX_ROOT_TEXTBOOK, call)
+from .device import DeviceArray, fetch
+
+OTNODE_DTYPE = np.dtype(
+    [
+        ("children", "<i4", (8,)),
+        ("corner", "<f4", (3,)),
+        ("cell_size", "<f4"),
+        ("child_node_mask", "<i4"),
+        ("child_leaf_mask", "<i4"),
+    ]
+)
+assert OTNODE_DTYPE.itemsize == 56
+
+
+def _dev(a, dtype=None) -> DeviceArray:
+    if isinstance(a, DeviceArray):
+        return a
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    return DeviceArray.from_numpy(arr)
+
+
+def _ws(name: str, *args_before_ws, stream=None) -> DeviceArray:
+    """Two-phase workspace query: returns a device buffer of the asked size."""
+    size = C.c_size_t(0)
+    call(name, *args_before_ws, None, C.byref(size), stream)
+    return DeviceArray((max(int(size.value), 16),), np.uint8)
+
+
+# ---------------------------------------------------------------------------
+# Synthetic workloads (generated by the library's host code).
+# ---------------------------------------------------------------------------
+def _xyz(n):
+    return np.empty(n, np.float32), np.empty(n, np.float32), np.empty(n, np.float32)
+
+
+def gen_sphere(seed: int, n: int, sigma: float = 0.002):
+    x, y, z = _xyz(n)
+    call("sx_gen_sphere", seed, n, sigma, x.ctypes.data, y.ctypes.data, z.ctypes.data)
+    return x, y, z
+
+
+def gen_scene(seed: int, n: int, n_prims: int = 200, box: float = 100.0, noise: float = 0.01,
+              outliers: float = 0.01):
+    x, y, z = _xyz(n)
+    call("sx_gen_scene", seed, n, n_prims, box, noise, outliers, x.ctypes.data, y.ctypes.data, z.ctypes.data)
+    return x, y, z
+
+
+def gen_gmm_scene(seed: int, n: int, n_prims: int = 200, n_components: int = 64, box: float = 100.0,
+                  noise: float = 0.01, max_ratio: float = 1e4):
+    x, y, z = _xyz(n)
+    f = C.c_float(0)
+    call("sx_gen_gmm_scene", seed, n, n_prims, n_components, box, noise, max_ratio, x.ctypes.data,
+         y.ctypes.data, z.ctypes.data, C.byref(f))
+    return x, y, z
+
+
+def gen_batch_sizes(seed: int, n_clouds: int, lo: int = 1000, hi: int = 200000) -> np.ndarray:
+    s = np.empty(n_clouds, np.int64)
+    call("sx_gen_batch_sizes", seed, n_clouds, lo, hi, s.ctypes.data)
+    return s
+
+
+def gen_degenerate(seed: int, n: int, structs_per_class: int = 64, offset: float = 1e4, box: float = 100.0):
+    x, y, z = _xyz(n)
+    cls = np.empty(n, np.uint8)
+    d = np.empty((n, 3), np.float32)
+    call("sx_gen_degenerate", seed, n, structs_per_class, offset, box, x.ctypes.data, y.ctypes.data,
+         z.ctypes.data, cls.ctypes.data, d.ctypes.data)
+    return x, y, z, cls, d
+
+
+def fnv64(arr: np.ndarray) -> int:
+    arr = np.ascontiguousarray(arr)
+    return int(_lib.load().sx_fnv64(arr.ctypes.data, arr.nbytes))
+
+
+@dataclass(frozen=True)
+class Workload:
+    name: str
+    n: int
+    k: int
+    code_bits: int
+    seed: int
+    kind: str
+
+    def generate(self):
+        if self.kind == "sphere":
+            return gen_sphere(self.seed, self.n, 0.002)
+        if self.kind == "scene":
+            return gen_scene(self.seed, self.n)
+        if self.kind == "gmm":
+            return gen_gmm_scene(self.seed, self.n)
+        if self.kind == "degenerate":
+            return gen_degenerate(self.seed, self.n)[:3]
+        raise ValueError(self.kind)
+
+
+WORKLOADS = {
+    "sphere100k": Workload("sphere100k", 100_000, 8, 30, 1, "sphere"),
+    "scene8m": Workload("scene8m", 8_388_608, 16, 63, 2, "scene"),
+    "gmm16m": Workload("gmm16m", 16_777_216, 32, 63, 3, "gmm"),
+    "degenerate4m": Workload("degenerate4m", 4_194_304, 24, 63, 5, "degenerate"),
+}
+
+
+# ---------------------------------------------------------------------------
+# Stage calls (one per reference kernel); inputs/outputs are DeviceArrays.
+# ---------------------------------------------------------------------------
+def bbox(x: DeviceArray, y: DeviceArray, z: DeviceArray, stream=None):
+    n = len(x)
+    bounds = DeviceArray((4,), np.float32)
+    bad = DeviceArray.zeros((1,), np.int32)
+    ws = _ws("sx_bbox", x.ptr, y.ptr, z.ptr, n, bounds.ptr, bad.ptr, stream=stream)
+    call("sx_bbox", x.ptr, y.ptr, z.ptr, n, bounds.ptr, bad.ptr, ws.ptr, C.byref(C.c_size_t(ws.nbytes)), stream)
+    return bounds, bad
+
+
+def morton_encode(x, y, z, bounds: DeviceArray, code_bits: int, stream=None) -> DeviceArray:
+    codes = DeviceArray((len(x),), np.uint64)
+    call("sx_morton_encode", x.ptr, y.ptr, z.ptr, len(x), bounds.ptr, code_bits, codes.ptr, stream)
+    return codes
+
+
+def morton_decode(codes: DeviceArray, bounds: DeviceArray, code_bits: int, count: DeviceArray | None = None,
+                  stream=None) -> DeviceArray:
+    pts = DeviceArray((len(codes), 3), np.float32)
+    call("sx_morton_decode", codes.ptr, count.ptr if count else None, len(codes), bounds.ptr, code_bits, pts.ptr,
+         stream)
+    return pts
+
+
+def sort_pairs(keys: DeviceArray, key_bits: int, vals: DeviceArray | None = None, stream=None):
+    n = len(keys)
+    ko = DeviceArray((n,), np.uint64)
+    vo = DeviceArray((n,), np.uint32)
+    vp = vals.ptr if vals is not None else None
+    ws = _ws("sx_sort_pairs", keys.ptr, vp, n, key_bits, ko.ptr, vo.ptr, stream=stream)
+    call("sx_sort_pairs", keys.ptr, vp, n, key_bits, ko.ptr, vo.ptr, ws.ptr, C.byref(C.c_size_t(ws.nbytes)), stream)
+    return ko, vo
+
+
+def unique(sorted_keys: DeviceArray, vals: DeviceArray | None, stream=None):
+    n = len(sorted_keys)
+    uq = DeviceArray((n,), np.uint64)
+    rep = DeviceArray((n,), np.uint32)
+    uos = DeviceArray((n,), np.int32)
+    cnt = DeviceArray.zeros((1,), np.int32)
+    vp = vals.ptr if vals is not None else None
+    ws = _ws("sx_unique", sorted_keys.ptr, vp, n, uq.ptr, rep.ptr, uos.ptr, cnt.ptr, stream=stream)
+    call("sx_unique", sorted_keys.ptr, vp, n, uq.ptr, rep.ptr, uos.ptr, cnt.ptr, ws.ptr,
+         C.byref(C.c_size_t(ws.nbytes)), stream)
+    return uq, rep, uos, cnt
+
+
+def radix_tree(codes: DeviceArray, count: DeviceArray, code_bits: int, root_mode: int = SX_ROOT_TEXTBOOK,
+               stream=None) -> dict:
+    n = len(codes)
+    out = {
+        "prefix_n": DeviceArray((n,), np.uint8),
+        "left_child": DeviceArray((n,), np.int32),
+        "has_leaf_left": DeviceArray((n,), np.uint8),
+        "has_leaf_right": DeviceArray((n,), np.uint8),
+        "parent": DeviceArray((n,), np.int32),
+        "node_end": DeviceArray((n,), np.int32),
+        "leaf_parent": DeviceArray((n,), np.int32),
+    }
+    call("sx_build_radix_tree", codes.ptr, count.ptr, n, code_bits, root_mode, out["prefix_n"].ptr,
+         out["left_child"].ptr, out["has_leaf_left"].ptr, out["has_leaf_right"].ptr, out["parent"].ptr,
+         out["node_end"].ptr, out["leaf_parent"].ptr, stream)
+    return out
+
+
+def octree(codes: DeviceArray, count: DeviceArray, code_bits: int, bounds: DeviceArray, rt: dict,
+           capacity: int | None = None, stream=None):
+    n = len(codes)
+    cap = capacity if capacity is not None else max(16, 3 * n)
+    nodes = DeviceArray((cap,), OTNODE_DTYPE)
+    ocnt = DeviceArray.zeros((1,), np.int32)
+    args = (codes.ptr, count.ptr, n, code_bits, bounds.ptr, rt["prefix_n"].ptr, rt["left_child"].ptr,
+            rt["has_leaf_left"].ptr, rt["has_leaf_right"].ptr, rt["parent"].ptr, nodes.ptr, cap, ocnt.ptr)
+    ws = _ws("sx_build_octree", *args, stream=stream)
+    call("sx_build_octree", *args, ws.ptr, C.byref(C.c_size_t(ws.nbytes)), stream)
+    return nodes, ocnt
+
+
+def check_counts(count: DeviceArray | None, cap: int, oct_count: DeviceArray | None = None, oct_cap: int = 0,
+                 bad: DeviceArray | None = None, stream=None):
+    u = C.c_int64(0)
+    o = C.c_int64(0)
+    call("sx_check_counts", count.ptr if count else None, cap, oct_count.ptr if oct_count else None, oct_cap,
+         bad.ptr if bad else None, C.byref(u), C.byref(o), stream)
+    return u.value, o.value
+
+
+# ---------------------------------------------------------------------------
+# Index object.
+# ---------------------------------------------------------------------------
+class Index:
+    """One cloud's index (RT::RadixTree + OT::Octree state) on the GPU."""
+
+    def __init__(self, capacity: int, code_bits: int = 63, root_mode: int = SX_ROOT_TEXTBOOK,
+                 oct_factor: float = 3.0):
+        h = C.c_void_p()
+        call("sx_index_create", C.byref(h), int(capacity), code_bits, root_mode, oct_factor)
+        self.handle = h.value
+        self.capacity = int(capacity)
+        self.code_bits = code_bits
+        self._lib = _lib.load()
+        self._view = None
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.sx_index_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_bucket(self, bucket: int):
+        call("sx_index_set_bucket", self.handle, bucket)
+
+    def build(self, x, y, z, stream=None) -> "Index":
+        dx, dy, dz = _dev(x, np.float32), _dev(y, np.float32), _dev(z, np.float32)
+        self._inputs = (dx, dy, dz)  # keep alive until the stream has consumed them
+        call("sx_index_build", self.handle, dx.ptr, dy.ptr, dz.ptr, len(dx), stream)
+        self._view = None
+        return self
+
+    def view(self, stream=None) -> _lib.IndexView:
+        v = _lib.IndexView()
+        call("sx_index_get_view", self.handle, C.byref(v), stream)
+        self._view = v
+        return v
+
+    @property
+    def n_unique(self) -> int:
+        return int((self._view or self.view()).n_unique)
+
+    def fetch(self) -> dict:
+        """Copy every index array to the host (tests)."""
+        v = self._view or self.view()
+        U, O, N = int(v.n_unique), int(v.n_oct), int(v.n_input)
+        return {
+            "n_input": N,
+            "n_unique": U,
+            "n_oct": O,
+            "min_coord": float(v.min_coord),
+            "max_coord": float(v.max_coord),
+            "codes": fetch(v.codes, U, np.uint64),
+            "prefix_n": fetch(v.prefix_n, U, np.uint8),
+            "left_child": fetch(v.left_child, U, np.int32),
+            "has_leaf_left": fetch(v.has_leaf_left, U, np.uint8),
+            "has_leaf_right": fetch(v.has_leaf_right, U, np.uint8),
+            "parent": fetch(v.parent, U, np.int32),
+            "oct_nodes": fetch(v.oct_nodes, O, OTNODE_DTYPE),
+            "points": fetch(v.points, 3 * U, np.float32).reshape(U, 3),
+            "sorted_perm": fetch(v.sorted_perm, N, np.uint32),
+            "rep": fetch(v.rep, U, np.uint32),
+            "unique_of_input": fetch(v.unique_of_input, N, np.int32),
+            "bounds": fetch(v.d_bounds, 3, np.float32),
+        }
+
+    def knn(self, k: int, q_start: int = 0, n_q: int | None = None, mode: int = SX_KNN_PLAIN,
+            radius2: DeviceArray | None = None, out: DeviceArray | None = None, stream=None) -> DeviceArray:
+        if n_q is None:
+            n_q = self.n_unique - q_start
+        out = out if out is not None else DeviceArray((n_q, k), np.int32)
+        call("sx_knn", self.handle, q_start, n_q, k, mode, radius2.ptr if radius2 is not None else None, out.ptr,
+             stream)
+        return out
+
+    def normals(self, nn: DeviceArray, k: int, q_start: int = 0, n_q: int | None = None,
+                precision: int = SX_PREC_FP32, viewpoint=(0.0, 0.0, 0.0), flags: bool = False, stream=None):
+        if n_q is None:
+            n_q = self.n_unique - q_start
+        out = DeviceArray((n_q, 3), np.float32)
+        fl = DeviceArray((max(n_q, 1),), np.uint8) if flags else None
+        call("sx_normals", self.handle, nn.ptr, q_start, n_q, k, precision, float(viewpoint[0]),
+             float(viewpoint[1]), float(viewpoint[2]), out.ptr, fl.ptr if fl else None, stream)
+        return (out, fl) if flags else out
+
+    def estimate_normals(self, k: int, q_start: int = 0, n_q: int | None = None, mode: int = SX_KNN_PLAIN,
+                         radius2: DeviceArray | None = None, precision: int = SX_PREC_FP32,
+                         nn: DeviceArray | None = None, out: DeviceArray | None = None, stream=None):
+        if n_q is None:
+            n_q = self.n_unique - q_start
+        out = out if out is not None else DeviceArray((n_q, 3), np.float32)
+        call("sx_estimate_normals", self.handle, q_start, n_q, k, mode,
+             radius2.ptr if radius2 is not None else None, precision, nn.ptr if nn is not None else None, out.ptr,
+             stream)
+        return out
+
+    def knn_reruns(self, reset: bool = True, stream=None) -> int:
+        r = C.c_int64(0)
+        call("sx_index_knn_reruns", self.handle, C.byref(r), 1 if reset else 0, stream)
+        return r.value
+
+
+__all__ = [
+    "Index", "Workload", "WORKLOADS", "OTNODE_DTYPE", "bbox", "morton_encode", "morton_decode", "sort_pairs",
+    "unique", "radix_tree", "octree", "check_counts", "gen_sphere", "gen_scene", "gen_gmm_scene",
+    "gen_batch_sizes", "gen_degenerate", "fnv64", "SX_KNN_PLAIN", "SX_KNN_RADIUS", "SX_PREC_FP32", "SX_PREC_FP64",
+    "SX_ROOT_TEXTBOOK", "SX_ROOT_REFERENCE",
+]

@@ -1,0 +1,322 @@
+// sx_index: one cloud's spatial index, built by chaining every stage on one
+// stream with device-resident counts (no host synchronisation until a caller
+// asks for the view).  Replaces the RT::RadixTree and OT::Octree constructors
+// (src/RadixTree.cu:305-382, src/Octree.cu:213-319), which synchronise after
+// every kernel and read counts through managed memory.
+//
+// HBM layout (U = unique points, O = octree nodes, capacity-sized):
+//   codes_in[N] u64 -> (sort) codes_sorted[N] u64 + perm[N] u32
+//   ucodes[U] u64, rep[U] u32, unique_of_input[N] i32
+//   points12[3U] f32 (API, Octree::u_points), points16[U] float4 (kernels)
+//   RT::Nodes arrays: prefix_n/has_leaf_left/has_leaf_right[U] u8,
+//                     left_child/parent[U] i32
+//   trav[U] TravNode (64 B: child boxes + range/split/parent) for kNN
+//   oct[O] sx_otnode (56 B, Octree::u_nodes)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "sx_internal.hpp"
+
+namespace sx {
+// launchers from the other translation units
+size_t bbox_ws_bytes();
+int bbox_launch(const float* x, const float* y, const float* z, int64_t n, float* bounds, int32_t* d_bad, void* ws,
+                bool clear_counter, hipStream_t st);
+int encode_launch(const float* x, const float* y, const float* z, int64_t n, const float* bounds, int code_bits,
+                  uint64_t* codes, hipStream_t st);
+int decode_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, const float* bounds, int code_bits,
+                  float* pts12, float4* pts16, hipStream_t st);
+size_t sort_ws_bytes(int64_t n);
+int sort_pairs_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n, int key_bits, uint64_t* keys_out,
+                      uint32_t* vals_out, void* ws, hipStream_t st);
+int unique_launch(const uint64_t* sorted, const uint32_t* vals, int64_t n, uint64_t* uniq, uint32_t* rep,
+                  int32_t* uos, int32_t* uoi, int32_t* d_count, void* ws, hipStream_t st);
+int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
+                  uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
+                  int32_t* node_end, int32_t* leaf_parent, TravNode* trav, hipStream_t st);
+int aabb_launch(const float4* pts, const int32_t* d_count, int64_t n_max, const int32_t* leaf_parent, TravNode* trav,
+                unsigned* visits, hipStream_t st);
+size_t octree_ws_bytes(int64_t n_max);
+int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, const float* bounds,
+                  const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
+                  const int32_t* parent, sx_otnode* nodes, int64_t cap, int32_t* d_oct_count, int32_t* leaf_oct,
+                  void* ws, hipStream_t st);
+int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q, int k,
+               const float* radius2, int bucket, int32_t* out, int32_t* reruns, hipStream_t st);
+int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn, int64_t q_start, int64_t n_q, int k,
+                   int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st);
+}  // namespace sx
+
+// counts[] slots
+enum { kCntUnique = 0, kCntOct = 1, kCntBad = 2, kCntReruns = 3 };
+
+struct sx_index {
+    int64_t cap = 0;
+    int64_t oct_cap = 0;
+    int code_bits = 63;
+    int root_mode = SX_ROOT_TEXTBOOK;
+    int bucket = 16;
+    void* mem = nullptr;
+    float* bounds = nullptr;
+    int32_t* counts = nullptr;
+    uint64_t* codes_in = nullptr;
+    uint64_t* codes_sorted = nullptr;
+    uint64_t* ucodes = nullptr;
+    uint32_t* perm = nullptr;
+    uint32_t* rep = nullptr;
+    int32_t* uoi = nullptr;
+    float* pts12 = nullptr;
+    float4* pts16 = nullptr;
+    uint8_t* pn = nullptr;
+    uint8_t* hll = nullptr;
+    uint8_t* hlr = nullptr;
+    int32_t* lc = nullptr;
+    int32_t* parent = nullptr;
+    int32_t* leaf_parent = nullptr;
+    int32_t* leaf_oct = nullptr;
+    sx::TravNode* trav = nullptr;
+    unsigned* visits = nullptr;
+    sx_otnode* oct = nullptr;
+    void* ws_bbox = nullptr;
+    void* ws = nullptr;
+    int32_t* nn_buf = nullptr;
+    int64_t nn_buf_len = 0;
+    // host state
+    int64_t n_input = 0;
+    bool built = false;
+    bool counts_valid = false;
+    int64_t n_unique = 0;
+    int64_t n_oct = 0;
+    float host_bounds[3] = {0, 0, 0};
+};
+
+namespace {
+
+size_t index_layout(sx_index* ix, void* base) {
+    sx::Carver c(base);
+    const size_t n = static_cast<size_t>(ix->cap);
+    ix->bounds = c.take<float>(4);
+    ix->counts = c.take<int32_t>(8);
+    ix->codes_in = c.take<uint64_t>(n);
+    ix->codes_sorted = c.take<uint64_t>(n);
+    ix->ucodes = c.take<uint64_t>(n);
+    ix->perm = c.take<uint32_t>(n);
+    ix->rep = c.take<uint32_t>(n);
+    ix->uoi = c.take<int32_t>(n);
+    ix->pts12 = c.take<float>(3 * n);
+    ix->pts16 = c.take<float4>(n);
+    ix->pn = c.take<uint8_t>(n);
+    ix->hll = c.take<uint8_t>(n);
+    ix->hlr = c.take<uint8_t>(n);
+    ix->lc = c.take<int32_t>(n);
+    ix->parent = c.take<int32_t>(n);
+    ix->leaf_parent = c.take<int32_t>(n);
+    ix->leaf_oct = c.take<int32_t>(n);
+    ix->trav = c.take<sx::TravNode>(n);
+    ix->visits = c.take<unsigned>(n);
+    ix->oct = c.take<sx_otnode>(static_cast<size_t>(ix->oct_cap));
+    ix->ws_bbox = c.take<char>(sx::bbox_ws_bytes());
+    const size_t ws = std::max(sx::sort_ws_bytes(ix->cap), std::max(sx::scan_ws_bytes(ix->cap), sx::octree_ws_bytes(ix->cap)));
+    ix->ws = c.take<char>(ws);
+    return c.off;
+}
+
+int check_index(const sx_index* ix) {
+    if (!ix) return sx::fail(SX_EINVAL, "null index");
+    if (!ix->built) return sx::fail(SX_ESTATE, "index has not been built");
+    return SX_OK;
+}
+
+int check_queries(const sx_index* ix, int64_t q_start, int64_t n_q) {
+    const int64_t limit = ix->counts_valid ? ix->n_unique : ix->n_input;
+    if (q_start < 0 || n_q < 0 || q_start + n_q > limit)
+        return sx::fail(SX_EINVAL, "query range [" + std::to_string(q_start) + ", " + std::to_string(q_start + n_q) +
+                                       ") outside the index points (" + std::to_string(limit) + ")");
+    return SX_OK;
+}
+
+bool valid_k(int k) { return k == 8 || k == 16 || k == 24 || k == 32; }
+
+}  // namespace
+
+using sx::fail;
+
+extern "C" {
+
+int sx_index_create(sx_index** out, int64_t capacity, int code_bits, int root_mode, float oct_factor) {
+    if (!out) return fail(SX_EINVAL, "sx_index_create: null out");
+    *out = nullptr;
+    if (capacity <= 0 || capacity >= (1ll << 31) - 1) return fail(SX_EINVAL, "sx_index_create: capacity must be in [1, 2^31-1)");
+    if (code_bits != 30 && code_bits != 63) return fail(SX_EINVAL, "sx_index_create: code_bits must be 30 or 63");
+    if (root_mode != SX_ROOT_TEXTBOOK && root_mode != SX_ROOT_REFERENCE)
+        return fail(SX_EINVAL, "sx_index_create: bad root_mode");
+    sx_index* ix = new sx_index();
+    ix->cap = capacity;
+    ix->code_bits = code_bits;
+    ix->root_mode = root_mode;
+    const double f = oct_factor > 0 ? oct_factor : 3.0;
+    ix->oct_cap = std::max<int64_t>(16, static_cast<int64_t>(f * static_cast<double>(capacity)));
+    const size_t bytes = index_layout(ix, nullptr);
+    hipError_t e = hipMalloc(&ix->mem, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        delete ix;
+        return fail(SX_ENOMEM, "sx_index_create: hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    index_layout(ix, ix->mem);
+    int rc = sx::fill_u8(ix->bounds, 0, 256, nullptr);                    // bounds + counts
+    if (rc == SX_OK) rc = sx::fill_u8(ix->ws_bbox, 0, sx::bbox_ws_bytes(), nullptr);  // bbox counter
+    if (rc == SX_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(SX_EHIP, "sx_index_create: sync failed");
+    if (rc != SX_OK) {
+        (void)hipFree(ix->mem);
+        delete ix;
+        return rc;
+    }
+    *out = ix;
+    return SX_OK;
+}
+
+int sx_index_destroy(sx_index* ix) {
+    if (!ix) return SX_OK;
+    int rc = SX_OK;
+    if (ix->nn_buf && hipFree(ix->nn_buf) != hipSuccess) rc = fail(SX_EHIP, "sx_index_destroy: hipFree failed");
+    if (ix->mem && hipFree(ix->mem) != hipSuccess) rc = fail(SX_EHIP, "sx_index_destroy: hipFree failed");
+    delete ix;
+    return rc;
+}
+
+int sx_index_set_bucket(sx_index* ix, int bucket) {
+    if (!ix) return fail(SX_EINVAL, "sx_index_set_bucket: null index");
+    if (bucket < 1 || bucket > 256) return fail(SX_EINVAL, "sx_index_set_bucket: bucket must be in [1, 256]");
+    ix->bucket = bucket;
+    return SX_OK;
+}
+
+int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float* d_z, int64_t n, void* stream) {
+    if (!ix) return fail(SX_EINVAL, "sx_index_build: null index");
+    if (n <= 0 || n > ix->cap) return fail(SX_EINVAL, "sx_index_build: n must be in [1, capacity]");
+    if (!d_x || !d_y || !d_z) return fail(SX_EINVAL, "sx_index_build: null coordinates");
+    hipStream_t st = sx::as_stream(stream);
+    ix->built = false;
+    ix->counts_valid = false;
+    const int cb = ix->code_bits;
+    SX_TRY(sx::fill_u8(ix->counts, 0, 32, st));
+    SX_TRY(sx::bbox_launch(d_x, d_y, d_z, n, ix->bounds, ix->counts + kCntBad, ix->ws_bbox, false, st));
+    SX_TRY(sx::encode_launch(d_x, d_y, d_z, n, ix->bounds, cb, ix->codes_in, st));
+    SX_TRY(sx::sort_pairs_launch(ix->codes_in, nullptr, n, cb, ix->codes_sorted, ix->perm, ix->ws, st));
+    SX_TRY(sx::unique_launch(ix->codes_sorted, ix->perm, n, ix->ucodes, ix->rep, nullptr, ix->uoi,
+                             ix->counts + kCntUnique, ix->ws, st));
+    SX_TRY(sx::decode_launch(ix->ucodes, ix->counts + kCntUnique, n, ix->bounds, cb, ix->pts12, ix->pts16, st));
+    if (ix->root_mode == SX_ROOT_TEXTBOOK) {
+        SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, ix->pn, ix->lc, ix->hll,
+                                 ix->hlr, ix->parent, nullptr, ix->leaf_parent, ix->trav, st));
+    } else {
+        SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_REFERENCE, ix->pn, ix->lc,
+                                 ix->hll, ix->hlr, ix->parent, nullptr, nullptr, nullptr, st));
+        SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, nullptr, nullptr,
+                                 nullptr, nullptr, nullptr, nullptr, ix->leaf_parent, ix->trav, st));
+    }
+    SX_TRY(sx::aabb_launch(ix->pts16, ix->counts + kCntUnique, n, ix->leaf_parent, ix->trav, ix->visits, st));
+    SX_TRY(sx::octree_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, ix->bounds, ix->pn, ix->lc, ix->hll, ix->hlr,
+                             ix->parent, ix->oct, ix->oct_cap, ix->counts + kCntOct, ix->leaf_oct, ix->ws, st));
+    ix->n_input = n;
+    ix->built = true;
+    return SX_OK;
+}
+
+int sx_index_get_view(sx_index* ix, sx_index_view* view, void* stream) {
+    SX_TRY(check_index(ix));
+    if (!view) return fail(SX_EINVAL, "sx_index_get_view: null view");
+    int64_t nu = 0, no = 0;
+    SX_TRY(sx_check_counts(ix->counts + kCntUnique, ix->n_input, ix->counts + kCntOct, ix->oct_cap,
+                           ix->counts + kCntBad, &nu, &no, stream));
+    SX_HIP(hipMemcpy(ix->host_bounds, ix->bounds, sizeof(float) * 3, hipMemcpyDeviceToHost));
+    ix->n_unique = nu;
+    ix->n_oct = no;
+    ix->counts_valid = true;
+    view->n_input = ix->n_input;
+    view->n_unique = nu;
+    view->n_oct = no;
+    view->min_coord = ix->host_bounds[0];
+    view->max_coord = ix->host_bounds[1];
+    view->code_bits = ix->code_bits;
+    view->codes = ix->ucodes;
+    view->has_leaf_left = ix->hll;
+    view->has_leaf_right = ix->hlr;
+    view->prefix_n = ix->pn;
+    view->left_child = ix->lc;
+    view->parent = ix->parent;
+    view->oct_nodes = ix->oct;
+    view->points = ix->pts12;
+    view->sorted_perm = ix->perm;
+    view->rep = ix->rep;
+    view->unique_of_input = ix->uoi;
+    view->d_counts = ix->counts;
+    view->d_bounds = ix->bounds;
+    return SX_OK;
+}
+
+int sx_index_knn_reruns(sx_index* ix, int64_t* reruns, int reset, void* stream) {
+    if (!ix || !reruns) return fail(SX_EINVAL, "sx_index_knn_reruns: null argument");
+    int32_t h = 0;
+    hipStream_t st = sx::as_stream(stream);
+    SX_HIP(hipMemcpyAsync(&h, ix->counts + kCntReruns, 4, hipMemcpyDeviceToHost, st));
+    SX_HIP(hipStreamSynchronize(st));
+    *reruns = h;
+    if (reset) SX_TRY(sx::fill_i32(ix->counts + kCntReruns, 0, 1, st));
+    return SX_OK;
+}
+
+int sx_knn(sx_index* ix, int64_t q_start, int64_t n_q, int k, int mode, const float* d_radius2, int32_t* d_out,
+           void* stream) {
+    SX_TRY(check_index(ix));
+    if (!valid_k(k)) return fail(SX_EINVAL, "sx_knn: k must be 8, 16, 24 or 32");
+    if (mode != SX_KNN_PLAIN && mode != SX_KNN_RADIUS) return fail(SX_EINVAL, "sx_knn: bad mode");
+    if (mode == SX_KNN_RADIUS && !d_radius2) return fail(SX_EINVAL, "sx_knn: radius mode needs d_radius2");
+    SX_TRY(check_queries(ix, q_start, n_q));
+    if (n_q > 0 && !d_out) return fail(SX_EINVAL, "sx_knn: null output");
+    return sx::knn_launch(ix->pts16, ix->trav, ix->counts, q_start, n_q, k,
+                          mode == SX_KNN_RADIUS ? d_radius2 : nullptr, ix->bucket, d_out, ix->counts + kCntReruns,
+                          sx::as_stream(stream));
+}
+
+int sx_normals(sx_index* ix, const int32_t* d_nn, int64_t q_start, int64_t n_q, int k, int precision, float vx,
+               float vy, float vz, float* d_normals, uint8_t* d_flags, void* stream) {
+    SX_TRY(check_index(ix));
+    if (!valid_k(k)) return fail(SX_EINVAL, "sx_normals: k must be 8, 16, 24 or 32");
+    SX_TRY(check_queries(ix, q_start, n_q));
+    if (n_q > 0 && (!d_nn || !d_normals)) return fail(SX_EINVAL, "sx_normals: null pointer");
+    return sx::normals_launch(ix->pts16, ix->counts, d_nn, q_start, n_q, k, precision, vx, vy, vz, d_normals, d_flags,
+                              sx::as_stream(stream));
+}
+
+int sx_estimate_normals(sx_index* ix, int64_t q_start, int64_t n_q, int k, int knn_mode, const float* d_radius2,
+                        int precision, int32_t* d_nn, float* d_normals, void* stream) {
+    SX_TRY(check_index(ix));
+    if (!valid_k(k)) return fail(SX_EINVAL, "sx_estimate_normals: k must be 8, 16, 24 or 32");
+    SX_TRY(check_queries(ix, q_start, n_q));
+    if (n_q == 0) return SX_OK;
+    if (!d_normals) return fail(SX_EINVAL, "sx_estimate_normals: null normals");
+    int32_t* nn = d_nn;
+    if (!nn) {
+        const int64_t need = n_q * k;
+        if (need > ix->nn_buf_len) {
+            if (ix->nn_buf) SX_HIP(hipFree(ix->nn_buf));
+            ix->nn_buf = nullptr;
+            ix->nn_buf_len = 0;
+            hipError_t e = hipMalloc(&ix->nn_buf, static_cast<size_t>(need) * sizeof(int32_t));
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                ix->nn_buf = nullptr;
+                return fail(SX_ENOMEM, "sx_estimate_normals: neighbour buffer allocation failed");
+            }
+            ix->nn_buf_len = need;
+        }
+        nn = ix->nn_buf;
+    }
+    SX_TRY(sx_knn(ix, q_start, n_q, k, knn_mode, d_radius2, nn, stream));
+    return sx_normals(ix, nn, q_start, n_q, k, precision, 0.0f, 0.0f, 0.0f, d_normals, nullptr, stream);
+}
+
+}  // extern "C"

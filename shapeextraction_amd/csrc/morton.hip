@@ -1,0 +1,248 @@
+// Cube bounds, Morton encode and decode.
+//
+// bbox:   one streaming pass over x, y, z (12 B/point) with a last-block-done
+//         reduction; replaces the six cub reductions and the host min/max of
+//         RadixTree::encodePoints (src/RadixTree.cu:208-241).
+// encode: 4 points per thread, float4 loads, 20 B/point of HBM traffic;
+//         replaces makeCodes / pointToCode (src/RadixTree.cu:21-39,
+//         src/MortonUtils.cu:7-19).
+// decode: codeToPoint per unique code (src/MortonUtils.cu:21-28,
+//         src/Octree.cu:14-24), written both as the API's 12-byte AoS points
+//         and as the 16-byte float4 copy the traversal kernels load.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "sx_internal.hpp"
+
+namespace sx {
+
+constexpr int kBBoxBlock = 256;
+constexpr int kBBoxMaxBlocks = 1024;
+
+__device__ __forceinline__ void bbox_acc(float v, float& mn, float& mx, int& bad) {
+    if (isfinite(v)) {
+        mn = fminf(mn, v);
+        mx = fmaxf(mx, v);
+    } else {
+        bad += 1;
+    }
+}
+
+__device__ __forceinline__ void block_minmax(float& mn, float& mx, int& bad) {
+    __shared__ float s_mn[kBBoxBlock / kWave], s_mx[kBBoxBlock / kWave];
+    __shared__ int s_bad[kBBoxBlock / kWave];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        mn = fminf(mn, __shfl_xor(mn, o, kWave));
+        mx = fmaxf(mx, __shfl_xor(mx, o, kWave));
+        bad += __shfl_xor(bad, o, kWave);
+    }
+    const int w = static_cast<int>(threadIdx.x) / kWave;
+    if (lane_id() == 0) { s_mn[w] = mn; s_mx[w] = mx; s_bad[w] = bad; }
+    __syncthreads();
+    mn = s_mn[0]; mx = s_mx[0]; bad = s_bad[0];
+#pragma unroll
+    for (int i = 1; i < kBBoxBlock / kWave; ++i) {
+        mn = fminf(mn, s_mn[i]);
+        mx = fmaxf(mx, s_mx[i]);
+        bad += s_bad[i];
+    }
+    __syncthreads();
+}
+
+// partial: [blocks] float2 (min, max); bad_partial: [blocks]; counter: one
+// uint32 that is zero on entry and left zero on exit.
+__global__ void __launch_bounds__(kBBoxBlock) bbox_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                          const float* __restrict__ z, int64_t n, float2* partial,
+                                                          int* bad_partial, unsigned* counter, float* bounds,
+                                                          int32_t* d_bad) {
+    float mn = INFINITY, mx = -INFINITY;
+    int bad = 0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        bbox_acc(x[i], mn, mx, bad);
+        bbox_acc(y[i], mn, mx, bad);
+        bbox_acc(z[i], mn, mx, bad);
+    }
+    block_minmax(mn, mx, bad);
+    __shared__ bool s_last;
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = make_float2(mn, mx);
+        bad_partial[blockIdx.x] = bad;
+        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    mn = INFINITY; mx = -INFINITY; bad = 0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+        float pmn = __hip_atomic_load(&partial[b].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        float pmx = __hip_atomic_load(&partial[b].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int pb = __hip_atomic_load(&bad_partial[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        mn = fminf(mn, pmn);
+        mx = fmaxf(mx, pmx);
+        bad += pb;
+    }
+    block_minmax(mn, mx, bad);
+    if (threadIdx.x == 0) {
+        bounds[0] = mn;
+        bounds[1] = mx;
+        bounds[2] = __fsub_rn(mx, mn);
+        if (d_bad) *d_bad = bad;
+        *counter = 0u;
+    }
+}
+
+inline int bbox_blocks(int64_t n) {
+    int64_t b = (n + kBBoxBlock * 16 - 1) / (kBBoxBlock * 16);
+    if (b < 1) b = 1;
+    if (b > kBBoxMaxBlocks) b = kBBoxMaxBlocks;
+    return static_cast<int>(b);
+}
+
+size_t bbox_ws_bytes() {
+    Carver c(nullptr);
+    c.take<float2>(kBBoxMaxBlocks);
+    c.take<int>(kBBoxMaxBlocks);
+    c.take<unsigned>(1);
+    return c.off;
+}
+
+// The workspace counter must be zero on the first use; the kernel leaves it
+// zero, so a workspace that was filled once can be reused.
+int bbox_launch(const float* x, const float* y, const float* z, int64_t n, float* bounds, int32_t* d_bad, void* ws,
+                bool clear_counter, hipStream_t st) {
+    Carver c(ws);
+    float2* partial = c.take<float2>(kBBoxMaxBlocks);
+    int* badp = c.take<int>(kBBoxMaxBlocks);
+    unsigned* counter = c.take<unsigned>(1);
+    if (clear_counter) SX_TRY(fill_u8(counter, 0, sizeof(unsigned) * 4, st));
+    bbox_kernel<<<bbox_blocks(n), kBBoxBlock, 0, st>>>(x, y, z, n, partial, badp, counter, bounds, d_bad);
+    SX_LAUNCH_CHECK("bbox_kernel");
+    return SX_OK;
+}
+
+__device__ __forceinline__ uint64_t encode_one(float x, float y, float z, float mn, float range, float fscale) {
+    return interleave3(quantize(x, mn, range, fscale), quantize(y, mn, range, fscale), quantize(z, mn, range, fscale));
+}
+
+constexpr int kEncodeBlock = 256;
+
+__global__ void __launch_bounds__(kEncodeBlock) encode_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                              const float* __restrict__ z, int64_t n,
+                                                              const float* __restrict__ bounds, int code_bits,
+                                                              uint64_t* __restrict__ codes) {
+    const float mn = bounds[0], range = bounds[2];
+    const float fscale = static_cast<float>(axis_scale(code_bits));
+    const int64_t i4 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+    if (i4 + 3 < n) {
+        const float4 vx = *reinterpret_cast<const float4*>(x + i4);
+        const float4 vy = *reinterpret_cast<const float4*>(y + i4);
+        const float4 vz = *reinterpret_cast<const float4*>(z + i4);
+        ulonglong2 c01, c23;
+        c01.x = encode_one(vx.x, vy.x, vz.x, mn, range, fscale);
+        c01.y = encode_one(vx.y, vy.y, vz.y, mn, range, fscale);
+        c23.x = encode_one(vx.z, vy.z, vz.z, mn, range, fscale);
+        c23.y = encode_one(vx.w, vy.w, vz.w, mn, range, fscale);
+        reinterpret_cast<ulonglong2*>(codes + i4)[0] = c01;
+        reinterpret_cast<ulonglong2*>(codes + i4)[1] = c23;
+    } else {
+        for (int64_t i = i4; i < n; ++i) codes[i] = encode_one(x[i], y[i], z[i], mn, range, fscale);
+    }
+}
+
+// Scalar variant for unaligned inputs.
+__global__ void __launch_bounds__(kEncodeBlock) encode_kernel_scalar(const float* __restrict__ x,
+                                                                     const float* __restrict__ y,
+                                                                     const float* __restrict__ z, int64_t n,
+                                                                     const float* __restrict__ bounds, int code_bits,
+                                                                     uint64_t* __restrict__ codes) {
+    const float mn = bounds[0], range = bounds[2];
+    const float fscale = static_cast<float>(axis_scale(code_bits));
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) codes[i] = encode_one(x[i], y[i], z[i], mn, range, fscale);
+}
+
+int encode_launch(const float* x, const float* y, const float* z, int64_t n, const float* bounds, int code_bits,
+                  uint64_t* codes, hipStream_t st) {
+    if (n <= 0) return SX_OK;
+    const bool aligned = (reinterpret_cast<uintptr_t>(x) % 16 == 0) && (reinterpret_cast<uintptr_t>(y) % 16 == 0) &&
+                         (reinterpret_cast<uintptr_t>(z) % 16 == 0) && (reinterpret_cast<uintptr_t>(codes) % 16 == 0);
+    if (aligned) {
+        encode_kernel<<<grid_for((n + 3) / 4, kEncodeBlock), kEncodeBlock, 0, st>>>(x, y, z, n, bounds, code_bits,
+                                                                                     codes);
+        SX_LAUNCH_CHECK("encode_kernel");
+    } else {
+        encode_kernel_scalar<<<grid_for(n, kEncodeBlock), kEncodeBlock, 0, st>>>(x, y, z, n, bounds, code_bits,
+                                                                                  codes);
+        SX_LAUNCH_CHECK("encode_kernel_scalar");
+    }
+    return SX_OK;
+}
+
+__global__ void __launch_bounds__(256) decode_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count,
+                                                     int64_t n_max, const float* __restrict__ bounds, int code_bits,
+                                                     float* __restrict__ pts12, float4* __restrict__ pts16) {
+    const int64_t n = scan_count(d_count, n_max);
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float mn = bounds[0], range = bounds[2];
+    const float fscale = static_cast<float>(axis_scale(code_bits));
+    const uint64_t c = codes[i];
+    const float px = dequantize(compact3(c), fscale, range, mn);
+    const float py = dequantize(compact3(c >> 1), fscale, range, mn);
+    const float pz = dequantize(compact3(c >> 2), fscale, range, mn);
+    if (pts12) {
+        pts12[3 * i] = px;
+        pts12[3 * i + 1] = py;
+        pts12[3 * i + 2] = pz;
+    }
+    if (pts16) pts16[i] = make_float4(px, py, pz, 0.0f);
+}
+
+int decode_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, const float* bounds, int code_bits,
+                  float* pts12, float4* pts16, hipStream_t st) {
+    if (n_max <= 0) return SX_OK;
+    decode_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(codes, d_count, n_max, bounds, code_bits, pts12, pts16);
+    SX_LAUNCH_CHECK("decode_kernel");
+    return SX_OK;
+}
+
+inline bool valid_code_bits(int b) { return b == 63 || b == 30; }
+
+}  // namespace sx
+
+using sx::fail;
+
+extern "C" {
+
+int sx_bbox(const float* d_x, const float* d_y, const float* d_z, int64_t n, float* d_bounds, int32_t* d_bad, void* ws,
+            size_t* ws_bytes, void* stream) {
+    if (!ws_bytes) return fail(SX_EINVAL, "sx_bbox: ws_bytes is null");
+    if (!ws) {
+        *ws_bytes = sx::bbox_ws_bytes();
+        return SX_OK;
+    }
+    if (*ws_bytes < sx::bbox_ws_bytes()) return fail(SX_ENOMEM, "sx_bbox: workspace too small");
+    if (n <= 0 || !d_x || !d_y || !d_z || !d_bounds) return fail(SX_EINVAL, "sx_bbox: bad arguments");
+    return sx::bbox_launch(d_x, d_y, d_z, n, d_bounds, d_bad, ws, true, sx::as_stream(stream));
+}
+
+int sx_morton_encode(const float* d_x, const float* d_y, const float* d_z, int64_t n, const float* d_bounds,
+                     int code_bits, uint64_t* d_codes, void* stream) {
+    if (!sx::valid_code_bits(code_bits)) return fail(SX_EINVAL, "sx_morton_encode: code_bits must be 30 or 63");
+    if (n < 0 || (n > 0 && (!d_x || !d_y || !d_z || !d_bounds || !d_codes)))
+        return fail(SX_EINVAL, "sx_morton_encode: bad arguments");
+    return sx::encode_launch(d_x, d_y, d_z, n, d_bounds, code_bits, d_codes, sx::as_stream(stream));
+}
+
+int sx_morton_decode(const uint64_t* d_codes, const int32_t* d_count, int64_t n_max, const float* d_bounds,
+                     int code_bits, float* d_points, void* stream) {
+    if (!sx::valid_code_bits(code_bits)) return fail(SX_EINVAL, "sx_morton_decode: code_bits must be 30 or 63");
+    if (n_max < 0 || (n_max > 0 && (!d_codes || !d_bounds || !d_points)))
+        return fail(SX_EINVAL, "sx_morton_decode: bad arguments");
+    return sx::decode_launch(d_codes, d_count, n_max, d_bounds, code_bits, d_points, nullptr, sx::as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,197 @@
+// Internal definitions shared by the HIP translation units: error macro,
+// launch helpers, the traversal node layout and the device-wide scan.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "sx_device.hpp"
+#include "sx_status.hpp"
+
+#define SX_HIP(call)                                                                            \
+    do {                                                                                        \
+        hipError_t sx_e_ = (call);                                                              \
+        if (sx_e_ != hipSuccess)                                                                \
+            return sx::fail(SX_EHIP, std::string(#call) + " failed: " + hipGetErrorString(sx_e_)); \
+    } while (0)
+
+#define SX_LAUNCH_CHECK(name)                                                                   \
+    do {                                                                                        \
+        hipError_t sx_e_ = hipGetLastError();                                                   \
+        if (sx_e_ != hipSuccess)                                                                \
+            return sx::fail(SX_EHIP, std::string(name) + " launch failed: " + hipGetErrorString(sx_e_)); \
+    } while (0)
+
+#define SX_TRY(expr)                      \
+    do {                                  \
+        int sx_rc_ = (expr);              \
+        if (sx_rc_ != SX_OK) return sx_rc_; \
+    } while (0)
+
+namespace sx {
+
+inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+inline unsigned grid_for(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    return static_cast<unsigned>(g < 1 ? 1 : g);
+}
+
+inline size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
+
+// Carves 256-byte aligned sub-buffers out of one workspace allocation.  With
+// base == nullptr it only measures.
+struct Carver {
+    char* base;
+    size_t off = 0;
+    explicit Carver(void* b) : base(static_cast<char*>(b)) {}
+    template <typename T>
+    T* take(size_t count) {
+        size_t o = off;
+        off = align_up(off + count * sizeof(T));
+        return base ? reinterpret_cast<T*>(base + o) : nullptr;
+    }
+};
+
+// Traversal record of internal radix-tree node i (a BVH node over the sorted
+// unique points): the boxes of its two children plus, in the .w lanes, the
+// leaf range [a, b] it covers, its split gamma (left child covers [a, gamma],
+// right child [gamma+1, b]) and its parent (-1 at the root).  64 bytes.
+struct __align__(16) TravNode {
+    float4 lo_l;  // w: a
+    float4 hi_l;  // w: gamma
+    float4 lo_r;  // w: b
+    float4 hi_r;  // w: parent
+};
+static_assert(sizeof(TravNode) == 64, "TravNode must be 64 bytes");
+
+__device__ __forceinline__ int w_int(float w) { return __float_as_int(w); }
+
+// ---------------------------------------------------------------------------
+// Device-wide scan of int32 values (reduce, then scan of tile sums, then scan
+// within tiles).  In: int operator()(int64_t i) const for i < n.  Out:
+// void operator()(int64_t i, int exclusive, int value) const.  The element
+// count is n_max, or min(*d_n, n_max) when d_n is given (a device count
+// produced by an earlier kernel).  The total is written to *d_total.
+// ---------------------------------------------------------------------------
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ int wave_incl_sum(int v) {
+    const int lane = static_cast<int>(lane_id());
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        int t = __shfl_up(v, o, kWave);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive block scan for kScanBlock threads; returns the block total in *total.
+__device__ __forceinline__ int block_excl_sum(int v, int* total) {
+    __shared__ int wsum[kScanBlock / kWave];
+    const int lane = static_cast<int>(lane_id());
+    const int w = static_cast<int>(threadIdx.x) / kWave;
+    int incl = wave_incl_sum(v);
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int before = 0, all = 0;
+#pragma unroll
+    for (int i = 0; i < kScanBlock / kWave; ++i) {
+        int s = wsum[i];
+        before += (i < w) ? s : 0;
+        all += s;
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - v;
+}
+
+__device__ __forceinline__ int64_t scan_count(const int32_t* d_n, int64_t n_max) {
+    if (!d_n) return n_max;
+    int64_t n = *d_n;
+    return n < 0 ? 0 : (n < n_max ? n : n_max);
+}
+
+template <typename In>
+__global__ void __launch_bounds__(kScanBlock) scan_reduce_kernel(In in, const int32_t* d_n, int64_t n_max,
+                                                                 int* tile_sums) {
+    const int64_t n = scan_count(d_n, n_max);
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + static_cast<int64_t>(threadIdx.x) * kScanItems;
+    int s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        int64_t i = base + j;
+        if (i < n) s += in(i);
+    }
+    int total;
+    block_excl_sum(s, &total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of the tile sums in place, total to *d_total.
+template <int kBlock = kScanBlock>
+__global__ void __launch_bounds__(kBlock) scan_tiles_kernel(int* tile_sums, int n_tiles, int32_t* d_total) {
+    int carry = 0;
+    for (int base = 0; base < n_tiles; base += kBlock) {
+        const int i = base + static_cast<int>(threadIdx.x);
+        const int v = (i < n_tiles) ? tile_sums[i] : 0;
+        int total;
+        const int ex = block_excl_sum(v, &total);
+        if (i < n_tiles) tile_sums[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0 && d_total) *d_total = carry;
+}
+
+template <typename In, typename Out>
+__global__ void __launch_bounds__(kScanBlock) scan_apply_kernel(In in, Out out, const int32_t* d_n, int64_t n_max,
+                                                                const int* tile_prefix) {
+    const int64_t n = scan_count(d_n, n_max);
+    const int64_t tile_begin = static_cast<int64_t>(blockIdx.x) * kScanTile;
+    if (tile_begin >= n) return;
+    const int64_t base = tile_begin + static_cast<int64_t>(threadIdx.x) * kScanItems;
+    int vals[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        int64_t i = base + j;
+        vals[j] = (i < n) ? in(i) : 0;
+        s += vals[j];
+    }
+    int total;
+    int run = block_excl_sum(s, &total) + tile_prefix[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        int64_t i = base + j;
+        if (i < n) out(i, run, vals[j]);
+        run += vals[j];
+    }
+}
+
+inline int64_t scan_tiles(int64_t n_max) { return (n_max + kScanTile - 1) / kScanTile; }
+inline size_t scan_ws_bytes(int64_t n_max) { return align_up(static_cast<size_t>(scan_tiles(n_max) + 1) * sizeof(int)); }
+
+template <typename In, typename Out>
+int device_scan(In in, Out out, const int32_t* d_n, int64_t n_max, int32_t* d_total, void* ws, hipStream_t st) {
+    if (n_max <= 0) return fail(SX_EINVAL, "device_scan: empty range");
+    const int64_t tiles = scan_tiles(n_max);
+    if (tiles > (1ll << 30)) return fail(SX_EINVAL, "device_scan: too many tiles");
+    int* tile_sums = static_cast<int*>(ws);
+    scan_reduce_kernel<In><<<static_cast<unsigned>(tiles), kScanBlock, 0, st>>>(in, d_n, n_max, tile_sums);
+    SX_LAUNCH_CHECK("scan_reduce_kernel");
+    scan_tiles_kernel<kScanBlock><<<1, kScanBlock, 0, st>>>(tile_sums, static_cast<int>(tiles), d_total);
+    SX_LAUNCH_CHECK("scan_tiles_kernel");
+    scan_apply_kernel<In, Out><<<static_cast<unsigned>(tiles), kScanBlock, 0, st>>>(in, out, d_n, n_max, tile_sums);
+    SX_LAUNCH_CHECK("scan_apply_kernel");
+    return SX_OK;
+}
+
+// Host-side launchers implemented in the .hip files.
+int fill_u8(void* dst, uint8_t value, size_t bytes, hipStream_t st);
+int fill_i32(int32_t* dst, int32_t value, int64_t n, hipStream_t st);
+
+}  // namespace sx

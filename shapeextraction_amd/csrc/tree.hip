@@ -1,0 +1,210 @@
+// Duplicate removal, Karras radix tree, bottom-up bounding boxes.
+//
+// unique:  head flags (code differs from its predecessor) + device scan +
+//          compaction in the scan's output step; replaces findDups,
+//          cub InclusiveSum and moveDups (src/RadixTree.cu:71-97,258-303).
+//          The representative of a run is its first element in stable sort
+//          order, i.e. the lowest input index carrying that code.
+// karras:  one thread per internal node (Karras 2012), the same direction,
+//          exponential + binary range search and ceil-step split search as
+//          constructTree (src/RadixTree.cu:99-194); root rule selectable.
+// aabb:    one thread per leaf walks up; at every internal node the second
+//          arriving child (agent-scope acq_rel counter) merges both child
+//          boxes and continues, so each box is written exactly once.
+#include <hip/hip_runtime.h>
+
+#include "sx_internal.hpp"
+
+namespace sx {
+
+struct UniqueIn {
+    const uint64_t* s;
+    __device__ int operator()(int64_t i) const { return (i == 0 || s[i] != s[i - 1]) ? 1 : 0; }
+};
+
+struct UniqueOut {
+    const uint64_t* s;
+    const uint32_t* vals;
+    uint64_t* uniq;
+    uint32_t* rep;
+    int32_t* uos;  // unique id of each sorted position
+    int32_t* uoi;  // unique id of each input index (needs vals)
+    __device__ void operator()(int64_t i, int ex, int v) const {
+        if (v) {
+            uniq[ex] = s[i];
+            if (rep) rep[ex] = vals ? vals[i] : static_cast<uint32_t>(i);
+        }
+        const int u = ex + v - 1;
+        if (uos) uos[i] = u;
+        if (uoi && vals) uoi[vals[i]] = u;
+    }
+};
+
+int unique_launch(const uint64_t* sorted, const uint32_t* vals, int64_t n, uint64_t* uniq, uint32_t* rep,
+                  int32_t* uos, int32_t* uoi, int32_t* d_count, void* ws, hipStream_t st) {
+    return device_scan(UniqueIn{sorted}, UniqueOut{sorted, vals, uniq, rep, uos, uoi}, nullptr, n, d_count, ws, st);
+}
+
+__device__ __forceinline__ int ceil_log2_i64(int64_t l) {
+    return l <= 1 ? 0 : 64 - __clzll(static_cast<long long>(l - 1));
+}
+
+__global__ void __launch_bounds__(256) karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count,
+                                                     int64_t n_max, int code_bits, int root_mode,
+                                                     uint8_t* prefix_n, int32_t* left_child, uint8_t* has_leaf_left,
+                                                     uint8_t* has_leaf_right, int32_t* parent, int32_t* node_end,
+                                                     int32_t* leaf_parent, TravNode* trav) {
+    const int64_t U = scan_count(d_count, n_max);
+    const int64_t N = U - 1;  // internal nodes; leaves are 0..N
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == 0 && parent) parent[0] = -1;
+    if (i >= N) return;
+    const uint64_t ci = codes[i];
+    int d;
+    if (i == 0) {
+        d = 1;
+    } else {
+        const int diff = prefix_len(ci, codes[i + 1], code_bits) - prefix_len(ci, codes[i - 1], code_bits);
+        d = (diff > 0) - (diff < 0);
+    }
+    int64_t l = 0;
+    if (i == 0) {
+        // Reference root: l = N-1 (src/RadixTree.cu:128-131); with one internal
+        // node that length is 0 and the reference's split search is undefined,
+        // so both rules use the full range there.
+        l = (root_mode == SX_ROOT_REFERENCE && N >= 2) ? N - 1 : N;
+    } else {
+        const int dmin = prefix_len(ci, codes[i - d], code_bits);
+        int64_t lmax = 2;
+        while (true) {
+            const int64_t jj = i + lmax * d;
+            if (jj < 0 || jj > N || prefix_len(ci, codes[jj], code_bits) <= dmin) break;
+            lmax *= 2;
+        }
+        const int64_t cutoff = (d == -1) ? i : N - i;
+        for (int64_t t = lmax / 2; t >= 1; t /= 2)
+            if (l + t <= cutoff && prefix_len(ci, codes[i + (l + t) * d], code_bits) > dmin) l += t;
+    }
+    const int64_t j = i + l * d;
+    const int dnode = prefix_len(ci, codes[j], code_bits);
+    int64_t s = 0;
+    const int64_t scut = (d == -1) ? i - 1 : N - i - 1;
+    const int lg = ceil_log2_i64(l);
+    for (int e = 1; e <= lg; ++e) {
+        const int64_t div = 1ll << e;
+        const int64_t t = (l + div - 1) / div;
+        if (s + t <= scut && prefix_len(ci, codes[i + (s + t) * d], code_bits) > dnode) s += t;
+    }
+    const int64_t gamma = i + s * d + (d < 0 ? d : 0);
+    const int64_t a = i < j ? i : j, b = i < j ? j : i;
+    const bool hl = (a == gamma), hr = (b == gamma + 1);
+    if (prefix_n) prefix_n[i] = static_cast<uint8_t>(dnode);
+    if (left_child) left_child[i] = static_cast<int32_t>(gamma);
+    if (has_leaf_left) has_leaf_left[i] = hl ? 1 : 0;
+    if (has_leaf_right) has_leaf_right[i] = hr ? 1 : 0;
+    if (parent) {
+        if (!hl) parent[gamma] = static_cast<int32_t>(i);
+        if (!hr) parent[gamma + 1] = static_cast<int32_t>(i);
+    }
+    if (node_end) node_end[i] = static_cast<int32_t>(j);
+    if (leaf_parent) {
+        if (hl) leaf_parent[gamma] = static_cast<int32_t>(2 * i);
+        if (hr) leaf_parent[gamma + 1] = static_cast<int32_t>(2 * i + 1);
+    }
+    if (trav) {
+        TravNode* t = trav + i;
+        t->lo_l.w = __int_as_float(static_cast<int>(a));
+        t->hi_l.w = __int_as_float(static_cast<int>(gamma));
+        t->lo_r.w = __int_as_float(static_cast<int>(b));
+        if (i == 0) t->hi_r.w = __int_as_float(-1);
+        if (!hl) trav[gamma].hi_r.w = __int_as_float(static_cast<int>(i));
+        if (!hr) trav[gamma + 1].hi_r.w = __int_as_float(static_cast<int>(i));
+    }
+}
+
+int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
+                  uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
+                  int32_t* node_end, int32_t* leaf_parent, TravNode* trav, hipStream_t st) {
+    if (n_max <= 0) return SX_OK;
+    // Nodes that no other node claims (only possible under the reference root
+    // rule) keep parent -1.
+    if (parent) SX_TRY(fill_i32(parent, -1, n_max, st));
+    karras_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(codes, d_count, n_max, code_bits, root_mode, prefix_n,
+                                                        left_child, hll, hlr, parent, node_end, leaf_parent, trav);
+    SX_LAUNCH_CHECK("karras_kernel");
+    return SX_OK;
+}
+
+__global__ void __launch_bounds__(256) aabb_kernel(const float4* __restrict__ pts, const int32_t* d_count,
+                                                   int64_t n_max, const int32_t* __restrict__ leaf_parent,
+                                                   TravNode* trav, unsigned* visits) {
+    const int64_t U = scan_count(d_count, n_max);
+    const int64_t u = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (U < 2 || u >= U) return;
+    const int lp = leaf_parent[u];
+    int p = lp >> 1;
+    int side = lp & 1;
+    const float4 pt = pts[u];
+    float lx = pt.x, ly = pt.y, lz = pt.z, hx = pt.x, hy = pt.y, hz = pt.z;
+    while (true) {
+        TravNode* nd = trav + p;
+        float4* lo = side ? &nd->lo_r : &nd->lo_l;
+        float4* hi = side ? &nd->hi_r : &nd->hi_l;
+        lo->x = lx; lo->y = ly; lo->z = lz;
+        hi->x = hx; hi->y = hy; hi->z = hz;
+        const unsigned prev = __hip_atomic_fetch_add(visits + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == 0) return;
+        const float4* olo = side ? &nd->lo_l : &nd->lo_r;
+        const float4* ohi = side ? &nd->hi_l : &nd->hi_r;
+        lx = fminf(lx, olo->x); ly = fminf(ly, olo->y); lz = fminf(lz, olo->z);
+        hx = fmaxf(hx, ohi->x); hy = fmaxf(hy, ohi->y); hz = fmaxf(hz, ohi->z);
+        const int pp = w_int(nd->hi_r.w);
+        if (pp < 0) return;
+        side = (w_int(trav[pp].hi_l.w) == p) ? 0 : 1;
+        p = pp;
+    }
+}
+
+int aabb_launch(const float4* pts, const int32_t* d_count, int64_t n_max, const int32_t* leaf_parent, TravNode* trav,
+                unsigned* visits, hipStream_t st) {
+    if (n_max < 2) return SX_OK;
+    SX_TRY(fill_u8(visits, 0, static_cast<size_t>(n_max) * sizeof(unsigned), st));
+    aabb_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(pts, d_count, n_max, leaf_parent, trav, visits);
+    SX_LAUNCH_CHECK("aabb_kernel");
+    return SX_OK;
+}
+
+}  // namespace sx
+
+using sx::fail;
+
+extern "C" {
+
+int sx_unique(const uint64_t* d_sorted, const uint32_t* d_vals, int64_t n, uint64_t* d_unique, uint32_t* d_rep,
+              int32_t* d_unique_of_sorted, int32_t* d_count, void* ws, size_t* ws_bytes, void* stream) {
+    if (!ws_bytes) return fail(SX_EINVAL, "sx_unique: ws_bytes is null");
+    if (n <= 0) return fail(SX_EINVAL, "sx_unique: n must be positive");
+    const size_t need = sx::scan_ws_bytes(n);
+    if (!ws) {
+        *ws_bytes = need;
+        return SX_OK;
+    }
+    if (*ws_bytes < need) return fail(SX_ENOMEM, "sx_unique: workspace too small");
+    if (!d_sorted || !d_unique || !d_count) return fail(SX_EINVAL, "sx_unique: null pointer");
+    return sx::unique_launch(d_sorted, d_vals, n, d_unique, d_rep, d_unique_of_sorted, nullptr, d_count, ws,
+                             sx::as_stream(stream));
+}
+
+int sx_build_radix_tree(const uint64_t* d_codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
+                        uint8_t* d_prefix_n, int32_t* d_left_child, uint8_t* d_has_leaf_left,
+                        uint8_t* d_has_leaf_right, int32_t* d_parent, int32_t* d_node_end, int32_t* d_leaf_parent,
+                        void* stream) {
+    if (code_bits != 30 && code_bits != 63) return fail(SX_EINVAL, "sx_build_radix_tree: code_bits must be 30 or 63");
+    if (root_mode != SX_ROOT_TEXTBOOK && root_mode != SX_ROOT_REFERENCE)
+        return fail(SX_EINVAL, "sx_build_radix_tree: bad root_mode");
+    if (n_max <= 0 || !d_codes) return fail(SX_EINVAL, "sx_build_radix_tree: bad arguments");
+    return sx::karras_launch(d_codes, d_count, n_max, code_bits, root_mode, d_prefix_n, d_left_child, d_has_leaf_left,
+                             d_has_leaf_right, d_parent, d_node_end, d_leaf_parent, nullptr, sx::as_stream(stream));
+}
+
+}  // extern "C"
