@@ -50,6 +50,7 @@ def cpu_baseline(workload_n: int, k: int) -> dict:
     from oracle import oracle as O
 
     import shapeextraction_amd as sx
+    from shapeextraction_amd import learned
 
     n = int(os.environ.get("SX_CPU_SAMPLE", 1_048_576))
     x, y, z = sx.gen_scene(2, n)
@@ -75,6 +76,8 @@ def main() -> int:
     ap.add_argument("--k", type=int, default=16)
     ap.add_argument("--n", type=int, default=8_388_608)
     ap.add_argument("--bucket", type=int, default=16)
+    ap.add_argument("--knn", choices=["plain", "learned", "both"], default="both",
+                    help="learned/both use shapeextraction_amd/models/radius_k{k}.bin (trained here if missing)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -97,7 +100,6 @@ def main() -> int:
     dx, dy, dz = (sx.DeviceArray.from_numpy(a) for a in (x, y, z))
     ix = sx.Index(n, 63)
     ix.set_bucket(args.bucket)
-    lib = sx.load()
 
     # first build to learn U (the only host read of a count; untimed)
     ix.build(dx, dy, dz, stream=st.handle)
@@ -106,46 +108,73 @@ def main() -> int:
     log(f"U={U} octree nodes={O_nodes} r={O_nodes / U:.3f}")
     nn = sx.DeviceArray((U, k), np.int32)
     normals = sx.DeviceArray((U, 3), np.float32)
-    ev = [[sx.Event() for _ in range(4)] for _ in range(args.steps)]
+    feats = sx.DeviceArray((U, 8), np.float32)
+    radius2 = sx.DeviceArray((U,), np.float32)
 
-    def step(events=None):
-        if events:
-            events[0].record(st.handle)
+    model = None
+    model_path = learned.default_model_path(k)
+    if args.knn in ("learned", "both"):
+        if not model_path.exists():
+            log(f"no trained radius model at {model_path}; training one on this GPU")
+            model = learned.train(k, log=log)
+        else:
+            model = learned.RadiusModel.load(model_path)
+
+    def step(mode, events=None):
+        """One full pass; events (5) bracket build | features+MLP | kNN | normals."""
+        rec = (lambda i: events[i].record(st.handle)) if events else (lambda i: None)
+        rec(0)
         sx._lib.call("sx_index_build", ix.handle, dx.ptr, dy.ptr, dz.ptr, n, st.handle)
-        if events:
-            events[1].record(st.handle)
-        sx._lib.call("sx_knn", ix.handle, 0, U, k, sx.SX_KNN_PLAIN, None, nn.ptr, st.handle)
-        if events:
-            events[2].record(st.handle)
+        rec(1)
+        if mode == "learned":
+            sx._lib.call("sx_radius_features", ix.handle, 0, U, feats.ptr, st.handle)
+            model.predict_radius2(feats, U, radius2, stream=st.handle)
+        rec(2)
+        sx._lib.call("sx_knn", ix.handle, 0, U, k, sx.SX_KNN_RADIUS if mode == "learned" else sx.SX_KNN_PLAIN,
+                     radius2.ptr if mode == "learned" else None, nn.ptr, st.handle)
+        rec(3)
         sx._lib.call("sx_normals", ix.handle, nn.ptr, 0, U, k, sx.SX_PREC_FP32, 0.0, 0.0, 0.0, normals.ptr, None,
                      st.handle)
-        if events:
-            events[3].record(st.handle)
+        rec(4)
 
-    for _ in range(args.warmup):
-        step()
-    st.sync()
-    sx.synchronize()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        step(ev[s])
-    st.sync()
-    sx.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    build_ms = np.array([e[0].elapsed_ms(e[1]) for e in ev])
-    knn_ms = np.array([e[1].elapsed_ms(e[2]) for e in ev])
-    nrm_ms = np.array([e[2].elapsed_ms(e[3]) for e in ev])
-    step_ms = np.array([e[0].elapsed_ms(e[3]) for e in ev])
-    # sanity: counts unchanged, outputs finite
-    v2 = ix.view(stream=st.handle)
-    assert int(v2.n_unique) == U and int(v2.n_oct) == O_nodes
-    nrm_host = normals.numpy()
-    assert np.isfinite(nrm_host).all()
+    def measure(mode):
+        ev = [[sx.Event() for _ in range(5)] for _ in range(args.steps)]
+        for _ in range(args.warmup):
+            step(mode)
+        st.sync()
+        ix.knn_reruns(reset=True, stream=st.handle)
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            step(mode, ev[s])
+        st.sync()
+        t1 = time.perf_counter()
+        el = lambda i, j: np.array([e[i].elapsed_ms(e[j]) for e in ev])  # noqa: E731
+        v2 = ix.view(stream=st.handle)
+        assert int(v2.n_unique) == U and int(v2.n_oct) == O_nodes
+        res = {
+            "ms_per_step": (t1 - t0) * 1000.0 / args.steps,
+            "build_ms": el(0, 1), "radius_ms": el(1, 2), "knn_ms": el(2, 3), "normals_ms": el(3, 4),
+            "step_ms": el(0, 4),
+            "reruns": ix.knn_reruns(reset=True, stream=st.handle) / max(args.steps, 1),
+            "nn": nn.numpy(), "normals": normals.numpy(),
+        }
+        assert np.isfinite(res["normals"]).all()
+        log(f"{mode}: {res['ms_per_step']:.2f} ms/step (build {np.median(res['build_ms']):.2f}, radius "
+            f"{np.median(res['radius_ms']):.2f}, knn {np.median(res['knn_ms']):.2f}, normals "
+            f"{np.median(res['normals_ms']):.2f}), reruns/step {res['reruns']:.0f}")
+        return res
 
+    modes = ["plain"] + (["learned"] if model is not None else [])
+    results = {m: measure(m) for m in modes}
+    if "learned" in results:
+        assert np.array_equal(results["learned"]["nn"], results["plain"]["nn"]), "learned kNN differs from plain"
+        assert np.array_equal(results["learned"]["normals"], results["plain"]["normals"])
+    head = "learned" if "learned" in results else "plain"
+    R = results[head]
+    ms_per_step = R["ms_per_step"]
     value = n / (ms_per_step / 1000.0)
     kb = knn_bytes_per_query(k) * U
-    knn_avg = float(knn_ms.mean())
+    knn_avg = float(R["knn_ms"].mean())
     achieved = kb / (knn_avg / 1000.0) / 1e9
     traffic = None
     tf = ROOT / "profiles" / "knn_scene8m_traffic.json"
@@ -153,6 +182,18 @@ def main() -> int:
         traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
     r = O_nodes / U
     pipe_bytes = pipeline_bytes_per_point(8, 8, k, r) * n
+
+    def stage(res):
+        return {
+            "ms_per_step": res["ms_per_step"],
+            "points_per_s": n / (res["ms_per_step"] / 1000.0),
+            "index_build_ms": float(np.median(res["build_ms"])),
+            "radius_ms": float(np.median(res["radius_ms"])),
+            "knn_ms": float(np.median(res["knn_ms"])),
+            "normals_ms": float(np.median(res["normals_ms"])),
+            "knn_reruns_per_step": res["reruns"],
+        }
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -170,20 +211,17 @@ def main() -> int:
             "workload": f"scene8m: {n} float32 points, 200 random primitives (planes/spheres/cylinders, "
                         f"sizes 1-20) in [0,100]^3, noise 0.01, 1% outliers, seed 2; 63-bit Morton codes; "
                         f"k={k}; FP32 normals",
-            "knn": "plain",
+            "knn": head,
             "bucket": args.bucket,
             "n_unique": U,
             "n_octree_nodes": O_nodes,
             "device": dev,
         },
-        "stage_ms": {
-            "index_build": float(np.median(build_ms)),
-            "knn": float(np.median(knn_ms)),
-            "normals": float(np.median(nrm_ms)),
-            "step_event_median": float(np.median(step_ms)),
-        },
+        "modes": {m: stage(results[m]) for m in results},
+        "learned_vs_plain": (results["plain"]["ms_per_step"] / results["learned"]["ms_per_step"]
+                             if "learned" in results else None),
         "roofline": {
-            "kernel": "knn_kernel<16>",
+            "kernel": f"knn_kernel<{k}>",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

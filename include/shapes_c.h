@@ -233,6 +233,12 @@ int sx_index_knn_reruns(sx_index* index, int64_t* reruns, int reset, void* strea
 int sx_knn(sx_index* index, int64_t q_start, int64_t n_q, int k, int mode, const float* d_radius2,
            int32_t* d_out, void* stream);
 
+/* Exact k nearest index points of n_q arbitrary query points (12-B AoS xyz,
+ * device); rows as in sx_knn.  The query API the reference's host
+ * Octree::knnSearch<k> aimed at (src/Octree.hpp:214-242, whose call passes
+ * its arguments in the wrong order at :232). */
+int sx_knn_points(sx_index* index, const float* d_queries, int64_t n_q, int k, int32_t* d_out, void* stream);
+
 /* Normals of the index points [q_start, q_start+n_q) from their k-neighbour
  * rows d_nn (as written by sx_knn): eigenvector of the smallest covariance
  * eigenvalue, unit length, flipped so that (viewpoint - p) . n >= 0.  Output
@@ -248,6 +254,38 @@ int sx_normals(sx_index* index, const int32_t* d_nn, int64_t q_start, int64_t n_
  * buffer holds the neighbour rows). */
 int sx_estimate_normals(sx_index* index, int64_t q_start, int64_t n_q, int k, int knn_mode,
                         const float* d_radius2, int precision, int32_t* d_nn, float* d_normals, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Learned search radius (no reference counterpart: the reference search has  */
+/* no radius, src/Octree.hpp:152-155).  A 8-64-64-1 MLP, BF16 weights, FP32   */
+/* accumulation on the matrix cores; trained on exact k-th neighbour          */
+/* distances from sx_knn.  Parameters are a flat FP32 vector: W1[64][8], b1,  */
+/* W2[64][64] (out, in), b2, w3[64], b3 (sx_mlp_num_params() floats).         */
+/* ------------------------------------------------------------------------ */
+typedef struct sx_mlp sx_mlp;
+/* max_batch bounds the sample count of sx_mlp_backward / sx_mlp_adam_step. */
+int sx_mlp_create(sx_mlp** out, int64_t max_batch);
+int sx_mlp_destroy(sx_mlp* mlp);
+int sx_mlp_num_params(void);
+int sx_mlp_set_params(sx_mlp* mlp, const float* h_params, void* stream);   /* resets Adam state */
+int sx_mlp_get_params(sx_mlp* mlp, float* h_params, void* stream);
+int sx_mlp_init(sx_mlp* mlp, uint64_t seed, void* stream);                 /* Glorot-uniform */
+/* Features are normalised as (f - mu) / sigma; sx_mlp_predict_radius2 returns
+ * exp(y + margin). */
+int sx_mlp_set_normalizer(sx_mlp* mlp, const float* h_mu, const float* h_sigma, float margin, void* stream);
+/* Raw prediction y (natural log of the squared k-th neighbour distance). */
+int sx_mlp_forward(sx_mlp* mlp, const float* d_feats, int64_t n, float* d_out, void* stream);
+int sx_mlp_predict_radius2(sx_mlp* mlp, const float* d_feats, int64_t n, float* d_radius2, void* stream);
+/* Gradient of mean((y - target)^2) into d_grad (sx_mlp_num_params() floats,
+ * device) and the summed squared error into *d_loss (device, nullable). */
+int sx_mlp_backward(sx_mlp* mlp, const float* d_feats, const float* d_targets, int64_t n, float* d_grad,
+                    float* d_loss, void* stream);
+/* One Adam step (betas 0.9/0.999, eps 1e-8, decoupled weight decay wd). */
+int sx_mlp_adam_step(sx_mlp* mlp, const float* d_feats, const float* d_targets, int64_t n, float lr, float wd,
+                     float* d_loss, void* stream);
+/* The 8 octree occupancy/depth features of index points [q_start, q_start+n_q)
+ * into d_feats (n_q x 8 floats, 16-byte aligned). */
+int sx_radius_features(sx_index* index, int64_t q_start, int64_t n_q, float* d_feats, void* stream);
 
 #ifdef __cplusplus
 }  /* extern "C" */

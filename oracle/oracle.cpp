@@ -797,3 +797,110 @@ int64_t orc_pipeline(const float* x, const float* y, const float* z, int64_t n, 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- learned radius MLP
+// y = w3 . relu(W2 relu(W1 x + b1) + b2) + b3 with x = (f - mu) / sigma,
+// parameters flattened as W1[64][8], b1[64], W2[64][64] (out, in), b2[64],
+// w3[64], b3.  FP64 accumulation.  With emulate = 1 the operands of every
+// matrix product are rounded to bf16 at the points where the GPU kernels
+// round them (inputs, weights, hidden activations, dz2, dz1); with emulate = 0
+// everything stays FP64 (for finite-difference checks of the formulas).
+namespace {
+constexpr int kMI = 8, kMH = 64;
+constexpr int kPW1 = 0, kPB1 = kPW1 + kMH * kMI, kPW2 = kPB1 + kMH, kPB2 = kPW2 + kMH * kMH, kPW3 = kPB2 + kMH,
+              kPB3 = kPW3 + kMH, kPN = kPB3 + 1;
+
+inline double to_bf16(double v) {
+    float f = static_cast<float>(v);
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+struct MlpSample {
+    double x[kMI], z1[kMH], h1[kMH], z2[kMH], h2[kMH], y;
+};
+
+void mlp_fwd_one(const float* P, const float* f, const float* mu, const float* sigma, int emu, MlpSample& s) {
+    auto R = [&](double v) { return emu ? to_bf16(v) : v; };
+    for (int i = 0; i < kMI; ++i) {
+        if (emu) {
+            float inv = 1.0f / sigma[i];
+            float xn = (f[i] - mu[i]) * inv;
+            s.x[i] = to_bf16(xn);
+        } else {
+            s.x[i] = (static_cast<double>(f[i]) - mu[i]) / sigma[i];
+        }
+    }
+    for (int o = 0; o < kMH; ++o) {
+        double a = P[kPB1 + o];
+        for (int i = 0; i < kMI; ++i) a += R(P[kPW1 + o * kMI + i]) * s.x[i];
+        s.z1[o] = a;
+        s.h1[o] = R(a > 0 ? a : 0.0);
+    }
+    for (int o = 0; o < kMH; ++o) {
+        double a = P[kPB2 + o];
+        for (int i = 0; i < kMH; ++i) a += R(P[kPW2 + o * kMH + i]) * s.h1[i];
+        s.z2[o] = a;
+        s.h2[o] = R(a > 0 ? a : 0.0);
+    }
+    double y = P[kPB3];
+    for (int i = 0; i < kMH; ++i) y += R(P[kPW3 + i]) * s.h2[i];
+    s.y = y;
+}
+}  // namespace
+
+extern "C" {
+
+int orc_mlp_num_params(void) { return kPN; }
+
+void orc_mlp_forward(const float* P, const float* feats, int64_t n, const float* mu, const float* sigma, int emulate,
+                     double* y) {
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < n; ++q) {
+        MlpSample s;
+        mlp_fwd_one(P, feats + kMI * q, mu, sigma, emulate, s);
+        y[q] = s.y;
+    }
+}
+
+// Gradient of mean((y - t)^2) over n samples; returns the loss.
+double orc_mlp_grad(const float* P, const float* feats, const float* targets, int64_t n, const float* mu,
+                    const float* sigma, int emulate, double* grad) {
+    auto R = [&](double v) { return emulate ? to_bf16(v) : v; };
+    std::vector<double> g(kPN, 0.0);
+    double loss = 0;
+    for (int64_t q = 0; q < n; ++q) {
+        MlpSample s;
+        mlp_fwd_one(P, feats + kMI * q, mu, sigma, emulate, s);
+        double e = s.y - targets[q];
+        loss += e * e;
+        double dy = 2.0 * e / static_cast<double>(n);
+        g[kPB3] += dy;
+        double dz2[kMH], dz1[kMH];
+        for (int o = 0; o < kMH; ++o) {
+            g[kPW3 + o] += dy * s.h2[o];
+            double d = s.z2[o] > 0 ? R(P[kPW3 + o]) * dy : 0.0;
+            g[kPB2 + o] += d;
+            dz2[o] = R(d);
+        }
+        for (int o = 0; o < kMH; ++o)
+            for (int i = 0; i < kMH; ++i) g[kPW2 + o * kMH + i] += dz2[o] * s.h1[i];
+        for (int i = 0; i < kMH; ++i) {
+            double a = 0;
+            for (int o = 0; o < kMH; ++o) a += R(P[kPW2 + o * kMH + i]) * dz2[o];
+            double d = s.z1[i] > 0 ? a : 0.0;
+            g[kPB1 + i] += d;
+            dz1[i] = R(d);
+        }
+        for (int o = 0; o < kMH; ++o)
+            for (int i = 0; i < kMI; ++i) g[kPW1 + o * kMI + i] += dz1[o] * s.x[i];
+    }
+    for (int j = 0; j < kPN; ++j) grad[j] = g[j];
+    return loss / static_cast<double>(n);
+}
+
+}  // extern "C"

@@ -60,6 +60,9 @@ def lib() -> C.CDLL:
             "orc_jacobi3": (None, [P, P, P]),
             "orc_normals_ref_f32": (None, [P, P, I64, I64, I, F, F, F, P]),
             "orc_pipeline": (I64, [P, P, P, I64, I, I, I, P, P]),
+            "orc_mlp_num_params": (I, []),
+            "orc_mlp_forward": (None, [P, P, I64, P, P, I, P]),
+            "orc_mlp_grad": (C.c_double, [P, P, P, I64, P, P, I, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -237,3 +240,22 @@ def angular_error_deg(n_test, n_ref) -> np.ndarray:
     b = b / np.linalg.norm(b, axis=1, keepdims=True)
     c = np.clip(np.abs(np.sum(a * b, axis=1)), 0.0, 1.0)
     return np.degrees(np.arccos(c))
+
+
+def mlp_forward(params, feats, mu, sigma, emulate: bool = True) -> np.ndarray:
+    P = _c(params, np.float32)
+    f = _c(feats, np.float32).reshape(-1, 8)
+    mu, sigma = _c(mu, np.float32), _c(sigma, np.float32)
+    y = np.empty(len(f), np.float64)
+    lib().orc_mlp_forward(_p(P), _p(f), len(f), _p(mu), _p(sigma), int(emulate), _p(y))
+    return y
+
+
+def mlp_grad(params, feats, targets, mu, sigma, emulate: bool = True):
+    P = _c(params, np.float32)
+    f = _c(feats, np.float32).reshape(-1, 8)
+    t = _c(targets, np.float32)
+    mu, sigma = _c(mu, np.float32), _c(sigma, np.float32)
+    g = np.empty(int(lib().orc_mlp_num_params()), np.float64)
+    loss = lib().orc_mlp_grad(_p(P), _p(f), _p(t), len(f), _p(mu), _p(sigma), int(emulate), _p(g))
+    return g, float(loss)

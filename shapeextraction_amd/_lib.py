@@ -122,8 +122,21 @@ SIGNATURES = {
     "sx_index_set_bucket": (I, [P, I]),
     "sx_index_knn_reruns": (I, [P, C.POINTER(C.c_int64), I, P]),
     "sx_knn": (I, [P, I64, I64, I, I, P, P, P]),
+    "sx_knn_points": (I, [P, P, I64, I, P, P]),
     "sx_normals": (I, [P, P, I64, I64, I, I, F, F, F, P, P, P]),
     "sx_estimate_normals": (I, [P, I64, I64, I, I, P, I, P, P, P]),
+    "sx_mlp_create": (I, [C.POINTER(C.c_void_p), I64]),
+    "sx_mlp_destroy": (I, [P]),
+    "sx_mlp_num_params": (I, []),
+    "sx_mlp_set_params": (I, [P, P, P]),
+    "sx_mlp_get_params": (I, [P, P, P]),
+    "sx_mlp_init": (I, [P, U64, P]),
+    "sx_mlp_set_normalizer": (I, [P, P, P, F, P]),
+    "sx_mlp_forward": (I, [P, P, I64, P, P]),
+    "sx_mlp_predict_radius2": (I, [P, P, I64, P, P]),
+    "sx_mlp_backward": (I, [P, P, P, I64, P, P, P]),
+    "sx_mlp_adam_step": (I, [P, P, P, I64, F, F, P, P]),
+    "sx_radius_features": (I, [P, I64, I64, P, P]),
 }
 
 _lib = None

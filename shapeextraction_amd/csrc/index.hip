@@ -35,15 +35,17 @@ int unique_launch(const uint64_t* sorted, const uint32_t* vals, int64_t n, uint6
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
                   int32_t* node_end, int32_t* leaf_parent, TravNode* trav, hipStream_t st);
-int aabb_launch(const float4* pts, const int32_t* d_count, int64_t n_max, const int32_t* leaf_parent, TravNode* trav,
-                unsigned* visits, hipStream_t st);
+size_t box_ws_bytes(int64_t n_max);
+int child_boxes_launch(const float4* pts, const int32_t* d_count, int64_t n_max, TravNode* trav, void* ws,
+                       hipStream_t st);
 size_t octree_ws_bytes(int64_t n_max);
 int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, const float* bounds,
                   const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
                   const int32_t* parent, sx_otnode* nodes, int64_t cap, int32_t* d_oct_count, int32_t* leaf_oct,
                   void* ws, hipStream_t st);
-int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q, int k,
-               const float* radius2, int bucket, int32_t* out, int32_t* reruns, hipStream_t st);
+int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
+               const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
+               hipStream_t st);
 int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn, int64_t q_start, int64_t n_q, int k,
                    int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st);
 }  // namespace sx
@@ -76,7 +78,6 @@ struct sx_index {
     int32_t* leaf_parent = nullptr;
     int32_t* leaf_oct = nullptr;
     sx::TravNode* trav = nullptr;
-    unsigned* visits = nullptr;
     sx_otnode* oct = nullptr;
     void* ws_bbox = nullptr;
     void* ws = nullptr;
@@ -114,10 +115,10 @@ size_t index_layout(sx_index* ix, void* base) {
     ix->leaf_parent = c.take<int32_t>(n);
     ix->leaf_oct = c.take<int32_t>(n);
     ix->trav = c.take<sx::TravNode>(n);
-    ix->visits = c.take<unsigned>(n);
     ix->oct = c.take<sx_otnode>(static_cast<size_t>(ix->oct_cap));
     ix->ws_bbox = c.take<char>(sx::bbox_ws_bytes());
-    const size_t ws = std::max(sx::sort_ws_bytes(ix->cap), std::max(sx::scan_ws_bytes(ix->cap), sx::octree_ws_bytes(ix->cap)));
+    const size_t ws = std::max({sx::sort_ws_bytes(ix->cap), sx::scan_ws_bytes(ix->cap), sx::octree_ws_bytes(ix->cap),
+                                sx::box_ws_bytes(ix->cap)});
     ix->ws = c.take<char>(ws);
     return c.off;
 }
@@ -139,6 +140,25 @@ int check_queries(const sx_index* ix, int64_t q_start, int64_t n_q) {
 bool valid_k(int k) { return k == 8 || k == 16 || k == 24 || k == 32; }
 
 }  // namespace
+
+namespace sx {
+// Arrays the learned-radius feature kernel reads (mlp.hip).
+int index_feature_arrays(sx_index* ix, const TravNode** trav, const int32_t** leaf_parent, const uint64_t** codes,
+                         const int32_t** leaf_oct, const sx_otnode** oct, const float** bounds,
+                         const int32_t** counts, int* code_bits, int64_t* limit) {
+    SX_TRY(check_index(ix));
+    *trav = ix->trav;
+    *leaf_parent = ix->leaf_parent;
+    *codes = ix->ucodes;
+    *leaf_oct = ix->leaf_oct;
+    *oct = ix->oct;
+    *bounds = ix->bounds;
+    *counts = ix->counts;
+    *code_bits = ix->code_bits;
+    *limit = ix->counts_valid ? ix->n_unique : ix->n_input;
+    return SX_OK;
+}
+}  // namespace sx
 
 using sx::fail;
 
@@ -217,7 +237,7 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, nullptr, nullptr,
                                  nullptr, nullptr, nullptr, nullptr, ix->leaf_parent, ix->trav, st));
     }
-    SX_TRY(sx::aabb_launch(ix->pts16, ix->counts + kCntUnique, n, ix->leaf_parent, ix->trav, ix->visits, st));
+    SX_TRY(sx::child_boxes_launch(ix->pts16, ix->counts + kCntUnique, n, ix->trav, ix->ws, st));
     SX_TRY(sx::octree_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, ix->bounds, ix->pn, ix->lc, ix->hll, ix->hlr,
                              ix->parent, ix->oct, ix->oct_cap, ix->counts + kCntOct, ix->leaf_oct, ix->ws, st));
     ix->n_input = n;
@@ -276,8 +296,16 @@ int sx_knn(sx_index* ix, int64_t q_start, int64_t n_q, int k, int mode, const fl
     if (mode == SX_KNN_RADIUS && !d_radius2) return fail(SX_EINVAL, "sx_knn: radius mode needs d_radius2");
     SX_TRY(check_queries(ix, q_start, n_q));
     if (n_q > 0 && !d_out) return fail(SX_EINVAL, "sx_knn: null output");
-    return sx::knn_launch(ix->pts16, ix->trav, ix->counts, q_start, n_q, k,
+    return sx::knn_launch(ix->pts16, ix->trav, ix->counts, q_start, n_q, nullptr, k,
                           mode == SX_KNN_RADIUS ? d_radius2 : nullptr, ix->bucket, d_out, ix->counts + kCntReruns,
+                          sx::as_stream(stream));
+}
+
+int sx_knn_points(sx_index* ix, const float* d_queries, int64_t n_q, int k, int32_t* d_out, void* stream) {
+    SX_TRY(check_index(ix));
+    if (!valid_k(k)) return fail(SX_EINVAL, "sx_knn_points: k must be 8, 16, 24 or 32");
+    if (n_q < 0 || (n_q > 0 && (!d_queries || !d_out))) return fail(SX_EINVAL, "sx_knn_points: bad arguments");
+    return sx::knn_launch(ix->pts16, ix->trav, ix->counts, 0, n_q, d_queries, k, nullptr, ix->bucket, d_out, nullptr,
                           sx::as_stream(stream));
 }
 

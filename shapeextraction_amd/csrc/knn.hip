@@ -63,93 +63,105 @@ __device__ __forceinline__ void scan_range(const float4* __restrict__ pts, int l
     }
 }
 
+// One query's exact search.  State machine with one record load per step:
+//   down at `node`: pick the nearer child; a bucket child is scanned (next
+//     step) and counts as returned-from; an internal child is entered.
+//   up at `p` coming back from child `from`: if `from` was the nearer child,
+//     the farther one is visited (scanned or entered) when not pruned;
+//     otherwise p is finished and the walk returns to p's parent.
+// `from` is compared with the split gamma of p: the left child's index is
+// gamma and the right child's gamma+1, for leaves and internal nodes alike.
 template <int K>
 __device__ void knn_search(const float4* __restrict__ pts, const TravNode* __restrict__ nodes, int U, float qx,
                            float qy, float qz, uint32_t r2bits, int bucket, TopK<K>& top) {
-    if (U <= bucket) {
-        scan_range<K>(pts, 0, U - 1, qx, qy, qz, top);
-        return;
-    }
-    const int* node_words = reinterpret_cast<const int*>(nodes);
-    int cur = 0;
-    int from = -1;
+    bool scan = U <= bucket;
+    bool done = scan;
+    int slo = 0, shi = U - 1;
+    int node = 0, p = -1, from = -1;
     bool down = true;
     while (true) {
+        if (scan) {
+            scan_range<K>(pts, slo, shi, qx, qy, qz, top);
+            scan = false;
+            if (done) break;
+        }
         const uint32_t kb = top.kth_bits();
         const uint32_t bound = kb < r2bits ? kb : r2bits;
+        const TravNode n = nodes[down ? node : p];
+        const int a = w_int(n.lo_l.w), g = w_int(n.hi_l.w), b = w_int(n.lo_r.w), parent = w_int(n.hi_r.w);
+        const float dl = box_dist2(qx, qy, qz, n.lo_l, n.hi_l);
+        const float dr = box_dist2(qx, qy, qz, n.lo_r, n.hi_r);
+        const bool lf = dl <= dr;
         if (down) {
-            const TravNode n = nodes[cur];
-            const int a = w_int(n.lo_l.w), g = w_int(n.hi_l.w), b = w_int(n.lo_r.w);
-            const float dl = box_dist2(qx, qy, qz, n.lo_l, n.hi_l);
-            const float dr = box_dist2(qx, qy, qz, n.lo_r, n.hi_r);
-            const bool lf = dl <= dr;
-            const uint32_t nd = __float_as_uint(lf ? dl : dr);
-            if (nd > bound) {
-                from = cur;
+            if (__float_as_uint(lf ? dl : dr) > bound) {  // both children pruned
+                from = node;
+                p = parent;
                 down = false;
+                if (p < 0) break;
                 continue;
             }
+            const int near = lf ? g : g + 1;
             const int nlo = lf ? a : g + 1, nhi = lf ? g : b;
             if (nhi - nlo < bucket) {
-                scan_range<K>(pts, nlo, nhi, qx, qy, qz, top);
-                const uint32_t kb2 = top.kth_bits();
-                const uint32_t bound2 = kb2 < r2bits ? kb2 : r2bits;
-                const uint32_t fd = __float_as_uint(lf ? dr : dl);
-                if (fd <= bound2) {
-                    const int flo = lf ? g + 1 : a, fhi = lf ? b : g;
-                    if (fhi - flo < bucket) {
-                        scan_range<K>(pts, flo, fhi, qx, qy, qz, top);
-                    } else {
-                        cur = lf ? g + 1 : g;
-                        continue;
-                    }
-                }
-                from = cur;
+                scan = true;
+                slo = nlo;
+                shi = nhi;
+                from = near;
+                p = node;
                 down = false;
-                continue;
+            } else {
+                node = near;
             }
-            cur = lf ? g : g + 1;
         } else {
-            if (from == 0) break;
-            const int p = node_words[static_cast<int64_t>(from) * 16 + 15];  // parent, in hi_r.w
-            const TravNode n = nodes[p];
-            const int a = w_int(n.lo_l.w), g = w_int(n.hi_l.w), b = w_int(n.lo_r.w);
-            const float dl = box_dist2(qx, qy, qz, n.lo_l, n.hi_l);
-            const float dr = box_dist2(qx, qy, qz, n.lo_r, n.hi_r);
-            const bool lf = dl <= dr;
-            const bool from_left = (from == g);
-            if (from_left == lf) {
-                const uint32_t fd = __float_as_uint(lf ? dr : dl);
-                if (fd <= bound) {
+            if ((from == g) == lf) {  // returned from the nearer child: the farther one is next
+                if (__float_as_uint(lf ? dr : dl) <= bound) {
+                    const int far = lf ? g + 1 : g;
                     const int flo = lf ? g + 1 : a, fhi = lf ? b : g;
                     if (fhi - flo < bucket) {
-                        scan_range<K>(pts, flo, fhi, qx, qy, qz, top);
+                        scan = true;
+                        slo = flo;
+                        shi = fhi;
+                        from = far;
                     } else {
-                        cur = lf ? g + 1 : g;
+                        node = far;
                         down = true;
-                        continue;
                     }
+                    continue;
                 }
             }
+            if (parent < 0) break;  // p (the root) is finished
             from = p;
+            p = parent;
         }
     }
 }
 
 constexpr int kKnnBlock = 256;
 
-template <int K>
+template <int K, bool kExternal>
 __global__ void __launch_bounds__(kKnnBlock) knn_kernel(const float4* __restrict__ pts,
                                                         const TravNode* __restrict__ nodes,
                                                         const int32_t* __restrict__ d_counts, int64_t q_start,
-                                                        int64_t n_q, const float* __restrict__ radius2, int bucket,
+                                                        int64_t n_q, const float* __restrict__ queries,
+                                                        const float* __restrict__ radius2, int bucket,
                                                         int32_t* __restrict__ out, int32_t* __restrict__ reruns) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n_q) return;
     const int U = d_counts[0];
-    const int64_t qi = q_start + t;
-    if (qi >= U || U <= 0) return;
-    const float4 q = pts[qi];
+    if (U <= 0) return;
+    float qx, qy, qz;
+    if (kExternal) {
+        qx = queries[3 * t];
+        qy = queries[3 * t + 1];
+        qz = queries[3 * t + 2];
+    } else {
+        const int64_t qi = q_start + t;
+        if (qi >= U) return;
+        const float4 q = pts[qi];
+        qx = q.x;
+        qy = q.y;
+        qz = q.z;
+    }
     TopK<K> top;
     top.init();
     uint32_t r2bits = 0xFFFFFFFFu;
@@ -157,12 +169,14 @@ __global__ void __launch_bounds__(kKnnBlock) knn_kernel(const float4* __restrict
         const float r2 = radius2[t];
         r2bits = (r2 >= 0.0f) ? __float_as_uint(r2) : 0xFFFFFFFFu;
     }
-    knn_search<K>(pts, nodes, U, q.x, q.y, q.z, r2bits, bucket, top);
-    if (r2bits != 0xFFFFFFFFu && top.kth_bits() > r2bits) {
-        // Fewer than K points within R: the seeded result is not proven exact.
+    // A seeded search that finds fewer than K points within R is not proven
+    // exact: it is repeated unseeded (one loop so the search is inlined once).
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        knn_search<K>(pts, nodes, U, qx, qy, qz, r2bits, bucket, top);
+        if (r2bits == 0xFFFFFFFFu || top.kth_bits() <= r2bits) break;
         if (reruns) atomicAdd(reruns, 1);
         top.init();
-        knn_search<K>(pts, nodes, U, q.x, q.y, q.z, 0xFFFFFFFFu, bucket, top);
+        r2bits = 0xFFFFFFFFu;
     }
     int32_t* row = out + t * K;
     const int32_t first = static_cast<int32_t>(top.e[0] & 0xFFFFFFFFull);
@@ -177,20 +191,32 @@ __global__ void __launch_bounds__(kKnnBlock) knn_kernel(const float4* __restrict
     }
 }
 
-int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q, int k,
-               const float* radius2, int bucket, int32_t* out, int32_t* reruns, hipStream_t st) {
-    if (n_q <= 0) return SX_OK;
-    if (reinterpret_cast<uintptr_t>(out) % 16 != 0) return fail(SX_EINVAL, "knn: output must be 16-byte aligned");
+template <bool kExternal>
+int knn_dispatch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
+                 const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
+                 hipStream_t st) {
     const unsigned g = grid_for(n_q, kKnnBlock);
     switch (k) {
-        case 8: knn_kernel<8><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, reruns); break;
-        case 16: knn_kernel<16><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, reruns); break;
-        case 24: knn_kernel<24><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, reruns); break;
-        case 32: knn_kernel<32><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, reruns); break;
+        case 8: knn_kernel<8, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns); break;
+        case 16: knn_kernel<16, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns); break;
+        case 24: knn_kernel<24, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns); break;
+        case 32: knn_kernel<32, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns); break;
         default: return fail(SX_EINVAL, "knn: k must be 8, 16, 24 or 32");
     }
     SX_LAUNCH_CHECK("knn_kernel");
     return SX_OK;
+}
+
+// queries == nullptr: the index points [q_start, q_start + n_q) are the
+// queries; otherwise n_q external points (12-byte AoS).
+int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
+               const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
+               hipStream_t st) {
+    if (n_q <= 0) return SX_OK;
+    if (reinterpret_cast<uintptr_t>(out) % 16 != 0) return fail(SX_EINVAL, "knn: output must be 16-byte aligned");
+    if (queries)
+        return knn_dispatch<true>(pts, nodes, d_counts, q_start, n_q, queries, k, radius2, bucket, out, reruns, st);
+    return knn_dispatch<false>(pts, nodes, d_counts, q_start, n_q, nullptr, k, radius2, bucket, out, reruns, st);
 }
 
 }  // namespace sx

@@ -1,0 +1,631 @@
+// Learned kNN radius: octree features + a 8-64-64-1 MLP with BF16 weights,
+// FP32 accumulation, on CDNA matrix cores (v_mfma_f32_16x16x16_bf16).
+//
+// The reference has no learned component; this module seeds the exact kNN
+// descent with a per-query radius (SX_KNN_RADIUS) and never changes its
+// result (the search re-runs when fewer than k points fall in the ball).
+//
+// Features of query u (the point's leaf in the radix tree / octree):
+//   f0..f3  log2 of the point count of the octree cells at depths D0, D0-1,
+//           D0-2, D0-3 containing u, D0 = depth of u's leaf-holding cell
+//   f4      log2 of that cell's edge length (log2(range) - D0)
+//   f5      D0 - root depth
+//   f6, f7  occupied leaf slots and child-cell slots of that octree node
+// Target: ln(squared distance to the k-th neighbour).
+//
+// MFMA layout (16x16x16, four bf16 per lane): lane l holds
+//   A[i = l%16][k = 4*(l/16)+e], B[k = 4*(l/16)+e][j = l%16],
+//   D[i = 4*(l/16)+e][j = l%16], e = 0..3.
+// Weights are the A operand ([out][in] row-major, 8-byte loads) and the
+// activations of 16 queries the B operand, so a layer's D tile (rows = units,
+// columns = queries) is already in the B layout of the next layer.
+// Weight gradients sum over queries; their operands are transposed through
+// LDS.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "sx_internal.hpp"
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+namespace sx {
+
+constexpr int kIn = 8, kInPad = 16, kH = 64;
+// flattened FP32 parameter vector
+constexpr int kOffW1 = 0;                     // [64][8]
+constexpr int kOffB1 = kOffW1 + kH * kIn;     // [64]
+constexpr int kOffW2 = kOffB1 + kH;           // [64][64] (out, in)
+constexpr int kOffB2 = kOffW2 + kH * kH;      // [64]
+constexpr int kOffW3 = kOffB2 + kH;           // [64]
+constexpr int kOffB3 = kOffW3 + kH;           // [1]
+constexpr int kNParams = kOffB3 + 1;          // 4801
+constexpr int kMlpBlock = 256;                // 4 waves, 64 queries per wave
+constexpr int kWavesPerBlock = kMlpBlock / 64;
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (values here are finite)
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+__device__ __forceinline__ bf16x4_t pack4(float a, float b, float c, float d) {
+    bf16x4_t v;
+    v[0] = static_cast<short>(f2bf(a));
+    v[1] = static_cast<short>(f2bf(b));
+    v[2] = static_cast<short>(f2bf(c));
+    v[3] = static_cast<short>(f2bf(d));
+    return v;
+}
+__device__ __forceinline__ bf16x4_t relu_pack(f32x4_t z) {
+    return pack4(fmaxf(z[0], 0.f), fmaxf(z[1], 0.f), fmaxf(z[2], 0.f), fmaxf(z[3], 0.f));
+}
+__device__ __forceinline__ bf16x4_t load4(const uint16_t* p) {
+    const uint2 w = *reinterpret_cast<const uint2*>(p);
+    bf16x4_t v;
+    v[0] = static_cast<short>(w.x & 0xFFFF);
+    v[1] = static_cast<short>(w.x >> 16);
+    v[2] = static_cast<short>(w.y & 0xFFFF);
+    v[3] = static_cast<short>(w.y >> 16);
+    return v;
+}
+__device__ __forceinline__ f32x4_t mfma(bf16x4_t a, bf16x4_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Device-side weights: BF16 copies laid out for the A operand.
+struct MlpDev {
+    const uint16_t* w1p;  // [64][16]  W1 padded with zero inputs 8..15
+    const uint16_t* w2;   // [64][64]  W2[out][in]
+    const uint16_t* w2t;  // [64][64]  W2T[in][out]
+    const uint16_t* w3;   // [64]
+    const float* p;       // FP32 master parameters (biases are read from here)
+    const float* norm;    // mu[8], inv_sigma[8], margin
+};
+
+struct Fwd {
+    f32x4_t z1[4], z2[4];
+    bf16x4_t h1[4], h2[4];
+    float y;  // valid in lanes 0..15: query l
+};
+
+// Forward of one tile of 16 queries; xb is the layer-1 B operand.
+__device__ __forceinline__ void mlp_tile_forward(const MlpDev& m, bf16x4_t xb, Fwd& f) {
+    const int l = static_cast<int>(lane_id());
+    const int r = l % 16, g = l / 16;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        f32x4_t c;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) c[e] = m.p[kOffB1 + 16 * mt + 4 * g + e];
+        const bf16x4_t a = load4(m.w1p + (16 * mt + r) * kInPad + 4 * g);
+        f.z1[mt] = mfma(a, xb, c);
+        f.h1[mt] = relu_pack(f.z1[mt]);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        f32x4_t c;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) c[e] = m.p[kOffB2 + 16 * mt + 4 * g + e];
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) c = mfma(load4(m.w2 + (16 * mt + r) * kH + 16 * kc + 4 * g), f.h1[kc], c);
+        f.z2[mt] = c;
+        f.h2[mt] = relu_pack(c);
+    }
+    f32x4_t c = {0.f, 0.f, 0.f, 0.f};
+    if (g == 0) c[0] = m.p[kOffB3];
+    const bf16x4_t zero4 = {0, 0, 0, 0};
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) {
+        const bf16x4_t a = (r == 0) ? load4(m.w3 + 16 * kc + 4 * g) : zero4;
+        c = mfma(a, f.h2[kc], c);
+    }
+    f.y = c[0];
+}
+
+// Layer-1 B operand for the tile starting at query q0: lane l holds the
+// normalised features 4*(l/16)+e of query q0 + l%16 (zero padding for l >= 32
+// and for queries >= n).
+__device__ __forceinline__ bf16x4_t load_x(const MlpDev& m, const float* __restrict__ feats, int64_t q0, int64_t n) {
+    const int l = static_cast<int>(lane_id());
+    const int r = l % 16, g = l / 16;
+    const int64_t q = q0 + r;
+    if (g >= 2 || q >= n) return bf16x4_t{0, 0, 0, 0};
+    const float4 v = *reinterpret_cast<const float4*>(feats + q * kIn + 4 * g);
+    const float* mu = m.norm;
+    const float* is = m.norm + kIn;
+    return pack4((v.x - mu[4 * g]) * is[4 * g], (v.y - mu[4 * g + 1]) * is[4 * g + 1],
+                 (v.z - mu[4 * g + 2]) * is[4 * g + 2], (v.w - mu[4 * g + 3]) * is[4 * g + 3]);
+}
+
+// mode 0: y (raw prediction); mode 1: exp(y + margin) (squared radius).
+__global__ void __launch_bounds__(kMlpBlock) mlp_forward_kernel(MlpDev m, const float* __restrict__ feats, int64_t n,
+                                                                int mode, float* __restrict__ out) {
+    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    const int l = static_cast<int>(lane_id());
+    const float margin = m.norm[2 * kIn];
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
+        const int64_t q0 = wave * 64 + 16 * t;
+        if (q0 >= n) break;  // wave-uniform
+        Fwd f;
+        mlp_tile_forward(m, load_x(m, feats, q0, n), f);
+        if (l < 16 && q0 + l < n) out[q0 + l] = mode == 0 ? f.y : __expf(f.y + margin);
+    }
+}
+
+// Gradient of mean((y - t)^2) over n samples.  Each wave handles 64 samples
+// (4 tiles) and writes its partial gradient (kNParams floats) to
+// partial[wave]; loss_partial[wave] receives its sum of squared errors.
+__global__ void __launch_bounds__(kMlpBlock) mlp_grad_kernel(MlpDev m, const float* __restrict__ feats,
+                                                             const float* __restrict__ target, int64_t n,
+                                                             float inv_n, float* __restrict__ partial,
+                                                             float* __restrict__ loss_partial) {
+    __shared__ uint16_t s_a[kMlpBlock / kWave][kH][16];  // dz2 or dz1 tile, [unit][query]
+    __shared__ uint16_t s_b[kMlpBlock / kWave][kH][16];  // h1 tile, [unit][query]
+    __shared__ uint16_t s_x[kMlpBlock / kWave][16][16];  // x tile, [feature][query]
+    const int wl = static_cast<int>(threadIdx.x) / kWave;
+    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    const int l = static_cast<int>(lane_id());
+    const int r = l % 16, g = l / 16;
+    f32x4_t gW2[4][4], gW1[4];
+    float gb1[4][4], gb2[4][4], gw3[4][4];
+    float gb3 = 0.f, sse = 0.f;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        gW1[a] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int b = 0; b < 4; ++b) { gW2[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f}; gb1[a][b] = gb2[a][b] = gw3[a][b] = 0.f; }
+    }
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
+        const int64_t q0 = wave * 64 + 16 * t;
+        const bf16x4_t xb = load_x(m, feats, q0, n);
+        Fwd f;
+        mlp_tile_forward(m, xb, f);
+        // dL/dy for query q0 + r, broadcast to every lane of column r
+        float dy = 0.f;
+        if (l < 16 && q0 + l < n) {
+            const float e = f.y - target[q0 + l];
+            dy = 2.0f * e * inv_n;
+            sse += e * e;
+        }
+        const float dyq = __shfl(dy, r, kWave);
+        gb3 += dy;
+        // layer 3 and dz2
+        bf16x4_t dz2b[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            float dz[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int o = 16 * mt + 4 * g + e;
+                gw3[mt][e] += dyq * bf2f(static_cast<uint16_t>(f.h2[mt][e]));
+                dz[e] = f.z2[mt][e] > 0.f ? bf2f(m.w3[o]) * dyq : 0.f;
+                gb2[mt][e] += dz[e];
+            }
+            dz2b[mt] = pack4(dz[0], dz[1], dz[2], dz[3]);
+        }
+        // dh1 = W2^T dz2, then dz1
+        bf16x4_t dz1b[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            f32x4_t c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) c = mfma(load4(m.w2t + (16 * mt + r) * kH + 16 * kc + 4 * g), dz2b[kc], c);
+            float dz[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                dz[e] = f.z1[mt][e] > 0.f ? c[e] : 0.f;
+                gb1[mt][e] += dz[e];
+            }
+            dz1b[mt] = pack4(dz[0], dz[1], dz[2], dz[3]);
+        }
+        // weight gradients: transpose operands through LDS ([unit][query])
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                s_a[wl][16 * mt + 4 * g + e][r] = static_cast<uint16_t>(dz2b[mt][e]);
+                s_b[wl][16 * mt + 4 * g + e][r] = static_cast<uint16_t>(f.h1[mt][e]);
+            }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) s_x[wl][4 * g + e][r] = static_cast<uint16_t>(xb[e]);
+        __syncthreads();
+        // dW2[o][i] += sum_q dz2[o][q] h1[i][q]: A = dz2 (o x q), B = h1^T (q x i)
+#pragma unroll
+        for (int mo = 0; mo < 4; ++mo) {
+            const bf16x4_t a = load4(&s_a[wl][16 * mo + r][4 * g]);
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi) gW2[mo][mi] = mfma(a, load4(&s_b[wl][16 * mi + r][4 * g]), gW2[mo][mi]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) s_a[wl][16 * mt + 4 * g + e][r] = static_cast<uint16_t>(dz1b[mt][e]);
+        __syncthreads();
+        // dW1[o][f] += sum_q dz1[o][q] x[f][q]: A = dz1 (o x q), B = x^T (q x f)
+        const bf16x4_t bx = load4(&s_x[wl][r][4 * g]);
+#pragma unroll
+        for (int mo = 0; mo < 4; ++mo) gW1[mo] = mfma(load4(&s_a[wl][16 * mo + r][4 * g]), bx, gW1[mo]);
+        __syncthreads();
+    }
+    // reduce the per-lane bias and w3 partial sums over the 16 query columns
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                gb1[a][b] += __shfl_xor(gb1[a][b], o, kWave);
+                gb2[a][b] += __shfl_xor(gb2[a][b], o, kWave);
+                gw3[a][b] += __shfl_xor(gw3[a][b], o, kWave);
+            }
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        gb3 += __shfl_xor(gb3, o, kWave);
+        sse += __shfl_xor(sse, o, kWave);
+    }
+    float* P = partial + wave * kNParams;
+#pragma unroll
+    for (int mo = 0; mo < 4; ++mo)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int o = 16 * mo + 4 * g + e;
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi) P[kOffW2 + o * kH + 16 * mi + r] = gW2[mo][mi][e];
+            if (r < kIn) P[kOffW1 + o * kIn + r] = gW1[mo][e];
+            if (r == 0) {
+                P[kOffB1 + o] = gb1[mo][e];
+                P[kOffB2 + o] = gb2[mo][e];
+                P[kOffW3 + o] = gw3[mo][e];
+            }
+        }
+    if (l == 0) {
+        P[kOffB3] = gb3;
+        loss_partial[wave] = sse;
+    }
+}
+
+__global__ void mlp_reduce_kernel(const float* __restrict__ partial, int64_t waves, float* __restrict__ grad) {
+    const int j = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= kNParams) return;
+    float s = 0.f;
+    for (int64_t w = 0; w < waves; ++w) s += partial[w * kNParams + j];
+    grad[j] = s;
+}
+
+__global__ void mlp_sum_kernel(const float* __restrict__ v, int64_t n, float* __restrict__ out) {
+    float s = 0.f;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += v[i];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kWave);
+    __shared__ float ws[4];
+    if (lane_id() == 0) ws[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+// Writes the BF16 operand copies of parameter j (padding is zero from init).
+__device__ __forceinline__ void write_bf16(int j, float v, uint16_t* w1p, uint16_t* w2, uint16_t* w2t, uint16_t* w3) {
+    if (j < kOffB1) {
+        const int o = j / kIn, f = j % kIn;
+        w1p[o * kInPad + f] = f2bf(v);
+    } else if (j >= kOffW2 && j < kOffB2) {
+        const int o = (j - kOffW2) / kH, i = (j - kOffW2) % kH;
+        w2[o * kH + i] = f2bf(v);
+        w2t[i * kH + o] = f2bf(v);
+    } else if (j >= kOffW3 && j < kOffB3) {
+        w3[j - kOffW3] = f2bf(v);
+    }
+}
+
+__global__ void mlp_sync_bf16_kernel(const float* __restrict__ p, uint16_t* w1p, uint16_t* w2, uint16_t* w2t,
+                                     uint16_t* w3) {
+    const int j = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j < kH * kInPad) w1p[j] = 0;
+    __syncthreads();
+    if (j < kNParams) write_bf16(j, p[j], w1p, w2, w2t, w3);
+}
+
+__global__ void mlp_adam_kernel(float* __restrict__ p, float* __restrict__ mo, float* __restrict__ ve,
+                                const float* __restrict__ grad, float lr, float b1, float b2, float eps, float c1,
+                                float c2, float wd, uint16_t* w1p, uint16_t* w2, uint16_t* w2t, uint16_t* w3) {
+    const int j = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= kNParams) return;
+    const float gj = grad[j] + wd * p[j];
+    const float m1 = b1 * mo[j] + (1.f - b1) * gj;
+    const float v1 = b2 * ve[j] + (1.f - b2) * gj * gj;
+    mo[j] = m1;
+    ve[j] = v1;
+    const float pj = p[j] - lr * (m1 * c1) / (sqrtf(v1 * c2) + eps);
+    p[j] = pj;
+    write_bf16(j, pj, w1p, w2, w2t, w3);
+}
+
+// ---------------------------------------------------------------- features
+__device__ __forceinline__ int node_prefix(const TravNode* __restrict__ trav, const uint64_t* __restrict__ codes,
+                                           int node, int code_bits) {
+    const float4 lo_l = trav[node].lo_l, lo_r = trav[node].lo_r;
+    return prefix_len(codes[w_int(lo_l.w)], codes[w_int(lo_r.w)], code_bits);
+}
+
+__global__ void __launch_bounds__(256) features_kernel(const TravNode* __restrict__ trav,
+                                                       const int32_t* __restrict__ leaf_parent,
+                                                       const uint64_t* __restrict__ codes,
+                                                       const int32_t* __restrict__ leaf_oct,
+                                                       const sx_otnode* __restrict__ oct,
+                                                       const float* __restrict__ bounds,
+                                                       const int32_t* __restrict__ d_counts, int code_bits,
+                                                       int64_t q_start, int64_t n_q, float* __restrict__ feats) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n_q) return;
+    const int U = d_counts[0];
+    const int64_t u = q_start + t;
+    float f[kIn] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (u < U && U >= 2) {
+        const int root_level = prefix_len(codes[0], codes[U - 1], code_bits) / 3;
+        int c = leaf_parent[u] >> 1;
+        const int d0 = node_prefix(trav, codes, c, code_bits) / 3;
+        int pc = w_int(trav[c].hi_r.w);
+        int ppn = pc >= 0 ? node_prefix(trav, codes, pc, code_bits) : -1;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int L = max(d0 - s, root_level);
+            // highest ancestor whose prefix still holds 3L bits
+            while (pc >= 0 && ppn >= 3 * L) {
+                c = pc;
+                pc = w_int(trav[c].hi_r.w);
+                ppn = pc >= 0 ? node_prefix(trav, codes, pc, code_bits) : -1;
+            }
+            const int cnt = w_int(trav[c].lo_r.w) - w_int(trav[c].lo_l.w) + 1;
+            f[s] = __log2f(static_cast<float>(cnt));
+        }
+        f[4] = __log2f(bounds[2]) - static_cast<float>(d0);
+        f[5] = static_cast<float>(d0 - root_level);
+        const int on = leaf_oct[u];
+        if (on >= 0) {
+            f[6] = static_cast<float>(__popc(oct[on].child_leaf_mask));
+            f[7] = static_cast<float>(__popc(oct[on].child_node_mask));
+        }
+    }
+    float4* out = reinterpret_cast<float4*>(feats + t * kIn);
+    out[0] = make_float4(f[0], f[1], f[2], f[3]);
+    out[1] = make_float4(f[4], f[5], f[6], f[7]);
+}
+
+}  // namespace sx
+
+// ---------------------------------------------------------------- C ABI
+struct sx_mlp {
+    void* mem = nullptr;
+    float* p = nullptr;      // master params
+    float* mo = nullptr;     // Adam first moment
+    float* ve = nullptr;     // Adam second moment
+    float* grad = nullptr;
+    float* norm = nullptr;   // mu[8], inv_sigma[8], margin
+    float* loss = nullptr;   // [1]
+    uint16_t* w1p = nullptr;
+    uint16_t* w2 = nullptr;
+    uint16_t* w2t = nullptr;
+    uint16_t* w3 = nullptr;
+    float* partial = nullptr;
+    float* loss_partial = nullptr;
+    int64_t max_batch = 0;
+    int64_t step = 0;
+};
+
+namespace {
+
+size_t mlp_layout(sx_mlp* m, void* base) {
+    sx::Carver c(base);
+    m->p = c.take<float>(sx::kNParams);
+    m->mo = c.take<float>(sx::kNParams);
+    m->ve = c.take<float>(sx::kNParams);
+    m->grad = c.take<float>(sx::kNParams);
+    m->norm = c.take<float>(2 * sx::kIn + 1);
+    m->loss = c.take<float>(1);
+    m->w1p = c.take<uint16_t>(sx::kH * sx::kInPad);
+    m->w2 = c.take<uint16_t>(sx::kH * sx::kH);
+    m->w2t = c.take<uint16_t>(sx::kH * sx::kH);
+    m->w3 = c.take<uint16_t>(sx::kH);
+    // whole blocks of waves: every wave of the last block writes its (zero) partial
+    const int64_t waves = ((m->max_batch + 63) / 64 + sx::kWavesPerBlock - 1) / sx::kWavesPerBlock * sx::kWavesPerBlock;
+    m->partial = c.take<float>(static_cast<size_t>(waves) * sx::kNParams);
+    m->loss_partial = c.take<float>(static_cast<size_t>(waves));
+    return c.off;
+}
+
+sx::MlpDev dev_view(const sx_mlp* m) { return sx::MlpDev{m->w1p, m->w2, m->w2t, m->w3, m->p, m->norm}; }
+
+int check_mlp(const sx_mlp* m) { return m ? SX_OK : sx::fail(SX_EINVAL, "null mlp"); }
+
+}  // namespace
+
+// sx_index internals needed by the feature kernel
+struct sx_index;
+namespace sx {
+int index_feature_arrays(sx_index* ix, const TravNode** trav, const int32_t** leaf_parent, const uint64_t** codes,
+                         const int32_t** leaf_oct, const sx_otnode** oct, const float** bounds,
+                         const int32_t** counts, int* code_bits, int64_t* limit);
+}
+
+using sx::fail;
+
+extern "C" {
+
+int sx_mlp_create(sx_mlp** out, int64_t max_batch) {
+    if (!out) return fail(SX_EINVAL, "sx_mlp_create: null out");
+    *out = nullptr;
+    if (max_batch <= 0 || max_batch > (1ll << 26)) return fail(SX_EINVAL, "sx_mlp_create: max_batch out of range");
+    sx_mlp* m = new sx_mlp();
+    m->max_batch = max_batch;
+    const size_t bytes = mlp_layout(m, nullptr);
+    if (hipMalloc(&m->mem, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        delete m;
+        return fail(SX_ENOMEM, "sx_mlp_create: allocation failed");
+    }
+    mlp_layout(m, m->mem);
+    if (sx::fill_u8(m->mem, 0, bytes - bytes % 16, nullptr) != SX_OK || hipDeviceSynchronize() != hipSuccess) {
+        (void)hipFree(m->mem);
+        delete m;
+        return fail(SX_EHIP, "sx_mlp_create: initialisation failed");
+    }
+    *out = m;
+    return SX_OK;
+}
+
+int sx_mlp_destroy(sx_mlp* m) {
+    if (!m) return SX_OK;
+    int rc = SX_OK;
+    if (m->mem && hipFree(m->mem) != hipSuccess) rc = fail(SX_EHIP, "sx_mlp_destroy: hipFree failed");
+    delete m;
+    return rc;
+}
+
+int sx_mlp_num_params(void) { return sx::kNParams; }
+
+// Host FP32 parameters in the flattened order W1[64][8], b1, W2[64][64], b2,
+// w3[64], b3; resets the Adam state.
+int sx_mlp_set_params(sx_mlp* m, const float* h_params, void* stream) {
+    SX_TRY(check_mlp(m));
+    if (!h_params) return fail(SX_EINVAL, "sx_mlp_set_params: null params");
+    hipStream_t st = sx::as_stream(stream);
+    SX_HIP(hipMemcpyAsync(m->p, h_params, sizeof(float) * sx::kNParams, hipMemcpyHostToDevice, st));
+    SX_TRY(sx::fill_u8(m->mo, 0, sizeof(float) * sx::kNParams, st));
+    SX_TRY(sx::fill_u8(m->ve, 0, sizeof(float) * sx::kNParams, st));
+    sx::mlp_sync_bf16_kernel<<<(sx::kNParams + 1023) / 1024 * 1, 1024, 0, st>>>(m->p, m->w1p, m->w2, m->w2t, m->w3);
+    SX_LAUNCH_CHECK("mlp_sync_bf16_kernel");
+    SX_HIP(hipStreamSynchronize(st));
+    m->step = 0;
+    return SX_OK;
+}
+
+int sx_mlp_get_params(sx_mlp* m, float* h_params, void* stream) {
+    SX_TRY(check_mlp(m));
+    if (!h_params) return fail(SX_EINVAL, "sx_mlp_get_params: null params");
+    hipStream_t st = sx::as_stream(stream);
+    SX_HIP(hipMemcpyAsync(h_params, m->p, sizeof(float) * sx::kNParams, hipMemcpyDeviceToHost, st));
+    SX_HIP(hipStreamSynchronize(st));
+    return SX_OK;
+}
+
+// Glorot-uniform weights, zero biases, from a host RNG seeded with `seed`.
+int sx_mlp_init(sx_mlp* m, uint64_t seed, void* stream) {
+    SX_TRY(check_mlp(m));
+    std::vector<float> p(sx::kNParams, 0.f);
+    std::mt19937_64 gen(seed);
+    auto fill = [&](int off, int fan_in, int fan_out, int count) {
+        const float a = std::sqrt(6.0f / static_cast<float>(fan_in + fan_out));
+        std::uniform_real_distribution<float> d(-a, a);
+        for (int i = 0; i < count; ++i) p[off + i] = d(gen);
+    };
+    fill(sx::kOffW1, sx::kIn, sx::kH, sx::kH * sx::kIn);
+    fill(sx::kOffW2, sx::kH, sx::kH, sx::kH * sx::kH);
+    fill(sx::kOffW3, sx::kH, 1, sx::kH);
+    return sx_mlp_set_params(m, p.data(), stream);
+}
+
+// mu[8], sigma[8] (features are used as (f - mu) / sigma) and the log-margin
+// added to the prediction when producing squared radii.
+int sx_mlp_set_normalizer(sx_mlp* m, const float* h_mu, const float* h_sigma, float margin, void* stream) {
+    SX_TRY(check_mlp(m));
+    if (!h_mu || !h_sigma) return fail(SX_EINVAL, "sx_mlp_set_normalizer: null pointer");
+    float h[2 * sx::kIn + 1];
+    for (int i = 0; i < sx::kIn; ++i) {
+        if (!(h_sigma[i] > 0.f)) return fail(SX_EINVAL, "sx_mlp_set_normalizer: sigma must be positive");
+        h[i] = h_mu[i];
+        h[sx::kIn + i] = 1.0f / h_sigma[i];
+    }
+    h[2 * sx::kIn] = margin;
+    hipStream_t st = sx::as_stream(stream);
+    SX_HIP(hipMemcpyAsync(m->norm, h, sizeof(h), hipMemcpyHostToDevice, st));
+    SX_HIP(hipStreamSynchronize(st));
+    return SX_OK;
+}
+
+int sx_mlp_forward(sx_mlp* m, const float* d_feats, int64_t n, float* d_out, void* stream) {
+    SX_TRY(check_mlp(m));
+    if (n < 0 || (n > 0 && (!d_feats || !d_out))) return fail(SX_EINVAL, "sx_mlp_forward: bad arguments");
+    if (n == 0) return SX_OK;
+    sx::mlp_forward_kernel<<<sx::grid_for(n, sx::kMlpBlock), sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
+        dev_view(m), d_feats, n, 0, d_out);
+    SX_LAUNCH_CHECK("mlp_forward_kernel");
+    return SX_OK;
+}
+
+int sx_mlp_predict_radius2(sx_mlp* m, const float* d_feats, int64_t n, float* d_radius2, void* stream) {
+    SX_TRY(check_mlp(m));
+    if (n < 0 || (n > 0 && (!d_feats || !d_radius2))) return fail(SX_EINVAL, "sx_mlp_predict_radius2: bad arguments");
+    if (n == 0) return SX_OK;
+    sx::mlp_forward_kernel<<<sx::grid_for(n, sx::kMlpBlock), sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
+        dev_view(m), d_feats, n, 1, d_radius2);
+    SX_LAUNCH_CHECK("mlp_forward_kernel");
+    return SX_OK;
+}
+
+// Gradient of the mean squared error over n samples into d_grad (kNParams
+// floats, device) and the summed squared error into *d_loss (device, nullable).
+int sx_mlp_backward(sx_mlp* m, const float* d_feats, const float* d_targets, int64_t n, float* d_grad, float* d_loss,
+                    void* stream) {
+    SX_TRY(check_mlp(m));
+    if (n <= 0 || n > m->max_batch || !d_feats || !d_targets)
+        return fail(SX_EINVAL, "sx_mlp_backward: n must be in [1, max_batch] with non-null inputs");
+    hipStream_t st = sx::as_stream(stream);
+    const int64_t waves = (n + 63) / 64;
+    sx::mlp_grad_kernel<<<sx::grid_for(waves * 64, sx::kMlpBlock), sx::kMlpBlock, 0, st>>>(
+        dev_view(m), d_feats, d_targets, n, 1.0f / static_cast<float>(n), m->partial, m->loss_partial);
+    SX_LAUNCH_CHECK("mlp_grad_kernel");
+    // waves beyond n inside the last block wrote nothing: reduce only real waves
+    sx::mlp_reduce_kernel<<<(sx::kNParams + 255) / 256, 256, 0, st>>>(m->partial, waves, d_grad ? d_grad : m->grad);
+    SX_LAUNCH_CHECK("mlp_reduce_kernel");
+    if (d_loss) {
+        sx::mlp_sum_kernel<<<1, 256, 0, st>>>(m->loss_partial, waves, d_loss);
+        SX_LAUNCH_CHECK("mlp_sum_kernel");
+    }
+    return SX_OK;
+}
+
+// One Adam step on a minibatch (lr, betas 0.9/0.999, eps 1e-8, decoupled
+// weight decay wd); the summed squared error before the step goes to *d_loss (nullable).
+int sx_mlp_adam_step(sx_mlp* m, const float* d_feats, const float* d_targets, int64_t n, float lr, float wd,
+                     float* d_loss, void* stream) {
+    SX_TRY(sx_mlp_backward(m, d_feats, d_targets, n, m->grad, d_loss, stream));
+    m->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f / (1.0f - std::pow(b1, static_cast<float>(m->step)));
+    const float c2 = 1.0f / (1.0f - std::pow(b2, static_cast<float>(m->step)));
+    sx::mlp_adam_kernel<<<(sx::kNParams + 255) / 256, 256, 0, sx::as_stream(stream)>>>(
+        m->p, m->mo, m->ve, m->grad, lr, b1, b2, 1e-8f, c1, c2, wd, m->w1p, m->w2, m->w2t, m->w3);
+    SX_LAUNCH_CHECK("mlp_adam_kernel");
+    return SX_OK;
+}
+
+// Features of index points [q_start, q_start + n_q) into d_feats (n_q x 8 floats).
+int sx_radius_features(sx_index* index, int64_t q_start, int64_t n_q, float* d_feats, void* stream) {
+    const sx::TravNode* trav;
+    const int32_t *lp, *lo, *counts;
+    const uint64_t* codes;
+    const sx_otnode* oct;
+    const float* bounds;
+    int cb;
+    int64_t limit;
+    SX_TRY(sx::index_feature_arrays(index, &trav, &lp, &codes, &lo, &oct, &bounds, &counts, &cb, &limit));
+    if (q_start < 0 || n_q < 0 || q_start + n_q > limit) return fail(SX_EINVAL, "sx_radius_features: bad query range");
+    if (n_q == 0) return SX_OK;
+    if (!d_feats || reinterpret_cast<uintptr_t>(d_feats) % 16 != 0)
+        return fail(SX_EINVAL, "sx_radius_features: output must be 16-byte aligned");
+    sx::features_kernel<<<sx::grid_for(n_q, 256), 256, 0, sx::as_stream(stream)>>>(trav, lp, codes, lo, oct, bounds,
+                                                                                   counts, cb, q_start, n_q, d_feats);
+    SX_LAUNCH_CHECK("features_kernel");
+    return SX_OK;
+}
+
+}  // extern "C"

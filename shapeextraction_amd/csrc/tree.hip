@@ -8,14 +8,15 @@
 // karras:  one thread per internal node (Karras 2012), the same direction,
 //          exponential + binary range search and ceil-step split search as
 //          constructTree (src/RadixTree.cu:99-194); root rule selectable.
-// aabb:    one thread per leaf walks up; at every internal node the second
-//          arriving child (agent-scope acq_rel counter) merges both child
-//          boxes and continues, so each box is written exactly once.
+// boxes:   child boxes of every internal node as range min/max queries over
+//          per-level block boxes (no atomics or fences between threads).
 #include <hip/hip_runtime.h>
 
 #include "sx_internal.hpp"
 
 namespace sx {
+
+constexpr int kMaxBoxLevels = 9;
 
 struct UniqueIn {
     const uint64_t* s;
@@ -135,42 +136,143 @@ int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
     return SX_OK;
 }
 
-__global__ void __launch_bounds__(256) aabb_kernel(const float4* __restrict__ pts, const int32_t* d_count,
-                                                   int64_t n_max, const int32_t* __restrict__ leaf_parent,
-                                                   TravNode* trav, unsigned* visits) {
+// ---------------------------------------------------------------------------
+// Child boxes of every internal node, without inter-thread communication.
+// Every internal node covers a contiguous range of Morton-sorted points, so a
+// box is a range min/max.  Level L >= 1 holds the box of every aligned block
+// of 16^L points (one kernel per level); a range query consumes up to 15
+// elements at each end of the range per level and moves the aligned middle up
+// a level.  min/max are exact, so the boxes equal a bottom-up merge.
+// ---------------------------------------------------------------------------
+constexpr int kBoxFanLog = 4;
+constexpr int kBoxFan = 1 << kBoxFanLog;
+
+struct BoxLevels {
+    float4* lo[kMaxBoxLevels];
+    float4* hi[kMaxBoxLevels];
+    int n_levels;  // levels 1..n_levels-1 are stored; level 0 is the points
+};
+
+__device__ __forceinline__ int64_t level_count(int64_t U, int L) {
+    int64_t n = U;
+    for (int i = 0; i < L; ++i) n = (n + kBoxFan - 1) >> kBoxFanLog;
+    return n;
+}
+
+__global__ void __launch_bounds__(256) box_level_kernel(const float4* __restrict__ pts, BoxLevels lv, int L,
+                                                        const int32_t* d_count, int64_t n_max) {
     const int64_t U = scan_count(d_count, n_max);
-    const int64_t u = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (U < 2 || u >= U) return;
-    const int lp = leaf_parent[u];
-    int p = lp >> 1;
-    int side = lp & 1;
-    const float4 pt = pts[u];
-    float lx = pt.x, ly = pt.y, lz = pt.z, hx = pt.x, hy = pt.y, hz = pt.z;
-    while (true) {
-        TravNode* nd = trav + p;
-        float4* lo = side ? &nd->lo_r : &nd->lo_l;
-        float4* hi = side ? &nd->hi_r : &nd->hi_l;
-        lo->x = lx; lo->y = ly; lo->z = lz;
-        hi->x = hx; hi->y = hy; hi->z = hz;
-        const unsigned prev = __hip_atomic_fetch_add(visits + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == 0) return;
-        const float4* olo = side ? &nd->lo_l : &nd->lo_r;
-        const float4* ohi = side ? &nd->hi_l : &nd->hi_r;
-        lx = fminf(lx, olo->x); ly = fminf(ly, olo->y); lz = fminf(lz, olo->z);
-        hx = fmaxf(hx, ohi->x); hy = fmaxf(hy, ohi->y); hz = fmaxf(hz, ohi->z);
-        const int pp = w_int(nd->hi_r.w);
-        if (pp < 0) return;
-        side = (w_int(trav[pp].hi_l.w) == p) ? 0 : 1;
-        p = pp;
+    const int64_t n_dst = level_count(U, L), n_src = level_count(U, L - 1);
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n_dst) return;
+    float lx = INFINITY, ly = INFINITY, lz = INFINITY, hx = -INFINITY, hy = -INFINITY, hz = -INFINITY;
+    const int64_t s0 = j << kBoxFanLog;
+    const int64_t s1 = s0 + kBoxFan < n_src ? s0 + kBoxFan : n_src;
+    for (int64_t s = s0; s < s1; ++s) {
+        float4 l, h;
+        if (L == 1) {
+            l = h = pts[s];
+        } else {
+            l = lv.lo[L - 1][s];
+            h = lv.hi[L - 1][s];
+        }
+        lx = fminf(lx, l.x); ly = fminf(ly, l.y); lz = fminf(lz, l.z);
+        hx = fmaxf(hx, h.x); hy = fmaxf(hy, h.y); hz = fmaxf(hz, h.z);
+    }
+    lv.lo[L][j] = make_float4(lx, ly, lz, 0.0f);
+    lv.hi[L][j] = make_float4(hx, hy, hz, 0.0f);
+}
+
+struct Box3 {
+    float lx, ly, lz, hx, hy, hz;
+    __device__ __forceinline__ void init() {
+        lx = ly = lz = INFINITY;
+        hx = hy = hz = -INFINITY;
+    }
+    __device__ __forceinline__ void add(float4 l, float4 h) {
+        lx = fminf(lx, l.x); ly = fminf(ly, l.y); lz = fminf(lz, l.z);
+        hx = fmaxf(hx, h.x); hy = fmaxf(hy, h.y); hz = fmaxf(hz, h.z);
+    }
+};
+
+__device__ __forceinline__ void add_elem(const float4* __restrict__ pts, const BoxLevels& lv, int L, int64_t j,
+                                         Box3& bx) {
+    if (L == 0) {
+        const float4 p = pts[j];
+        bx.add(p, p);
+    } else {
+        bx.add(lv.lo[L][j], lv.hi[L][j]);
     }
 }
 
-int aabb_launch(const float4* pts, const int32_t* d_count, int64_t n_max, const int32_t* leaf_parent, TravNode* trav,
-                unsigned* visits, hipStream_t st) {
+// Box of the points [a, b] (inclusive).
+__device__ Box3 range_box(const float4* __restrict__ pts, const BoxLevels& lv, int64_t a, int64_t b) {
+    Box3 bx;
+    bx.init();
+    int L = 0;
+    while (a <= b) {
+        if (L == lv.n_levels - 1) {
+            for (int64_t j = a; j <= b; ++j) add_elem(pts, lv, L, j, bx);
+            break;
+        }
+        while (a <= b && (a & (kBoxFan - 1))) add_elem(pts, lv, L, a++, bx);
+        while (a <= b && ((b + 1) & (kBoxFan - 1))) add_elem(pts, lv, L, b--, bx);
+        if (a > b) break;
+        a >>= kBoxFanLog;
+        b = ((b + 1) >> kBoxFanLog) - 1;
+        ++L;
+    }
+    return bx;
+}
+
+__global__ void __launch_bounds__(256) child_box_kernel(const float4* __restrict__ pts, BoxLevels lv,
+                                                        const int32_t* d_count, int64_t n_max, TravNode* trav) {
+    const int64_t U = scan_count(d_count, n_max);
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= U - 1) return;
+    TravNode* t = trav + i;
+    const int a = w_int(t->lo_l.w), g = w_int(t->hi_l.w), b = w_int(t->lo_r.w);
+    const Box3 l = range_box(pts, lv, a, g);
+    const Box3 r = range_box(pts, lv, g + 1, b);
+    t->lo_l.x = l.lx; t->lo_l.y = l.ly; t->lo_l.z = l.lz;
+    t->hi_l.x = l.hx; t->hi_l.y = l.hy; t->hi_l.z = l.hz;
+    t->lo_r.x = r.lx; t->lo_r.y = r.ly; t->lo_r.z = r.lz;
+    t->hi_r.x = r.hx; t->hi_r.y = r.hy; t->hi_r.z = r.hz;
+}
+
+int box_levels_for(int64_t n_max) {
+    int n = 1;
+    for (int64_t c = n_max; c > 1 && n < kMaxBoxLevels; c = (c + kBoxFan - 1) / kBoxFan) ++n;
+    return n;
+}
+
+size_t box_ws_bytes(int64_t n_max) {
+    Carver c(nullptr);
+    int64_t cnt = n_max;
+    for (int L = 1; L < box_levels_for(n_max); ++L) {
+        cnt = (cnt + kBoxFan - 1) / kBoxFan;
+        c.take<float4>(static_cast<size_t>(cnt));
+        c.take<float4>(static_cast<size_t>(cnt));
+    }
+    return c.off;
+}
+
+int child_boxes_launch(const float4* pts, const int32_t* d_count, int64_t n_max, TravNode* trav, void* ws,
+                       hipStream_t st) {
     if (n_max < 2) return SX_OK;
-    SX_TRY(fill_u8(visits, 0, static_cast<size_t>(n_max) * sizeof(unsigned), st));
-    aabb_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(pts, d_count, n_max, leaf_parent, trav, visits);
-    SX_LAUNCH_CHECK("aabb_kernel");
+    BoxLevels lv{};
+    lv.n_levels = box_levels_for(n_max);
+    Carver c(ws);
+    int64_t cnt = n_max;
+    for (int L = 1; L < lv.n_levels; ++L) {
+        cnt = (cnt + kBoxFan - 1) / kBoxFan;
+        lv.lo[L] = c.take<float4>(static_cast<size_t>(cnt));
+        lv.hi[L] = c.take<float4>(static_cast<size_t>(cnt));
+        box_level_kernel<<<grid_for(cnt, 256), 256, 0, st>>>(pts, lv, L, d_count, n_max);
+        SX_LAUNCH_CHECK("box_level_kernel");
+    }
+    child_box_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(pts, lv, d_count, n_max, trav);
+    SX_LAUNCH_CHECK("child_box_kernel");
     return SX_OK;
 }
 

@@ -125,7 +125,18 @@ def test_reference_root_degenerate_codes(gpu, O):
         o = O.karras(codes, 63, mode)
         for k in ("prefix_n", "left_child", "has_leaf_left", "has_leaf_right"):
             assert np.array_equal(rt[k].numpy()[:4], o[k][:4]), (mode, k)
-        assert np.array_equal(rt["parent"].numpy(), o["parent"]), mode
+        # Under the reference root this input makes two nodes claim the same
+        # children (the reference's constructTree races on parent[] there), so
+        # only children claimed once have a defined parent.
+        claims = np.zeros(len(codes), int)
+        for i in range(len(codes) - 1):
+            g = o["left_child"][i]
+            claims[g] += 0 if o["has_leaf_left"][i] else 1
+            claims[g + 1] += 0 if o["has_leaf_right"][i] else 1
+        once = claims <= 1
+        assert np.array_equal(rt["parent"].numpy()[once], o["parent"][once]), mode
+        if mode == 0:
+            assert once.all()
 
 
 def test_sort_pairs_arbitrary_keys(gpu):

@@ -309,3 +309,36 @@ def test_reference_f32_normals_close_to_f64(O, sx):
     assert np.median(err) < 0.5
     # orientation rule: (0 - p) . n >= 0
     assert (np.sum(-pts[:5000] * ref32, axis=1) >= 0).all()
+
+
+# ---------------------------------------------------------------- MLP oracle
+def test_mlp_oracle_gradient_matches_finite_differences(O):
+    g = rng(8)
+    P = (g.normal(size=4801) * 0.2).astype(np.float32)
+    f = g.normal(size=(40, 8)).astype(np.float32)
+    t = g.normal(size=40).astype(np.float32)
+    mu, sg = np.zeros(8, np.float32), np.ones(8, np.float32)
+    grad, _ = O.mlp_grad(P, f, t, mu, sg, emulate=False)
+
+    def loss(PP):
+        return np.mean((O.mlp_forward(PP, f, mu, sg, emulate=False) - t) ** 2)
+
+    # one index per parameter block: W1, b1, W2, b2, w3, b3
+    for j in (3, 517, 1000, 4200, 4650, 4750, 4800):
+        e = 3e-5
+        a, b = P.copy(), P.copy()
+        a[j] += e
+        b[j] -= e
+        num = (loss(a) - loss(b)) / (np.float64(a[j]) - np.float64(b[j]))
+        assert abs(num - grad[j]) <= 1e-3 * abs(grad[j]) + 1e-7, (j, num, grad[j])
+
+
+def test_mlp_oracle_bf16_emulation_is_close(O):
+    g = rng(9)
+    P = (g.normal(size=4801) * 0.2).astype(np.float32)
+    f = g.normal(size=(200, 8)).astype(np.float32)
+    mu, sg = np.zeros(8, np.float32), np.ones(8, np.float32)
+    y64 = O.mlp_forward(P, f, mu, sg, emulate=False)
+    y16 = O.mlp_forward(P, f, mu, sg, emulate=True)
+    assert np.max(np.abs(y64 - y16)) < 0.05 * (1 + np.abs(y64).max())
+    assert not np.array_equal(y64, y16)
