@@ -50,7 +50,6 @@ def cpu_baseline(workload_n: int, k: int) -> dict:
     from oracle import oracle as O
 
     import shapeextraction_amd as sx
-    from shapeextraction_amd import learned
 
     n = int(os.environ.get("SX_CPU_SAMPLE", 1_048_576))
     x, y, z = sx.gen_scene(2, n)
@@ -86,6 +85,7 @@ def main() -> int:
         return 2
 
     import shapeextraction_amd as sx
+    from shapeextraction_amd import learned
 
     if sx.device_count() == 0:
         print("no HIP device visible", file=sys.stderr)

@@ -54,6 +54,12 @@ extern "C" {
 #define SX_KNN_PLAIN 0    /* exact nearest-first descent from the root            */
 #define SX_KNN_RADIUS 1   /* exact descent seeded with a per-query radius          */
 
+/* Index algorithm choices (results are identical; speed differs). */
+#define SX_KNN_ALGO_TILES 0     /* wave-cooperative tiles + per-lane queue (default) */
+#define SX_KNN_ALGO_LANE 1      /* one lane per query, stackless descent            */
+#define SX_SORT_ONESWEEP 0      /* single-pass-per-digit radix sort (default)       */
+#define SX_SORT_THREE_KERNEL 1  /* histogram, scan, scatter per digit               */
+
 /* Normal precision modes. */
 #define SX_PREC_FP32 0    /* compensated FP32 covariance + closed-form FP32 eigen,
                              FP64 Jacobi for near-repeated smallest eigenvalues   */
@@ -218,8 +224,11 @@ int sx_index_get_view(sx_index* index, sx_index_view* view, void* stream);
  * on results).  No reference counterpart (the reference descends to single
  * points, src/Octree.hpp:160-178). */
 int sx_index_set_bucket(sx_index* index, int bucket);
-/* Number of radius-seeded queries that had to be searched again unseeded
- * since the last reset (synchronises `stream`). */
+/* Algorithms used by this index (SX_KNN_ALGO_*, SX_SORT_*). */
+int sx_index_set_algorithms(sx_index* index, int knn_algo, int sort_algo);
+/* Number of queries the kNN could not settle on its fast path (tile search,
+ * or the seeded search of SX_KNN_RADIUS) and searched again one lane per
+ * query, since the last reset (synchronises `stream`). */
 int sx_index_knn_reruns(sx_index* index, int64_t* reruns, int reset, void* stream);
 
 /* Exact k nearest neighbours of the index points [q_start, q_start+n_q)

@@ -16,6 +16,8 @@ SX_OK, SX_EINVAL, SX_ENOMEM, SX_EHIP, SX_ERANGE, SX_EDATA, SX_ESTATE = range(7)
 SX_ROOT_TEXTBOOK, SX_ROOT_REFERENCE = 0, 1
 SX_KNN_PLAIN, SX_KNN_RADIUS = 0, 1
 SX_PREC_FP32, SX_PREC_FP64 = 0, 1
+SX_KNN_ALGO_TILES, SX_KNN_ALGO_LANE = 0, 1
+SX_SORT_ONESWEEP, SX_SORT_THREE_KERNEL = 0, 1
 
 STATUS_NAMES = {
     SX_OK: "SX_OK",
@@ -120,6 +122,7 @@ SIGNATURES = {
     "sx_index_build": (I, [P, P, P, P, I64, P]),
     "sx_index_get_view": (I, [P, C.POINTER(IndexView), P]),
     "sx_index_set_bucket": (I, [P, I]),
+    "sx_index_set_algorithms": (I, [P, I, I]),
     "sx_index_knn_reruns": (I, [P, C.POINTER(C.c_int64), I, P]),
     "sx_knn": (I, [P, I64, I64, I, I, P, P, P]),
     "sx_knn_points": (I, [P, P, I64, I, P, P]),

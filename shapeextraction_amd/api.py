@@ -13,8 +13,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import (SX_KNN_PLAIN, SX_KNN_RADIUS, SX_PREC_FP32, SX_PREC_FP64, SX_ROOT_REFERENCE,
-                   SX_ROOT_TEXTBOOK, call)
+from ._lib import (SX_KNN_ALGO_LANE, SX_KNN_ALGO_TILES, SX_KNN_PLAIN, SX_KNN_RADIUS, SX_PREC_FP32, SX_PREC_FP64,
+                   SX_ROOT_REFERENCE, SX_ROOT_TEXTBOOK, SX_SORT_ONESWEEP, SX_SORT_THREE_KERNEL, call)
 from .device import DeviceArray, fetch
 
 OTNODE_DTYPE = np.dtype(
@@ -240,6 +240,9 @@ class Index:
     def set_bucket(self, bucket: int):
         call("sx_index_set_bucket", self.handle, bucket)
 
+    def set_algorithms(self, knn: int = _lib.SX_KNN_ALGO_TILES, sort: int = _lib.SX_SORT_ONESWEEP):
+        call("sx_index_set_algorithms", self.handle, knn, sort)
+
     def build(self, x, y, z, stream=None) -> "Index":
         dx, dy, dz = _dev(x, np.float32), _dev(y, np.float32), _dev(z, np.float32)
         self._inputs = (dx, dy, dz)  # keep alive until the stream has consumed them
@@ -321,5 +324,6 @@ __all__ = [
     "Index", "Workload", "WORKLOADS", "OTNODE_DTYPE", "bbox", "morton_encode", "morton_decode", "sort_pairs",
     "unique", "radix_tree", "octree", "check_counts", "gen_sphere", "gen_scene", "gen_gmm_scene",
     "gen_batch_sizes", "gen_degenerate", "fnv64", "SX_KNN_PLAIN", "SX_KNN_RADIUS", "SX_PREC_FP32", "SX_PREC_FP64",
-    "SX_ROOT_TEXTBOOK", "SX_ROOT_REFERENCE",
+    "SX_ROOT_TEXTBOOK", "SX_ROOT_REFERENCE", "SX_KNN_ALGO_TILES", "SX_KNN_ALGO_LANE", "SX_SORT_ONESWEEP",
+    "SX_SORT_THREE_KERNEL",
 ]

@@ -29,7 +29,7 @@ int decode_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
                   float* pts12, float4* pts16, hipStream_t st);
 size_t sort_ws_bytes(int64_t n);
 int sort_pairs_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n, int key_bits, uint64_t* keys_out,
-                      uint32_t* vals_out, void* ws, hipStream_t st);
+                      uint32_t* vals_out, void* ws, int algo, hipStream_t st);
 int unique_launch(const uint64_t* sorted, const uint32_t* vals, int64_t n, uint64_t* uniq, uint32_t* rep,
                   int32_t* uos, int32_t* uoi, int32_t* d_count, void* ws, hipStream_t st);
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
@@ -46,12 +46,18 @@ int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
 int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
                hipStream_t st);
+int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
+                     int k, const float* radius2, int bucket, int32_t* out, int32_t* queue, int32_t* queue_count,
+                     int32_t* deferred_total, hipStream_t st);
 int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn, int64_t q_start, int64_t n_q, int k,
                    int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st);
 }  // namespace sx
 
-// counts[] slots
-enum { kCntUnique = 0, kCntOct = 1, kCntBad = 2, kCntReruns = 3 };
+// counts[] slots: unique points, octree nodes, non-finite inputs, queries the
+// kNN deferred to the per-lane search (cumulative), the last call's queue size
+enum { kCntUnique = 0, kCntOct = 1, kCntBad = 2, kCntReruns = 3, kCntQueue = 4 };
+
+
 
 struct sx_index {
     int64_t cap = 0;
@@ -59,6 +65,8 @@ struct sx_index {
     int code_bits = 63;
     int root_mode = SX_ROOT_TEXTBOOK;
     int bucket = 16;
+    int knn_algo = SX_KNN_ALGO_TILES;
+    int sort_algo = SX_SORT_ONESWEEP;
     void* mem = nullptr;
     float* bounds = nullptr;
     int32_t* counts = nullptr;
@@ -77,6 +85,7 @@ struct sx_index {
     int32_t* parent = nullptr;
     int32_t* leaf_parent = nullptr;
     int32_t* leaf_oct = nullptr;
+    int32_t* knn_queue = nullptr;
     sx::TravNode* trav = nullptr;
     sx_otnode* oct = nullptr;
     void* ws_bbox = nullptr;
@@ -114,6 +123,7 @@ size_t index_layout(sx_index* ix, void* base) {
     ix->parent = c.take<int32_t>(n);
     ix->leaf_parent = c.take<int32_t>(n);
     ix->leaf_oct = c.take<int32_t>(n);
+    ix->knn_queue = c.take<int32_t>(n);
     ix->trav = c.take<sx::TravNode>(n);
     ix->oct = c.take<sx_otnode>(static_cast<size_t>(ix->oct_cap));
     ix->ws_bbox = c.take<char>(sx::bbox_ws_bytes());
@@ -213,6 +223,17 @@ int sx_index_set_bucket(sx_index* ix, int bucket) {
     return SX_OK;
 }
 
+int sx_index_set_algorithms(sx_index* ix, int knn_algo, int sort_algo) {
+    if (!ix) return fail(SX_EINVAL, "sx_index_set_algorithms: null index");
+    if (knn_algo != SX_KNN_ALGO_TILES && knn_algo != SX_KNN_ALGO_LANE)
+        return fail(SX_EINVAL, "sx_index_set_algorithms: bad kNN algorithm");
+    if (sort_algo != SX_SORT_ONESWEEP && sort_algo != SX_SORT_THREE_KERNEL)
+        return fail(SX_EINVAL, "sx_index_set_algorithms: bad sort algorithm");
+    ix->knn_algo = knn_algo;
+    ix->sort_algo = sort_algo;
+    return SX_OK;
+}
+
 int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float* d_z, int64_t n, void* stream) {
     if (!ix) return fail(SX_EINVAL, "sx_index_build: null index");
     if (n <= 0 || n > ix->cap) return fail(SX_EINVAL, "sx_index_build: n must be in [1, capacity]");
@@ -224,7 +245,7 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
     SX_TRY(sx::fill_u8(ix->counts, 0, 32, st));
     SX_TRY(sx::bbox_launch(d_x, d_y, d_z, n, ix->bounds, ix->counts + kCntBad, ix->ws_bbox, false, st));
     SX_TRY(sx::encode_launch(d_x, d_y, d_z, n, ix->bounds, cb, ix->codes_in, st));
-    SX_TRY(sx::sort_pairs_launch(ix->codes_in, nullptr, n, cb, ix->codes_sorted, ix->perm, ix->ws, st));
+    SX_TRY(sx::sort_pairs_launch(ix->codes_in, nullptr, n, cb, ix->codes_sorted, ix->perm, ix->ws, ix->sort_algo, st));
     SX_TRY(sx::unique_launch(ix->codes_sorted, ix->perm, n, ix->ucodes, ix->rep, nullptr, ix->uoi,
                              ix->counts + kCntUnique, ix->ws, st));
     SX_TRY(sx::decode_launch(ix->ucodes, ix->counts + kCntUnique, n, ix->bounds, cb, ix->pts12, ix->pts16, st));
@@ -296,9 +317,13 @@ int sx_knn(sx_index* ix, int64_t q_start, int64_t n_q, int k, int mode, const fl
     if (mode == SX_KNN_RADIUS && !d_radius2) return fail(SX_EINVAL, "sx_knn: radius mode needs d_radius2");
     SX_TRY(check_queries(ix, q_start, n_q));
     if (n_q > 0 && !d_out) return fail(SX_EINVAL, "sx_knn: null output");
-    return sx::knn_launch(ix->pts16, ix->trav, ix->counts, q_start, n_q, nullptr, k,
-                          mode == SX_KNN_RADIUS ? d_radius2 : nullptr, ix->bucket, d_out, ix->counts + kCntReruns,
-                          sx::as_stream(stream));
+    const float* r2 = mode == SX_KNN_RADIUS ? d_radius2 : nullptr;
+    if (ix->knn_algo == SX_KNN_ALGO_TILES)
+        return sx::knn_tiles_launch(ix->pts16, ix->trav, ix->counts, q_start, n_q, k, r2, ix->bucket, d_out,
+                                    ix->knn_queue, ix->counts + kCntQueue, ix->counts + kCntReruns,
+                                    sx::as_stream(stream));
+    return sx::knn_launch(ix->pts16, ix->trav, ix->counts, q_start, n_q, nullptr, k, r2, ix->bucket, d_out,
+                          ix->counts + kCntReruns, sx::as_stream(stream));
 }
 
 int sx_knn_points(sx_index* ix, const float* d_queries, int64_t n_q, int k, int32_t* d_out, void* stream) {

@@ -219,4 +219,273 @@ int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts
     return knn_dispatch<false>(pts, nodes, d_counts, q_start, n_q, nullptr, k, radius2, bucket, out, reruns, st);
 }
 
+
+// ---------------------------------------------------------------------------
+// Wave-cooperative tiles.  A wave owns 64 consecutive queries (Morton
+// neighbours, so spatially compact).  Each lane gets a squared radius r2:
+//   plain:   the k-th best squared distance among the index points within
+//            +-32 positions of the tile (a true upper bound of its k-th
+//            neighbour distance); those points seed the lane's list;
+//   learned: the predicted r2 (any value; checked at the end).
+// Lanes whose radius is at most twice the tile's mean join the tile search:
+// the wave walks the tree once, breadth first, for the union box E of their
+// balls, collecting the leaf ranges of every subtree whose box meets E.  E is
+// widened with directed rounding so that every point with dist2 <= r2 of a
+// joining lane lies inside it (same monotonicity argument as box_dist2).
+// Candidate points are then streamed with wave-uniform loads while every lane
+// updates its own list.  A lane's list is exact when (plain) it joined the
+// search, or (learned) it also found at least K points within its radius.
+// Other lanes - and whole tiles whose frontier or range list overflows LDS -
+// are appended to a queue that knn_queue_kernel searches one lane per query.
+// ---------------------------------------------------------------------------
+constexpr int kTileWaves = kKnnBlock / kWave;
+constexpr int kFrontCap = 128;
+constexpr int kRangeCap = 256;
+constexpr int kWindow = 32;
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ __forceinline__ bool box_meets(float4 lo, float4 hi, float ex0, float ey0, float ez0, float ex1, float ey1,
+                                          float ez1) {
+    return hi.x >= ex0 && lo.x <= ex1 && hi.y >= ey0 && lo.y <= ey1 && hi.z >= ez0 && lo.z <= ez1;
+}
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <int K>
+__device__ __forceinline__ void write_row(int32_t* row, const TopK<K>& top) {
+    const int32_t first = static_cast<int32_t>(top.e[0] & 0xFFFFFFFFull);
+#pragma unroll
+    for (int s = 0; s < K; s += 4) {
+        int4 v;
+        v.x = top.e[s] == ~0ull ? first : static_cast<int32_t>(top.e[s] & 0xFFFFFFFFull);
+        v.y = top.e[s + 1] == ~0ull ? first : static_cast<int32_t>(top.e[s + 1] & 0xFFFFFFFFull);
+        v.z = top.e[s + 2] == ~0ull ? first : static_cast<int32_t>(top.e[s + 2] & 0xFFFFFFFFull);
+        v.w = top.e[s + 3] == ~0ull ? first : static_cast<int32_t>(top.e[s + 3] & 0xFFFFFFFFull);
+        *reinterpret_cast<int4*>(row + s) = v;
+    }
+}
+
+template <int K, bool kLearned>
+__global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __restrict__ pts,
+                                                             const TravNode* __restrict__ nodes,
+                                                             const int32_t* __restrict__ d_counts, int64_t q_start,
+                                                             int64_t n_q, const float* __restrict__ radius2,
+                                                             int bucket, int32_t* __restrict__ out,
+                                                             int32_t* __restrict__ queue, int32_t* queue_count) {
+    __shared__ int s_front[kTileWaves][2][kFrontCap];
+    __shared__ int2 s_range[kTileWaves][kRangeCap];
+    const int w = static_cast<int>(threadIdx.x) / kWave;
+    const int lane = static_cast<int>(lane_id());
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const int U = d_counts[0];
+    int64_t q_end = q_start + n_q;
+    if (q_end > U) q_end = U;
+    const int64_t tile0 = q_start + (static_cast<int64_t>(blockIdx.x) * kTileWaves + w) * kWave;
+    if (tile0 >= q_end) return;  // whole wave idle (no block barriers below)
+    const int64_t qi = tile0 + lane;
+    const bool valid = qi < q_end;
+    const float4 q = valid ? pts[qi] : make_float4(0.f, 0.f, 0.f, 0.f);
+    TopK<K> top;
+    top.init();
+
+    // ---- per-lane radius
+    int wlo = 0, whi = -1;
+    uint32_t r2bits;
+    if (!kLearned) {
+        wlo = static_cast<int>(tile0 - kWindow > 0 ? tile0 - kWindow : 0);
+        whi = static_cast<int>(tile0 + kWave - 1 + kWindow < U - 1 ? tile0 + kWave - 1 + kWindow : U - 1);
+        for (int j = wlo; j <= whi; ++j) {
+            const float4 p = pts[j];
+            const float d = dist2(q.x, q.y, q.z, p.x, p.y, p.z);
+            const uint64_t key = (static_cast<uint64_t>(__float_as_uint(d)) << 32) | static_cast<uint32_t>(j);
+            if (key < top.e[K - 1]) top.insert(key);
+        }
+        r2bits = top.kth_bits();  // 0xFFFFFFFF when the window holds fewer than K points
+    } else {
+        const float r2 = valid ? radius2[qi - q_start] : -1.0f;
+        r2bits = (r2 >= 0.0f && r2 < INFINITY) ? __float_as_uint(r2) : 0xFFFFFFFFu;
+    }
+    const bool finite = valid && r2bits < 0x7F800000u;
+    const float r = finite ? sqrtf(__uint_as_float(r2bits)) : 0.0f;
+    const float n_fin = wave_sum(finite ? 1.0f : 0.0f);
+    const float mean_r = n_fin > 0.0f ? wave_sum(r) / n_fin : 0.0f;
+    const bool join = finite && r <= 2.0f * mean_r;
+    // padded radius: fl(rp * rp) > r2; each box bound is stepped one ulp
+    // outward so it bounds the exact q -+ rp
+    const float rp = r * 1.001f + 1e-19f;
+    const float ex0 = wave_min(join ? nextafterf(q.x - rp, -INFINITY) : INFINITY);
+    const float ey0 = wave_min(join ? nextafterf(q.y - rp, -INFINITY) : INFINITY);
+    const float ez0 = wave_min(join ? nextafterf(q.z - rp, -INFINITY) : INFINITY);
+    const float ex1 = wave_max(join ? nextafterf(q.x + rp, INFINITY) : -INFINITY);
+    const float ey1 = wave_max(join ? nextafterf(q.y + rp, INFINITY) : -INFINITY);
+    const float ez1 = wave_max(join ? nextafterf(q.z + rp, INFINITY) : -INFINITY);
+    const bool any_join = __ballot(join) != 0ull;
+
+    // ---- breadth-first range query for E
+    bool overflow = false;
+    int nr = 0;
+    if (any_join) {
+        int* cur = s_front[w][0];
+        int* nxt = s_front[w][1];
+        int fcount = 0;
+        if (U <= bucket) {
+            if (lane == 0) s_range[w][0] = make_int2(0, U - 1);
+            nr = 1;
+        } else {
+            if (lane == 0) cur[0] = 0;
+            fcount = 1;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        while (fcount > 0 && !overflow) {
+            int ncount = 0;
+            for (int base = 0; base < fcount; base += kWave) {
+                const int idx = base + lane;
+                bool nodeL = false, nodeR = false, rangeL = false, rangeR = false;
+                int a = 0, g = 0, b = 0;
+                if (idx < fcount) {
+                    const TravNode n = nodes[cur[idx]];
+                    a = w_int(n.lo_l.w);
+                    g = w_int(n.hi_l.w);
+                    b = w_int(n.lo_r.w);
+                    if (box_meets(n.lo_l, n.hi_l, ex0, ey0, ez0, ex1, ey1, ez1)) {
+                        if (g - a < bucket) rangeL = true; else nodeL = true;
+                    }
+                    if (box_meets(n.lo_r, n.hi_r, ex0, ey0, ez0, ex1, ey1, ez1)) {
+                        if (b - g - 1 < bucket) rangeR = true; else nodeR = true;
+                    }
+                }
+                const uint64_t mnl = __ballot(nodeL), mnr = __ballot(nodeR);
+                const uint64_t mrl = __ballot(rangeL), mrr = __ballot(rangeR);
+                const int pnl = ncount + __popcll(mnl & lt);
+                const int pnr = ncount + __popcll(mnl) + __popcll(mnr & lt);
+                const int prl = nr + __popcll(mrl & lt);
+                const int prr = nr + __popcll(mrl) + __popcll(mrr & lt);
+                if (nodeL && pnl < kFrontCap) nxt[pnl] = g;
+                if (nodeR && pnr < kFrontCap) nxt[pnr] = g + 1;
+                if (rangeL && prl < kRangeCap) s_range[w][prl] = make_int2(a, g);
+                if (rangeR && prr < kRangeCap) s_range[w][prr] = make_int2(g + 1, b);
+                ncount += __popcll(mnl) + __popcll(mnr);
+                nr += __popcll(mrl) + __popcll(mrr);
+            }
+            overflow = ncount > kFrontCap || nr > kRangeCap;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            int* t = cur;
+            cur = nxt;
+            nxt = t;
+            fcount = ncount;
+        }
+    }
+
+    // ---- candidate scan
+    if (any_join && !overflow) {
+        for (int ri = 0; ri < nr; ++ri) {
+            const int2 rg = s_range[w][ri];
+            int lo = uniform(rg.x);
+            const int hi = uniform(rg.y);
+            for (int j = lo; j <= hi; ++j) {
+                if (!kLearned && j >= wlo && j <= whi) continue;  // already seeded from the window
+                const float4 p = pts[j];
+                const float d = dist2(q.x, q.y, q.z, p.x, p.y, p.z);
+                const uint32_t db = __float_as_uint(d);
+                const uint64_t key = (static_cast<uint64_t>(db) << 32) | static_cast<uint32_t>(j);
+                if (join && (kLearned ? db <= r2bits : true) && key < top.e[K - 1]) top.insert(key);
+            }
+        }
+    }
+    const bool exact = join && !overflow && (kLearned ? top.kth_bits() <= r2bits : true);
+    if (valid && exact) write_row<K>(out + (qi - q_start) * K, top);
+    const bool defer = valid && !exact;
+    const uint64_t md = __ballot(defer);
+    if (md) {
+        int slot0 = 0;
+        if (lane == __ffsll(static_cast<unsigned long long>(md)) - 1) slot0 = atomicAdd(queue_count, __popcll(md));
+        slot0 = __shfl(slot0, __ffsll(static_cast<unsigned long long>(md)) - 1, kWave);
+        if (defer) queue[slot0 + __popcll(md & lt)] = static_cast<int32_t>(qi);
+    }
+}
+
+// Queued queries, one lane each (plain, or seeded with their radius).
+template <int K>
+__global__ void __launch_bounds__(kKnnBlock) knn_queue_kernel(const float4* __restrict__ pts,
+                                                              const TravNode* __restrict__ nodes,
+                                                              const int32_t* __restrict__ d_counts, int64_t q_start,
+                                                              const int32_t* __restrict__ queue,
+                                                              const int32_t* __restrict__ queue_count,
+                                                              const float* __restrict__ radius2, int bucket,
+                                                              int32_t* __restrict__ out, int32_t* deferred_total) {
+    const int U = d_counts[0];
+    const int n = *queue_count;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && deferred_total) atomicAdd(deferred_total, n);
+    for (int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); t < n;
+         t += static_cast<int>(gridDim.x * blockDim.x)) {
+        const int qi = queue[t];
+        const float4 q = pts[qi];
+        TopK<K> top;
+        top.init();
+        uint32_t r2bits = 0xFFFFFFFFu;
+        if (radius2) {
+            const float r2 = radius2[qi - q_start];
+            r2bits = (r2 >= 0.0f) ? __float_as_uint(r2) : 0xFFFFFFFFu;
+        }
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            knn_search<K>(pts, nodes, U, q.x, q.y, q.z, r2bits, bucket, top);
+            if (r2bits == 0xFFFFFFFFu || top.kth_bits() <= r2bits) break;
+            top.init();
+            r2bits = 0xFFFFFFFFu;
+        }
+        write_row<K>(out + (qi - q_start) * K, top);
+    }
+}
+
+template <int K>
+int knn_tiles_k(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
+                const float* radius2, int bucket, int32_t* out, int32_t* queue, int32_t* queue_count,
+                int32_t* deferred_total, hipStream_t st) {
+    const int64_t tiles = (n_q + kWave - 1) / kWave;
+    const unsigned g = static_cast<unsigned>((tiles + kTileWaves - 1) / kTileWaves);
+    if (radius2)
+        knn_tile_kernel<K, true><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out,
+                                                         queue, queue_count);
+    else
+        knn_tile_kernel<K, false><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, nullptr, bucket, out,
+                                                          queue, queue_count);
+    SX_LAUNCH_CHECK("knn_tile_kernel");
+    unsigned gq = grid_for(n_q, kKnnBlock);
+    if (gq > 2048) gq = 2048;
+    knn_queue_kernel<K><<<gq, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, queue, queue_count, radius2, bucket,
+                                                 out, deferred_total);
+    SX_LAUNCH_CHECK("knn_queue_kernel");
+    return SX_OK;
+}
+
+// Index queries through the tiles.  queue: n_q int32; queue_count: one int32,
+// cleared here; deferred_total accumulates the number of queued queries.
+int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
+                     int k, const float* radius2, int bucket, int32_t* out, int32_t* queue, int32_t* queue_count,
+                     int32_t* deferred_total, hipStream_t st) {
+    if (n_q <= 0) return SX_OK;
+    if (reinterpret_cast<uintptr_t>(out) % 16 != 0) return fail(SX_EINVAL, "knn: output must be 16-byte aligned");
+    SX_TRY(fill_i32(queue_count, 0, 1, st));
+    switch (k) {
+        case 8: return knn_tiles_k<8>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st);
+        case 16: return knn_tiles_k<16>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st);
+        case 24: return knn_tiles_k<24>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st);
+        case 32: return knn_tiles_k<32>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st);
+        default: return fail(SX_EINVAL, "knn: k must be 8, 16, 24 or 32");
+    }
+}
+
 }  // namespace sx

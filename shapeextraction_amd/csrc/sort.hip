@@ -21,6 +21,7 @@
 
 namespace sx {
 
+
 constexpr int kSortBlock = 256;
 constexpr int kSortWaves = kSortBlock / kWave;
 constexpr int kSortItems = 8;
@@ -120,7 +121,198 @@ struct HistOut {
 
 inline int64_t sort_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
 
-size_t sort_ws_bytes(int64_t n) {
+
+// ---------------------------------------------------------------------------
+// Onesweep (Adinets & Merrill, "Onesweep: A Faster Least Significant Digit
+// Radix Sort for GPUs", 2022), written for wave64:
+//   onesweep_hist_kernel   one pass over the keys: histograms of every digit
+//                          position at once; onesweep_base_kernel scans them
+//   onesweep_pass_kernel   per digit position, one kernel: tiles take a
+//                          ticket (so every tile a tile waits on is already
+//                          running), rank their keys (wave ballots), publish
+//                          per-digit counts, look back over earlier tiles for
+//                          the exclusive prefix, stage the tile in LDS in digit
+//                          order and write it out in coalesced runs.
+// Look-back words pack a 2-bit flag (1 = tile count, 2 = inclusive prefix)
+// with a 30-bit count, so one agent-scope atomic load/store carries both and
+// no fence is needed.  Requires n < 2^30.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kFlagAgg = 1u << 30, kFlagPrefix = 2u << 30, kCountMask = (1u << 30) - 1;
+constexpr int kMaxPasses = 8;
+
+__global__ void __launch_bounds__(kSortBlock) onesweep_hist_kernel(const uint64_t* __restrict__ keys, int64_t n,
+                                                                   int passes, uint32_t* __restrict__ ghist) {
+    __shared__ uint32_t h[kMaxPasses][kRadix];
+    for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kSortBlock) (&h[0][0])[i] = 0;
+    __syncthreads();
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kSortBlock;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kSortBlock + threadIdx.x; i < n; i += stride) {
+        const uint64_t k = keys[i];
+        for (int p = 0; p < passes; ++p) atomicAdd(&h[p][static_cast<uint32_t>(k >> (8 * p)) & (kRadix - 1)], 1u);
+    }
+    __syncthreads();
+    for (int p = 0; p < passes; ++p) {
+        const uint32_t c = h[p][threadIdx.x];
+        if (c) atomicAdd(&ghist[p * kRadix + threadIdx.x], c);
+    }
+}
+
+// One block per pass: exclusive scan of the 256 digit counts in place.
+__global__ void __launch_bounds__(kRadix) onesweep_base_kernel(uint32_t* ghist) {
+    uint32_t* h = ghist + blockIdx.x * kRadix;
+    const int v = static_cast<int>(h[threadIdx.x]);
+    int total;
+    const int ex = block_excl_sum(v, &total);
+    h[threadIdx.x] = static_cast<uint32_t>(ex);
+}
+
+__global__ void __launch_bounds__(kSortBlock) onesweep_pass_kernel(const uint64_t* __restrict__ keys_in,
+                                                                   const uint32_t* __restrict__ vals_in, int64_t n,
+                                                                   int shift, const uint32_t* __restrict__ gbase,
+                                                                   uint32_t* status, uint32_t* tile_counter,
+                                                                   uint64_t* __restrict__ keys_out,
+                                                                   uint32_t* __restrict__ vals_out) {
+    __shared__ uint32_t cnt[kSortWaves][kRadix];
+    __shared__ uint32_t tile_start[kRadix];
+    __shared__ uint32_t dest_base[kRadix];
+    __shared__ uint64_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint32_t s_tile;
+    const int tid = static_cast<int>(threadIdx.x);
+    const int w = tid / kWave;
+    const unsigned lane = lane_id();
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    for (int d = static_cast<int>(lane); d < kRadix; d += kWave) cnt[w][d] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const int64_t tile_base = static_cast<int64_t>(tile) * kSortTile;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const int64_t span = tile_base + static_cast<int64_t>(w) * kWaveSpan;
+    uint64_t k[kSortItems];
+    uint32_t v[kSortItems];
+    uint32_t rank[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int64_t i = span + r * kWave + lane;
+        const bool valid = i < n;
+        k[r] = valid ? keys_in[i] : ~0ull;
+        v[r] = valid ? (vals_in ? vals_in[i] : static_cast<uint32_t>(i)) : 0u;
+        const uint32_t d = static_cast<uint32_t>(k[r] >> shift) & (kRadix - 1);
+        const uint64_t m = match_digit(d);
+        const uint32_t c = cnt[w][d];
+        rank[r] = c + static_cast<uint32_t>(__popcll(m & lt));
+        if (((m >> lane) >> 1) == 0) cnt[w][d] = c + static_cast<uint32_t>(__popcll(m));
+    }
+    __syncthreads();
+    // per digit (thread tid = digit): wave offsets, tile count, look-back
+    uint32_t run = 0;
+#pragma unroll
+    for (int wv = 0; wv < kSortWaves; ++wv) {
+        const uint32_t c = cnt[wv][tid];
+        cnt[wv][tid] = run;
+        run += c;
+    }
+    uint32_t* my = status + static_cast<int64_t>(tile) * kRadix + tid;
+    uint32_t excl = 0;
+    if (tile == 0) {
+        __hip_atomic_store(my, kFlagPrefix | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(my, kFlagAgg | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int64_t t = static_cast<int64_t>(tile) - 1;
+        uint32_t spins = 0;
+        while (true) {
+            const uint32_t s = __hip_atomic_load(status + t * kRadix + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((s & ~kCountMask) == 0) {
+                // Tile t holds an earlier ticket, so it is running and will
+                // publish; the bound only guards against a broken invariant.
+                if (++spins > (1u << 22)) {
+                    atomicOr(tile_counter + kMaxPasses, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += s & kCountMask;
+            if ((s & ~kCountMask) == kFlagPrefix) break;
+            --t;
+        }
+        __hip_atomic_store(my, kFlagPrefix | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    int total;
+    const uint32_t start = static_cast<uint32_t>(block_excl_sum(static_cast<int>(run), &total));
+    tile_start[tid] = start;
+    dest_base[tid] = gbase[tid] + excl - start;  // output index = dest_base[d] + LDS slot
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const uint32_t d = static_cast<uint32_t>(k[r] >> shift) & (kRadix - 1);
+        const uint32_t pos = tile_start[d] + cnt[w][d] + rank[r];
+        s_keys[pos] = k[r];
+        s_vals[pos] = v[r];
+    }
+    __syncthreads();
+    const int64_t valid = n - tile_base < kSortTile ? n - tile_base : kSortTile;
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int j = r * kSortBlock + tid;
+        if (j < valid) {
+            const uint64_t key = s_keys[j];
+            const uint32_t d = static_cast<uint32_t>(key >> shift) & (kRadix - 1);
+            const uint32_t o = dest_base[d] + static_cast<uint32_t>(j);
+            keys_out[o] = key;
+            vals_out[o] = s_vals[j];
+        }
+    }
+}
+
+size_t onesweep_ws_bytes(int64_t n, int passes) {
+    const int64_t tiles = sort_tiles(n);
+    Carver c(nullptr);
+    c.take<uint64_t>(static_cast<size_t>(n));
+    c.take<uint32_t>(static_cast<size_t>(n));
+    c.take<uint32_t>(kMaxPasses * kRadix);
+    c.take<uint32_t>(kMaxPasses + 1);  // per-pass tile tickets + a look-back error flag
+    c.take<uint32_t>(static_cast<size_t>(passes) * kRadix * tiles);
+    return c.off;
+}
+
+int onesweep_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n, int key_bits, uint64_t* keys_out,
+                    uint32_t* vals_out, void* ws, hipStream_t st) {
+    const int passes = (key_bits + 7) / 8;
+    const int64_t tiles = sort_tiles(n);
+    Carver c(ws);
+    uint64_t* keys_tmp = c.take<uint64_t>(static_cast<size_t>(n));
+    uint32_t* vals_tmp = c.take<uint32_t>(static_cast<size_t>(n));
+    uint32_t* ghist = c.take<uint32_t>(kMaxPasses * kRadix);
+    uint32_t* counters = c.take<uint32_t>(kMaxPasses + 1);
+    uint32_t* status = c.take<uint32_t>(static_cast<size_t>(passes) * kRadix * tiles);
+    // ghist, counters and status are contiguous: clear them in one fill
+    const size_t clear = reinterpret_cast<char*>(status + static_cast<size_t>(passes) * kRadix * tiles) -
+                         reinterpret_cast<char*>(ghist);
+    SX_TRY(fill_u8(ghist, 0, clear, st));
+    unsigned hb = grid_for(n, kSortBlock * 16);
+    if (hb > 2048) hb = 2048;
+    onesweep_hist_kernel<<<hb, kSortBlock, 0, st>>>(keys_in, n, passes, ghist);
+    SX_LAUNCH_CHECK("onesweep_hist_kernel");
+    onesweep_base_kernel<<<passes, kRadix, 0, st>>>(ghist);
+    SX_LAUNCH_CHECK("onesweep_base_kernel");
+    const uint64_t* src_k = keys_in;
+    const uint32_t* src_v = vals_in;
+    for (int p = 0; p < passes; ++p) {
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint64_t* dst_k = to_out ? keys_out : keys_tmp;
+        uint32_t* dst_v = to_out ? vals_out : vals_tmp;
+        onesweep_pass_kernel<<<static_cast<unsigned>(tiles), kSortBlock, 0, st>>>(
+            src_k, src_v, n, 8 * p, ghist + p * kRadix, status + static_cast<size_t>(p) * kRadix * tiles,
+            counters + p, dst_k, dst_v);
+        SX_LAUNCH_CHECK("onesweep_pass_kernel");
+        src_k = dst_k;
+        src_v = dst_v;
+    }
+    return SX_OK;
+}
+
+size_t sort_ws_bytes_3k(int64_t n) {
     const int64_t tiles = sort_tiles(n);
     Carver c(nullptr);
     c.take<uint64_t>(static_cast<size_t>(n));
@@ -130,8 +322,8 @@ size_t sort_ws_bytes(int64_t n) {
     return c.off;
 }
 
-int sort_pairs_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n, int key_bits, uint64_t* keys_out,
-                      uint32_t* vals_out, void* ws, hipStream_t st) {
+int sort_3k_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n, int key_bits, uint64_t* keys_out,
+                   uint32_t* vals_out, void* ws, hipStream_t st) {
     if (n <= 0) return SX_OK;
     if (n >= (1ll << 31)) return fail(SX_EINVAL, "sort: n must be below 2^31");
     const int64_t tiles = sort_tiles(n);
@@ -161,6 +353,21 @@ int sort_pairs_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t 
     return SX_OK;
 }
 
+
+// Onesweep below 2^30 keys unless the three-kernel passes (histogram, scan,
+// scatter) are asked for; those also serve larger inputs.
+size_t sort_ws_bytes(int64_t n) {
+    return n < (1ll << 30) ? onesweep_ws_bytes(n, kMaxPasses) : sort_ws_bytes_3k(n);
+}
+
+int sort_pairs_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n, int key_bits, uint64_t* keys_out,
+                      uint32_t* vals_out, void* ws, int algo, hipStream_t st) {
+    if (n <= 0) return SX_OK;
+    if (n < (1ll << 30) && algo == SX_SORT_ONESWEEP)
+        return onesweep_launch(keys_in, vals_in, n, key_bits, keys_out, vals_out, ws, st);
+    return sort_3k_launch(keys_in, vals_in, n, key_bits, keys_out, vals_out, ws, st);
+}
+
 }  // namespace sx
 
 using sx::fail;
@@ -179,5 +386,6 @@ extern "C" int sx_sort_pairs(const uint64_t* d_keys_in, const uint32_t* d_vals_i
     if (n > 0 && (!d_keys_in || !d_keys_out || !d_vals_out)) return fail(SX_EINVAL, "sx_sort_pairs: null pointer");
     if (d_keys_in == d_keys_out || (d_vals_in && d_vals_in == d_vals_out))
         return fail(SX_EINVAL, "sx_sort_pairs: input and output buffers must differ");
-    return sx::sort_pairs_launch(d_keys_in, d_vals_in, n, key_bits, d_keys_out, d_vals_out, ws, sx::as_stream(stream));
+    return sx::sort_pairs_launch(d_keys_in, d_vals_in, n, key_bits, d_keys_out, d_vals_out, ws, SX_SORT_ONESWEEP,
+                                 sx::as_stream(stream));
 }

@@ -2,11 +2,15 @@
 // bit interleave, the Karras prefix length, and the two distance functions
 // whose rounding must match the host (CPU) restatement bit for bit.
 //
-// Every float operation on a bit-exact path goes through __fadd_rn / __fsub_rn
-// / __fmul_rn / __fdiv_rn, which lower to single IEEE round-to-nearest-even
-// instructions and can never be contracted into an FMA, whatever -ffp-contract
-// says.  The host restatement uses the same operation order.
+// Every float operation on a bit-exact path is a single IEEE
+// round-to-nearest-even operation written in an explicit order (__fadd_rn /
+// __fsub_rn / __fmul_rn / __fdiv_rn, plain operators in this HIP version), and
+// no pair may be contracted into an FMA: the build passes -ffp-contract=off
+// and the pragma below repeats it for every file including this header.  The
+// host restatement uses the same operation order.
 #pragma once
+
+#pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
 #include <cstdint>

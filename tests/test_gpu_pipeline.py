@@ -267,10 +267,43 @@ def test_knn_radius_seed_is_exact(gpu):
         got = ix.knn(k, mode=gpu.SX_KNN_RADIUS, radius2=gpu.DeviceArray.from_numpy(r2.astype(np.float32))).numpy()
         assert np.array_equal(got, plain), scale
         reruns = ix.knn_reruns()
+        print(f"[radius] scale {scale}: {reruns / len(kth):.3%} searched again")
         if scale == 0.25:
             assert reruns > 0.9 * len(kth)
         if scale is not None and scale >= 1.0:
-            assert reruns == 0
+            assert reruns < 0.05 * len(kth)
+
+
+@pytest.mark.parametrize("kind,seed,n,k", [("scene", 2, 1_000_000, 16), ("gmm", 3, 600_000, 32),
+                                            ("degenerate", 5, 400_000, 24), ("sphere", 1, 100_000, 8)])
+def test_knn_algorithms_agree(gpu, kind, seed, n, k):
+    """Tiles (+ queue) and one-lane-per-query give identical rows, plain and seeded."""
+    x, y, z = _workload(gpu, kind, seed, n)
+    ix = gpu.Index(n, 63).build(x, y, z)
+    tiles = ix.knn(k).numpy()
+    deferred = ix.knn_reruns()
+    ix.set_algorithms(knn=gpu.SX_KNN_ALGO_LANE)
+    lane = ix.knn(k).numpy()
+    assert np.array_equal(tiles, lane)
+    pts = ix.fetch()["points"]
+    kth = np.sum((pts[lane[:, -1]] - pts) ** 2, axis=1).astype(np.float32)
+    r2 = gpu.DeviceArray.from_numpy((kth * rng(seed).uniform(0.3, 3.0, len(kth))).astype(np.float32))
+    seeded_lane = ix.knn(k, mode=gpu.SX_KNN_RADIUS, radius2=r2).numpy()
+    ix.set_algorithms(knn=gpu.SX_KNN_ALGO_TILES)
+    seeded_tiles = ix.knn(k, mode=gpu.SX_KNN_RADIUS, radius2=r2).numpy()
+    assert np.array_equal(seeded_lane, lane) and np.array_equal(seeded_tiles, lane)
+    print(f"[knn] {kind}: plain tiles deferred {deferred / len(lane):.3%} to one lane per query")
+
+
+def test_sort_algorithms_agree(gpu):
+    x, y, z = gpu.gen_scene(2, 2_000_000)
+    a = gpu.Index(len(x), 63).build(x, y, z).fetch()
+    ixb = gpu.Index(len(x), 63)
+    ixb.set_algorithms(sort=gpu.SX_SORT_THREE_KERNEL)
+    b = ixb.build(x, y, z).fetch()
+    for key in ("codes", "sorted_perm", "rep", "points"):
+        assert np.array_equal(a[key], b[key]), key
+    assert a["oct_nodes"].tobytes() == b["oct_nodes"].tobytes()
 
 
 # ---------------------------------------------------------------- normals
