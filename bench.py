@@ -78,6 +78,8 @@ def main() -> int:
     ap.add_argument("--knn", choices=["plain", "learned", "both"], default="both",
                     help="learned/both use shapeextraction_amd/models/radius_k{k}.bin (trained here if missing)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run three device-to-device copies of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -92,6 +94,15 @@ def main() -> int:
         return 1
     dev = sx.device_name(0)
     log(f"device {dev}")
+    if args.copy_probe:
+        nb = args.copy_probe << 20
+        src, dst = sx.DeviceArray((nb,), np.uint8), sx.DeviceArray((nb,), np.uint8)
+        src.fill_(1)
+        for _ in range(3):
+            sx._lib.call("sx_memcpy_dtod", dst.ptr, src.ptr, nb, None)
+        sx.synchronize()
+        log(f"copied {nb} bytes x3")
+        return 0
     n, k = args.n, args.k
     t = time.perf_counter()
     x, y, z = sx.gen_scene(2, n)

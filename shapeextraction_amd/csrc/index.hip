@@ -49,6 +49,7 @@ int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts
 int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                      int k, const float* radius2, int bucket, int32_t* out, int32_t* queue, int32_t* queue_count,
                      int32_t* deferred_total, hipStream_t st);
+int scatter_i32_launch(const int32_t* src, const uint32_t* idx, int64_t n, int32_t* dst, hipStream_t st);
 int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn, int64_t q_start, int64_t n_q, int k,
                    int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st);
 }  // namespace sx
@@ -75,6 +76,7 @@ struct sx_index {
     uint64_t* ucodes = nullptr;
     uint32_t* perm = nullptr;
     uint32_t* rep = nullptr;
+    int32_t* uos = nullptr;
     int32_t* uoi = nullptr;
     float* pts12 = nullptr;
     float4* pts16 = nullptr;
@@ -113,6 +115,7 @@ size_t index_layout(sx_index* ix, void* base) {
     ix->ucodes = c.take<uint64_t>(n);
     ix->perm = c.take<uint32_t>(n);
     ix->rep = c.take<uint32_t>(n);
+    ix->uos = c.take<int32_t>(n);
     ix->uoi = c.take<int32_t>(n);
     ix->pts12 = c.take<float>(3 * n);
     ix->pts16 = c.take<float4>(n);
@@ -246,7 +249,7 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
     SX_TRY(sx::bbox_launch(d_x, d_y, d_z, n, ix->bounds, ix->counts + kCntBad, ix->ws_bbox, false, st));
     SX_TRY(sx::encode_launch(d_x, d_y, d_z, n, ix->bounds, cb, ix->codes_in, st));
     SX_TRY(sx::sort_pairs_launch(ix->codes_in, nullptr, n, cb, ix->codes_sorted, ix->perm, ix->ws, ix->sort_algo, st));
-    SX_TRY(sx::unique_launch(ix->codes_sorted, ix->perm, n, ix->ucodes, ix->rep, nullptr, ix->uoi,
+    SX_TRY(sx::unique_launch(ix->codes_sorted, ix->perm, n, ix->ucodes, ix->rep, ix->uos, nullptr,
                              ix->counts + kCntUnique, ix->ws, st));
     SX_TRY(sx::decode_launch(ix->ucodes, ix->counts + kCntUnique, n, ix->bounds, cb, ix->pts12, ix->pts16, st));
     if (ix->root_mode == SX_ROOT_TEXTBOOK) {
@@ -269,6 +272,8 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
 int sx_index_get_view(sx_index* ix, sx_index_view* view, void* stream) {
     SX_TRY(check_index(ix));
     if (!view) return fail(SX_EINVAL, "sx_index_get_view: null view");
+    // the input-order map is only built when a caller asks for the view
+    SX_TRY(sx::scatter_i32_launch(ix->uos, ix->perm, ix->n_input, ix->uoi, sx::as_stream(stream)));
     int64_t nu = 0, no = 0;
     SX_TRY(sx_check_counts(ix->counts + kCntUnique, ix->n_input, ix->counts + kCntOct, ix->oct_cap,
                            ix->counts + kCntBad, &nu, &no, stream));

@@ -1,4 +1,9 @@
-// Exact k-nearest-neighbour search, one query per lane.
+// Exact k-nearest-neighbour search.  Three kernels share one result format:
+//   knn_kernel       one query per lane, stackless depth-first descent (below);
+//   knn_tile_kernel  64 Morton-consecutive queries per wave share one range
+//                    query and one candidate stream (the default for index
+//                    queries);
+//   knn_wave_kernel  one wave per query, for the queries a tile defers.
 //
 // Replaces knnSearchKernel<k> and its two per-thread min-max heaps in local
 // memory (src/Octree.hpp:136-193, src/dequeue.hpp).  The reference search
@@ -232,11 +237,12 @@ int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts
 // balls, collecting the leaf ranges of every subtree whose box meets E.  E is
 // widened with directed rounding so that every point with dist2 <= r2 of a
 // joining lane lies inside it (same monotonicity argument as box_dist2).
-// Candidate points are then streamed with wave-uniform loads while every lane
-// updates its own list.  A lane's list is exact when (plain) it joined the
-// search, or (learned) it also found at least K points within its radius.
-// Other lanes - and whole tiles whose frontier or range list overflows LDS -
-// are appended to a queue that knn_queue_kernel searches one lane per query.
+// Candidate points are then loaded 64 at a time, one per lane (coalesced);
+// those inside E are broadcast lane by lane while every lane updates its own
+// list.  A lane's list is exact when (plain) it joined the search, or
+// (learned) it also found at least K points within its radius.  Other lanes
+// - and whole tiles whose frontier or range list overflows LDS - are appended
+// to a queue that knn_wave_kernel searches, one wave per query.
 // ---------------------------------------------------------------------------
 constexpr int kTileWaves = kKnnBlock / kWave;
 constexpr int kFrontCap = 128;
@@ -263,6 +269,41 @@ __device__ __forceinline__ bool box_meets(float4 lo, float4 hi, float ex0, float
     return hi.x >= ex0 && lo.x <= ex1 && hi.y >= ey0 && lo.y <= ey1 && hi.z >= ez0 && lo.z <= ez1;
 }
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float read_lane(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ uint64_t read_lane64(uint64_t v, int src) {
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), src));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), src));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+    const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(v), src, kWave));
+    const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(v >> 32), src, kWave));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl64_xor(uint64_t v, int mask) {
+    const uint32_t lo = static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), mask, kWave));
+    const uint32_t hi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(v >> 32), mask, kWave));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+// Ascending bitonic sort of one 64-bit key per lane across the wave.
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t v) {
+    const int lane = static_cast<int>(lane_id());
+#pragma unroll
+    for (int k = 2; k <= kWave; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint64_t o = shfl64_xor(v, j);
+            const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+            v = keep_min ? (v < o ? v : o) : (v < o ? o : v);
+        }
+    }
+    return v;
+}
+__device__ __forceinline__ uint64_t make_key(float d, int j) {
+    return (static_cast<uint64_t>(__float_as_uint(d)) << 32) | static_cast<uint32_t>(j);
+}
 
 template <int K>
 __device__ __forceinline__ void write_row(int32_t* row, const TopK<K>& top) {
@@ -278,6 +319,32 @@ __device__ __forceinline__ void write_row(int32_t* row, const TopK<K>& top) {
     }
 }
 
+// Flattened position f (< total) of a list of inclusive index ranges with
+// inclusive size prefix `pre` -> point index.
+__device__ __forceinline__ int range_point(const int2* rng, const int* pre, int nr, int f) {
+    int lo = 0, hi = nr - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] > f) hi = mid; else lo = mid + 1;
+    }
+    return rng[lo].y - (pre[lo] - 1 - f);
+}
+
+// Inclusive size prefix of nr ranges into pre; returns the total (uniform).
+__device__ __forceinline__ int range_prefix(const int2* rng, int* pre, int nr) {
+    const int lane = static_cast<int>(lane_id());
+    int total = 0;
+    for (int base = 0; base < nr; base += kWave) {
+        const int idx = base + lane;
+        const int2 rg = idx < nr ? rng[idx] : make_int2(0, -1);
+        const int incl = total + wave_incl_sum(rg.y - rg.x + 1);
+        if (idx < nr) pre[idx] = incl;
+        total = __builtin_amdgcn_readlane(incl, kWave - 1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    return total;
+}
+
 template <int K, bool kLearned>
 __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __restrict__ pts,
                                                              const TravNode* __restrict__ nodes,
@@ -287,6 +354,7 @@ __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __res
                                                              int32_t* __restrict__ queue, int32_t* queue_count) {
     __shared__ int s_front[kTileWaves][2][kFrontCap];
     __shared__ int2 s_range[kTileWaves][kRangeCap];
+    static_assert(2 * kFrontCap >= kRangeCap, "the range prefix reuses the frontier buffers");
     const int w = static_cast<int>(threadIdx.x) / kWave;
     const int lane = static_cast<int>(lane_id());
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -307,11 +375,15 @@ __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __res
     if (!kLearned) {
         wlo = static_cast<int>(tile0 - kWindow > 0 ? tile0 - kWindow : 0);
         whi = static_cast<int>(tile0 + kWave - 1 + kWindow < U - 1 ? tile0 + kWave - 1 + kWindow : U - 1);
-        for (int j = wlo; j <= whi; ++j) {
-            const float4 p = pts[j];
-            const float d = dist2(q.x, q.y, q.z, p.x, p.y, p.z);
-            const uint64_t key = (static_cast<uint64_t>(__float_as_uint(d)) << 32) | static_cast<uint32_t>(j);
-            if (key < top.e[K - 1]) top.insert(key);
+        for (int base = wlo; base <= whi; base += kWave) {
+            const int j = base + lane;
+            const float4 p = j <= whi ? pts[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const int cnt = whi - base + 1 < kWave ? whi - base + 1 : kWave;
+            for (int src = 0; src < cnt; ++src) {
+                const float d = dist2(q.x, q.y, q.z, read_lane(p.x, src), read_lane(p.y, src), read_lane(p.z, src));
+                const uint64_t key = make_key(d, base + src);
+                if (key < top.e[K - 1]) top.insert(key);
+            }
         }
         r2bits = top.kth_bits();  // 0xFFFFFFFF when the window holds fewer than K points
     } else {
@@ -389,18 +461,26 @@ __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __res
         }
     }
 
-    // ---- candidate scan
+    // ---- candidate stream: 64 candidates per load, broadcast lane by lane
     if (any_join && !overflow) {
-        for (int ri = 0; ri < nr; ++ri) {
-            const int2 rg = s_range[w][ri];
-            int lo = uniform(rg.x);
-            const int hi = uniform(rg.y);
-            for (int j = lo; j <= hi; ++j) {
-                if (!kLearned && j >= wlo && j <= whi) continue;  // already seeded from the window
-                const float4 p = pts[j];
-                const float d = dist2(q.x, q.y, q.z, p.x, p.y, p.z);
+        int* pre = s_front[w][0];
+        const int total = range_prefix(s_range[w], pre, nr);
+        for (int f0 = 0; f0 < total; f0 += kWave) {
+            const int f = f0 + lane;
+            int j = f < total ? range_point(s_range[w], pre, nr, f) : -1;
+            if (!kLearned && j >= wlo && j <= whi) j = -1;  // already seeded from the window
+            float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (j >= 0) p = pts[j];
+            const bool in_e = j >= 0 && p.x >= ex0 && p.x <= ex1 && p.y >= ey0 && p.y <= ey1 && p.z >= ez0 &&
+                              p.z <= ez1;
+            uint64_t m = __ballot(in_e);
+            while (m) {
+                const int src = static_cast<int>(__builtin_ctzll(m));
+                m &= m - 1ull;
+                const float d = dist2(q.x, q.y, q.z, read_lane(p.x, src), read_lane(p.y, src), read_lane(p.z, src));
                 const uint32_t db = __float_as_uint(d);
-                const uint64_t key = (static_cast<uint64_t>(db) << 32) | static_cast<uint32_t>(j);
+                const uint64_t key = (static_cast<uint64_t>(db) << 32) |
+                                     static_cast<uint32_t>(__builtin_amdgcn_readlane(j, src));
                 if (join && (kLearned ? db <= r2bits : true) && key < top.e[K - 1]) top.insert(key);
             }
         }
@@ -417,36 +497,149 @@ __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __res
     }
 }
 
-// Queued queries, one lane each (plain, or seeded with their radius).
+// ---------------------------------------------------------------------------
+// Queued queries, one wave each.  The list is seeded with the 64 index
+// points nearest in Morton order (wave-sorted; lane i holds the i-th key),
+// then subtrees are expanded 64 at a time from an LDS stack, pruned by the
+// current k-th distance (strict, as in knn_search), and their leaf ranges
+// streamed one point per lane.  A candidate that beats the k-th key is
+// inserted into the distributed list by a ballot for its rank and a shift.
+// A query whose stack would overflow is searched by lane 0 with knn_search.
+// ---------------------------------------------------------------------------
+constexpr int kStackCap = 512;
+constexpr int kWaveRanges = 2 * kWave;
+
 template <int K>
-__global__ void __launch_bounds__(kKnnBlock) knn_queue_kernel(const float4* __restrict__ pts,
-                                                              const TravNode* __restrict__ nodes,
-                                                              const int32_t* __restrict__ d_counts, int64_t q_start,
-                                                              const int32_t* __restrict__ queue,
-                                                              const int32_t* __restrict__ queue_count,
-                                                              const float* __restrict__ radius2, int bucket,
-                                                              int32_t* __restrict__ out, int32_t* deferred_total) {
+__global__ void __launch_bounds__(kKnnBlock) knn_wave_kernel(const float4* __restrict__ pts,
+                                                             const TravNode* __restrict__ nodes,
+                                                             const int32_t* __restrict__ d_counts, int64_t q_start,
+                                                             const int32_t* __restrict__ queue,
+                                                             const int32_t* __restrict__ queue_count, int bucket,
+                                                             int32_t* __restrict__ out, int32_t* deferred_total) {
+    __shared__ int2 s_stack[kTileWaves][kStackCap];  // (node, box distance bits)
+    __shared__ int2 s_rng[kTileWaves][kWaveRanges];
+    __shared__ int s_pre[kTileWaves][kWaveRanges];
+    const int w = static_cast<int>(threadIdx.x) / kWave;
+    const int lane = static_cast<int>(lane_id());
+    const uint64_t lt = (1ull << lane) - 1ull;
     const int U = d_counts[0];
     const int n = *queue_count;
     if (blockIdx.x == 0 && threadIdx.x == 0 && deferred_total) atomicAdd(deferred_total, n);
-    for (int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); t < n;
-         t += static_cast<int>(gridDim.x * blockDim.x)) {
-        const int qi = queue[t];
+    const int n_waves = static_cast<int>(gridDim.x) * kTileWaves;
+    for (int t = static_cast<int>(blockIdx.x) * kTileWaves + w; t < n; t += n_waves) {
+        const int qi = uniform(queue[t]);
         const float4 q = pts[qi];
-        TopK<K> top;
-        top.init();
-        uint32_t r2bits = 0xFFFFFFFFu;
-        if (radius2) {
-            const float r2 = radius2[qi - q_start];
-            r2bits = (r2 >= 0.0f) ? __float_as_uint(r2) : 0xFFFFFFFFu;
+        // seed: the 64 Morton neighbours of the query
+        int wbase = qi - kWave / 2;
+        if (wbase > U - kWave) wbase = U - kWave;
+        if (wbase < 0) wbase = 0;
+        const int wend = wbase + kWave - 1;
+        uint64_t key = ~0ull;
+        {
+            const int j = wbase + lane;
+            if (j < U) {
+                const float4 p = pts[j];
+                key = make_key(dist2(q.x, q.y, q.z, p.x, p.y, p.z), j);
+            }
         }
-        for (int attempt = 0; attempt < 2; ++attempt) {
-            knn_search<K>(pts, nodes, U, q.x, q.y, q.z, r2bits, bucket, top);
-            if (r2bits == 0xFFFFFFFFu || top.kth_bits() <= r2bits) break;
+        key = wave_sort64(key);
+        uint64_t kth = read_lane64(key, K - 1);
+
+        int sp = 0, nr = 0;
+        bool overflow = false;
+        if (U <= kWave) {
+            sp = 0;  // the seed window holds every point
+        } else if (U <= bucket) {
+            if (lane == 0) s_rng[w][0] = make_int2(0, U - 1);
+            nr = 1;
+        } else {
+            if (lane == 0) s_stack[w][0] = make_int2(0, 0);
+            sp = 1;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        while (sp > 0 || nr > 0) {
+            if (sp > 0) {
+                const int c = sp < kWave ? sp : kWave;
+                sp -= c;
+                const int2 e = lane < c ? s_stack[w][sp + lane] : make_int2(-1, -1);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                const uint32_t kd = static_cast<uint32_t>(kth >> 32);
+                bool nodeL = false, nodeR = false, rangeL = false, rangeR = false;
+                int a = 0, g = 0, b = 0;
+                uint32_t dlb = 0, drb = 0;
+                if (lane < c && static_cast<uint32_t>(e.y) <= kd) {
+                    const TravNode nd = nodes[e.x];
+                    a = w_int(nd.lo_l.w);
+                    g = w_int(nd.hi_l.w);
+                    b = w_int(nd.lo_r.w);
+                    dlb = __float_as_uint(box_dist2(q.x, q.y, q.z, nd.lo_l, nd.hi_l));
+                    drb = __float_as_uint(box_dist2(q.x, q.y, q.z, nd.lo_r, nd.hi_r));
+                    if (dlb <= kd) {
+                        if (g - a < bucket) rangeL = true; else nodeL = true;
+                    }
+                    if (drb <= kd) {
+                        if (b - g - 1 < bucket) rangeR = true; else nodeR = true;
+                    }
+                }
+                const uint64_t mnl = __ballot(nodeL), mnr = __ballot(nodeR);
+                const uint64_t mrl = __ballot(rangeL), mrr = __ballot(rangeR);
+                const int pushes = __popcll(mnl) + __popcll(mnr);
+                if (sp + pushes > kStackCap) {
+                    overflow = true;
+                    break;
+                }
+                const int pnl = sp + __popcll(mnl & lt);
+                const int pnr = sp + __popcll(mnl) + __popcll(mnr & lt);
+                const int prl = nr + __popcll(mrl & lt);
+                const int prr = nr + __popcll(mrl) + __popcll(mrr & lt);
+                if (nodeL) s_stack[w][pnl] = make_int2(g, static_cast<int>(dlb));
+                if (nodeR) s_stack[w][pnr] = make_int2(g + 1, static_cast<int>(drb));
+                if (rangeL) s_rng[w][prl] = make_int2(a, g);
+                if (rangeR) s_rng[w][prr] = make_int2(g + 1, b);
+                sp += pushes;
+                nr += __popcll(mrl) + __popcll(mrr);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
+            if (nr > 0) {
+                const int total = range_prefix(s_rng[w], s_pre[w], nr);
+                for (int f0 = 0; f0 < total; f0 += kWave) {
+                    const int f = f0 + lane;
+                    int j = f < total ? range_point(s_rng[w], s_pre[w], nr, f) : -1;
+                    if (j >= wbase && j <= wend) j = -1;  // in the seed
+                    uint64_t ck = ~0ull;
+                    if (j >= 0) {
+                        const float4 p = pts[j];
+                        ck = make_key(dist2(q.x, q.y, q.z, p.x, p.y, p.z), j);
+                    }
+                    uint64_t m = __ballot(ck < kth);
+                    while (m) {
+                        const int src = static_cast<int>(__builtin_ctzll(m));
+                        m &= m - 1ull;
+                        const uint64_t x = read_lane64(ck, src);
+                        if (x < kth) {
+                            const int pos = __popcll(__ballot(lane < K && key < x));
+                            const uint64_t up = shfl64(key, lane > 0 ? lane - 1 : 0);
+                            key = lane == pos ? x : (lane > pos ? up : key);
+                            kth = read_lane64(key, K - 1);
+                        }
+                    }
+                }
+                nr = 0;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
+        }
+        int32_t* row = out + static_cast<int64_t>(qi - q_start) * K;
+        if (overflow) {
+            TopK<K> top;
             top.init();
-            r2bits = 0xFFFFFFFFu;
+            if (lane == 0) {
+                knn_search<K>(pts, nodes, U, q.x, q.y, q.z, 0xFFFFFFFFu, bucket, top);
+                write_row<K>(row, top);
+            }
+            continue;
         }
-        write_row<K>(out + (qi - q_start) * K, top);
+        const int first = __builtin_amdgcn_readfirstlane(static_cast<int>(key));
+        if (lane < K) row[lane] = key == ~0ull ? first : static_cast<int32_t>(key & 0xFFFFFFFFull);
     }
 }
 
@@ -463,11 +656,11 @@ int knn_tiles_k(const float4* pts, const TravNode* nodes, const int32_t* d_count
         knn_tile_kernel<K, false><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, nullptr, bucket, out,
                                                           queue, queue_count);
     SX_LAUNCH_CHECK("knn_tile_kernel");
-    unsigned gq = grid_for(n_q, kKnnBlock);
-    if (gq > 2048) gq = 2048;
-    knn_queue_kernel<K><<<gq, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, queue, queue_count, radius2, bucket,
-                                                 out, deferred_total);
-    SX_LAUNCH_CHECK("knn_queue_kernel");
+    int64_t gq = (n_q + kTileWaves - 1) / kTileWaves;  // one wave per query at most
+    if (gq > 8192) gq = 8192;
+    knn_wave_kernel<K><<<static_cast<unsigned>(gq), kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, queue,
+                                                                      queue_count, bucket, out, deferred_total);
+    SX_LAUNCH_CHECK("knn_wave_kernel");
     return SX_OK;
 }
 

@@ -143,20 +143,71 @@ __device__ __forceinline__ bf16x4_t load_x(const MlpDev& m, const float* __restr
                  (v.z - mu[4 * g + 2]) * is[4 * g + 2], (v.w - mu[4 * g + 3]) * is[4 * g + 3]);
 }
 
+// A operands and biases of one lane, loaded once per wave for inference.
+struct MlpRegs {
+    bf16x4_t a1[4], a2[4][4], a3[4];
+    f32x4_t b1[4], b2[4];
+    float b3;
+    __device__ __forceinline__ void load(const MlpDev& m) {
+        const int l = static_cast<int>(lane_id());
+        const int r = l % 16, g = l / 16;
+        const bf16x4_t zero4 = {0, 0, 0, 0};
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            a1[mt] = load4(m.w1p + (16 * mt + r) * kInPad + 4 * g);
+            a3[mt] = (r == 0) ? load4(m.w3 + 16 * mt + 4 * g) : zero4;
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) a2[mt][kc] = load4(m.w2 + (16 * mt + r) * kH + 16 * kc + 4 * g);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                b1[mt][e] = m.p[kOffB1 + 16 * mt + 4 * g + e];
+                b2[mt][e] = m.p[kOffB2 + 16 * mt + 4 * g + e];
+            }
+        }
+        b3 = (g == 0) ? m.p[kOffB3] : 0.0f;
+    }
+    // y of the tile's 16 queries (valid in lanes 0..15)
+    __device__ __forceinline__ float forward(bf16x4_t xb) const {
+        bf16x4_t h1[4], h2[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) h1[mt] = relu_pack(mfma(a1[mt], xb, b1[mt]));
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            f32x4_t c = b2[mt];
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) c = mfma(a2[mt][kc], h1[kc], c);
+            h2[mt] = relu_pack(c);
+        }
+        f32x4_t c = {b3, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) c = mfma(a3[kc], h2[kc], c);
+        return c[0];
+    }
+};
+
 // mode 0: y (raw prediction); mode 1: exp(y + margin) (squared radius).
+// Each wave keeps the weights in registers and walks tiles of 16 queries.
 __global__ void __launch_bounds__(kMlpBlock) mlp_forward_kernel(MlpDev m, const float* __restrict__ feats, int64_t n,
                                                                 int mode, float* __restrict__ out) {
     const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    const int64_t n_waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
     const int l = static_cast<int>(lane_id());
     const float margin = m.norm[2 * kIn];
-#pragma unroll 1
-    for (int t = 0; t < 4; ++t) {
-        const int64_t q0 = wave * 64 + 16 * t;
-        if (q0 >= n) break;  // wave-uniform
-        Fwd f;
-        mlp_tile_forward(m, load_x(m, feats, q0, n), f);
-        if (l < 16 && q0 + l < n) out[q0 + l] = mode == 0 ? f.y : __expf(f.y + margin);
+    MlpRegs w;
+    w.load(m);
+    const int64_t tiles = (n + 15) / 16;
+    for (int64_t t = wave; t < tiles; t += n_waves) {
+        const int64_t q0 = t * 16;
+        const float y = w.forward(load_x(m, feats, q0, n));
+        if (l < 16 && q0 + l < n) out[q0 + l] = mode == 0 ? y : __expf(y + margin);
     }
+}
+
+inline unsigned mlp_forward_grid(int64_t n) {
+    const int64_t tiles = (n + 15) / 16;
+    int64_t blocks = (tiles + 4 * kWavesPerBlock - 1) / (4 * kWavesPerBlock);  // about 4 tiles per wave
+    if (blocks > 4096) blocks = 4096;
+    return static_cast<unsigned>(blocks < 1 ? 1 : blocks);
 }
 
 // Gradient of mean((y - t)^2) over n samples.  Each wave handles 64 samples
@@ -555,7 +606,7 @@ int sx_mlp_forward(sx_mlp* m, const float* d_feats, int64_t n, float* d_out, voi
     SX_TRY(check_mlp(m));
     if (n < 0 || (n > 0 && (!d_feats || !d_out))) return fail(SX_EINVAL, "sx_mlp_forward: bad arguments");
     if (n == 0) return SX_OK;
-    sx::mlp_forward_kernel<<<sx::grid_for(n, sx::kMlpBlock), sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
+    sx::mlp_forward_kernel<<<sx::mlp_forward_grid(n), sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
         dev_view(m), d_feats, n, 0, d_out);
     SX_LAUNCH_CHECK("mlp_forward_kernel");
     return SX_OK;
@@ -565,7 +616,7 @@ int sx_mlp_predict_radius2(sx_mlp* m, const float* d_feats, int64_t n, float* d_
     SX_TRY(check_mlp(m));
     if (n < 0 || (n > 0 && (!d_feats || !d_radius2))) return fail(SX_EINVAL, "sx_mlp_predict_radius2: bad arguments");
     if (n == 0) return SX_OK;
-    sx::mlp_forward_kernel<<<sx::grid_for(n, sx::kMlpBlock), sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
+    sx::mlp_forward_kernel<<<sx::mlp_forward_grid(n), sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
         dev_view(m), d_feats, n, 1, d_radius2);
     SX_LAUNCH_CHECK("mlp_forward_kernel");
     return SX_OK;

@@ -5,16 +5,14 @@
 //   src/Octree.cu:38-50 and :218); offsets = exclusive scan (cub InclusiveSum
 //   into offsets+1, src/Octree.cu:228-247), computed by the device scan with
 //   the total landing in the octree node count.
-// nodes: radix node i emits a chain of count(i) octree nodes at
+// nodes: radix node i owns a chain of count(i) octree nodes at
 //   offsets[i] .. offsets[i]+count(i)-1, deepest first, each the child of the
 //   next; the top of the chain hangs under the bottom node of the nearest
-//   ancestor with a non-zero count (makeNodes, src/Octree.cu:114-161).  The
-//   root (node 0) gets cell_size = range (src/Octree.cu:257-261).
-// leaves: point leaves go into the bottom node of the nearest contributing
-//   ancestor, slot = the leaf's next 3-bit digit (linkLeafNodes,
-//   src/Octree.cu:65-112).
-// Masks are set with atomicOr (several radix nodes write one octree node);
-// every slot is written by exactly one thread.
+//   ancestor with a non-zero count (makeNodes, src/Octree.cu:114-161), and
+//   point leaves sit in that bottom node at the slot of their next 3-bit digit
+//   (linkLeafNodes, src/Octree.cu:65-112).  The root (node 0) gets
+//   cell_size = range (src/Octree.cu:257-261).
+// Nodes are gathered, not scattered: one thread writes each node whole.
 #include <hip/hip_runtime.h>
 
 #include "sx_internal.hpp"
@@ -55,94 +53,102 @@ __device__ __forceinline__ uint64_t code_prefix(uint64_t code, int level, int co
     return sh >= 64 ? 0ull : (code >> sh);
 }
 
-__global__ void __launch_bounds__(256) oct_zero_kernel(sx_otnode* nodes, const int32_t* d_oct_count, int64_t cap) {
-    int64_t O = *d_oct_count;
-    if (O > cap) O = cap;
-    const int64_t words = O * static_cast<int64_t>(sizeof(sx_otnode) / 4);
-    int32_t* w = reinterpret_cast<int32_t*>(nodes);
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += stride) w[i] = 0;
+// Every octree node is written whole by one thread (no atomics, no zeroing).
+// Radix node i with count c > 0 owns nodes offsets[i] .. offsets[i]+c-1 at
+// depths D, D-1, ..., D-c+1 (D = prefixN[i]/3).  A node above the bottom of
+// the chain holds exactly one child, the chain node below it: its cell
+// contains only i's points.  The bottom node (depth D) gathers its children
+// from i's two radix children: a leaf lands in the slot of its next digit; an
+// internal child with a non-zero count contributes the top of its own chain;
+// an internal child with count 0 lies at the same depth, so its children are
+// gathered in turn (at most 8 slots, so at most 7 such nodes).  This is the
+// node set and numbering makeNodes + linkLeafNodes produce
+// (src/Octree.cu:65-161), with unused child slots left 0 as after the
+// reference's cudaMemset (src/Octree.cu:253).
+__device__ __forceinline__ void store_node(sx_otnode* nodes, int64_t cap, int o, const int32_t ch[8], float3 corner,
+                                           float cell, int node_mask, int leaf_mask) {
+    if (o < 0 || o >= cap) return;
+    int2* w = reinterpret_cast<int2*>(nodes + o);  // 56-byte stride: 8-byte aligned
+    w[0] = make_int2(ch[0], ch[1]);
+    w[1] = make_int2(ch[2], ch[3]);
+    w[2] = make_int2(ch[4], ch[5]);
+    w[3] = make_int2(ch[6], ch[7]);
+    w[4] = make_int2(__float_as_int(corner.x), __float_as_int(corner.y));
+    w[5] = make_int2(__float_as_int(corner.z), __float_as_int(cell));
+    w[6] = make_int2(node_mask, leaf_mask);
 }
 
-__device__ __forceinline__ void set_child(sx_otnode* nodes, int64_t cap, int node, int child, int slot) {
-    if (node < 0 || node >= cap) return;
-    nodes[node].children[slot] = child;
-    atomicOr(&nodes[node].child_node_mask, 1 << slot);
-}
-
-__device__ __forceinline__ void set_leaf(sx_otnode* nodes, int64_t cap, int node, int leaf, int slot) {
-    if (node < 0 || node >= cap) return;
-    nodes[node].children[slot] = leaf;
-    atomicOr(&nodes[node].child_leaf_mask, 1 << slot);
-}
-
-__device__ __forceinline__ void set_cell(sx_otnode* nodes, int64_t cap, int node, float3 corner, float cell) {
-    if (node < 0 || node >= cap) return;
-    nodes[node].corner[0] = corner.x;
-    nodes[node].corner[1] = corner.y;
-    nodes[node].corner[2] = corner.z;
-    nodes[node].cell_size = cell;
-}
-
-__global__ void __launch_bounds__(256) oct_build_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count,
-                                                        int64_t n_max, int code_bits, const float* __restrict__ bounds,
-                                                        const uint8_t* __restrict__ pn,
-                                                        const int32_t* __restrict__ left_child,
-                                                        const uint8_t* __restrict__ hll,
-                                                        const uint8_t* __restrict__ hlr,
-                                                        const int32_t* __restrict__ parent,
-                                                        const int32_t* __restrict__ offsets,
-                                                        const int32_t* __restrict__ counts, sx_otnode* nodes,
-                                                        int64_t cap, int32_t* leaf_oct) {
+__global__ void __launch_bounds__(256) oct_gather_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count,
+                                                         int64_t n_max, int code_bits, const float* __restrict__ bounds,
+                                                         const uint8_t* __restrict__ pn,
+                                                         const int32_t* __restrict__ left_child,
+                                                         const uint8_t* __restrict__ hll,
+                                                         const uint8_t* __restrict__ hlr,
+                                                         const int32_t* __restrict__ offsets,
+                                                         const int32_t* __restrict__ counts, sx_otnode* nodes,
+                                                         int64_t cap, int32_t* leaf_oct) {
     const int64_t U = scan_count(d_count, n_max);
     const int64_t M = U - 1;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= M) return;
+    const int c = counts[i];
+    if (c <= 0) return;
     const float mn = bounds[0], range = bounds[2];
     const float fscale = static_cast<float>(axis_scale(code_bits));
     const int root_level = pn[0] / 3;
+    const int D = pn[i] / 3;
+    const int base = offsets[i];
+    const uint64_t ci = codes[i];
+    int32_t ch[8];
+    // chain nodes above the bottom
+    for (int j = 1; j < c; ++j) {
+        const int L = D - j;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) ch[s] = 0;
+        const int slot = static_cast<int>(code_prefix(ci, L + 1, code_bits) & 7);
+        ch[slot] = base + j - 1;
+        store_node(nodes, cap, base + j, ch, cell_corner(code_prefix(ci, L, code_bits), L, code_bits, mn, range, fscale),
+                   __fdiv_rn(range, static_cast<float>(1 << (L - root_level))), 1 << slot, 0);
+    }
+    // bottom node: gather children through count-0 radix descendants
+#pragma unroll
+    for (int s = 0; s < 8; ++s) ch[s] = 0;
+    int node_mask = 0, leaf_mask = 0;
+    int stack[8];
+    int sp = 0;
+    int cur = static_cast<int>(i);
+    while (true) {
+        const int g = left_child[cur];
+        const bool leaf_side[2] = {hll[cur] != 0, hlr[cur] != 0};
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int child = g + side;
+            if (leaf_side[side]) {
+                const int slot = static_cast<int>(code_prefix(codes[child], D + 1, code_bits) & 7);
+                ch[slot] = child;
+                leaf_mask |= 1 << slot;
+                if (leaf_oct) leaf_oct[child] = base;
+            } else if (counts[child] > 0) {
+                const int slot = static_cast<int>(code_prefix(codes[child], D + 1, code_bits) & 7);
+                ch[slot] = offsets[child] + counts[child] - 1;
+                node_mask |= 1 << slot;
+            } else if (sp < 8) {
+                stack[sp++] = child;
+            }
+        }
+        if (sp == 0) break;
+        cur = stack[--sp];
+    }
+    float3 corner;
+    float cell;
     if (i == 0) {
-        const uint64_t rp = code_prefix(codes[0], root_level, code_bits);
-        set_cell(nodes, cap, 0, cell_corner(rp, root_level, code_bits, mn, range, fscale), range);
+        corner = cell_corner(code_prefix(codes[0], root_level, code_bits), root_level, code_bits, mn, range, fscale);
+        cell = range;
     } else {
-        int oct = offsets[i];
-        const int n_new = counts[i];
-        const uint64_t ci = codes[i];
-        const int depth = pn[i] / 3;
-        for (int jj = 0; jj < n_new - 1; ++jj) {
-            const int level = depth - jj;
-            const uint64_t pref = code_prefix(ci, level, code_bits);
-            set_child(nodes, cap, oct + 1, oct, static_cast<int>(pref & 7));
-            set_cell(nodes, cap, oct, cell_corner(pref, level, code_bits, mn, range, fscale),
-                     __fdiv_rn(range, static_cast<float>(1 << (level - root_level))));
-            oct += 1;
-        }
-        if (n_new > 0) {
-            int rp = parent[i];
-            while (rp >= 0 && counts[rp] == 0) rp = parent[rp];
-            const int top = depth - n_new + 1;
-            const uint64_t pref = code_prefix(ci, top, code_bits);
-            if (rp >= 0) set_child(nodes, cap, offsets[rp], oct, static_cast<int>(pref & 7));
-            set_cell(nodes, cap, oct, cell_corner(pref, top, code_bits, mn, range, fscale),
-                     __fdiv_rn(range, static_cast<float>(1 << (top - root_level))));
-        }
+        corner = cell_corner(code_prefix(ci, D, code_bits), D, code_bits, mn, range, fscale);
+        cell = __fdiv_rn(range, static_cast<float>(1 << (D - root_level)));
     }
-    if (hll[i] || hlr[i]) {
-        int r = static_cast<int>(i);
-        while (r >= 0 && counts[r] == 0) r = parent[r];
-        const int bottom = r >= 0 ? offsets[r] : -1;
-        const int level = pn[i] / 3 + 1;
-        if (hll[i]) {
-            const int leaf = left_child[i];
-            set_leaf(nodes, cap, bottom, leaf, static_cast<int>(code_prefix(codes[leaf], level, code_bits) & 7));
-            if (leaf_oct) leaf_oct[leaf] = bottom;
-        }
-        if (hlr[i]) {
-            const int leaf = left_child[i] + 1;
-            set_leaf(nodes, cap, bottom, leaf, static_cast<int>(code_prefix(codes[leaf], level, code_bits) & 7));
-            if (leaf_oct) leaf_oct[leaf] = bottom;
-        }
-    }
+    store_node(nodes, cap, base, ch, corner, cell, node_mask, leaf_mask);
 }
 
 size_t octree_ws_bytes(int64_t n_max) {
@@ -165,13 +171,9 @@ int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
     void* scan_ws = c.take<char>(scan_ws_bytes(n_max));
     SX_TRY(device_scan(OctCountIn{pn, parent, d_count}, OctCountOut{offsets, counts}, d_count, n_max, d_oct_count,
                        scan_ws, st));
-    unsigned zg = grid_for(cap * static_cast<int64_t>(sizeof(sx_otnode) / 4), 256);
-    if (zg > 8192) zg = 8192;
-    oct_zero_kernel<<<zg, 256, 0, st>>>(nodes, d_oct_count, cap);
-    SX_LAUNCH_CHECK("oct_zero_kernel");
-    oct_build_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(codes, d_count, n_max, code_bits, bounds, pn, left_child,
-                                                           hll, hlr, parent, offsets, counts, nodes, cap, leaf_oct);
-    SX_LAUNCH_CHECK("oct_build_kernel");
+    oct_gather_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(codes, d_count, n_max, code_bits, bounds, pn, left_child,
+                                                            hll, hlr, offsets, counts, nodes, cap, leaf_oct);
+    SX_LAUNCH_CHECK("oct_gather_kernel");
     return SX_OK;
 }
 

@@ -29,6 +29,20 @@ __global__ void fill_i32_kernel(int32_t* dst, int32_t value, int64_t n) {
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = value;
 }
 
+__global__ void scatter_i32_kernel(const int32_t* __restrict__ src, const uint32_t* __restrict__ idx, int64_t n,
+                                   int32_t* __restrict__ dst) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[idx[i]] = src[i];
+}
+
+// dst[idx[i]] = src[i] for i < n.
+int scatter_i32_launch(const int32_t* src, const uint32_t* idx, int64_t n, int32_t* dst, hipStream_t st) {
+    if (n <= 0) return SX_OK;
+    scatter_i32_kernel<<<grid_for(n, 256), 256, 0, st>>>(src, idx, n, dst);
+    SX_LAUNCH_CHECK("scatter_i32_kernel");
+    return SX_OK;
+}
+
 int fill_u8(void* dst, uint8_t value, size_t bytes, hipStream_t st) {
     if (bytes == 0) return SX_OK;
     if (reinterpret_cast<uintptr_t>(dst) % 16 != 0) return fail(SX_EINVAL, "fill_u8: destination not 16-byte aligned");

@@ -5,7 +5,18 @@
 // the sort permutation (and with it the duplicate representatives and the
 // input-order scatter of results) is available.
 //
-// Per digit pass:
+// Two schedules share the tile ranking code:
+//
+// Onesweep (default): one histogram kernel counts every digit of every pass
+// in a single read of the keys; each pass is then ONE kernel in which tiles
+// take tickets in launch order, publish their digit counts, and look back
+// over the preceding tiles' published counts (decoupled look-back: one
+// 32-bit word per (pass, tile, digit) holding a 2-bit state and a 30-bit
+// count) to find their global output offsets.  The keys are read twice and
+// written once per pass; the scatter is staged through LDS so that each
+// digit's run leaves the block as contiguous stores.
+//
+// Three-kernel fallback (SX_SORT_THREE_KERNEL), per digit pass:
 //   1. radix_hist_kernel     per-tile 256-bin digit histogram (LDS atomics),
 //                            stored digit-major: hist[d * tiles + t]
 //   2. device_scan           exclusive scan of that array = global output

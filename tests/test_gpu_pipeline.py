@@ -270,7 +270,10 @@ def test_knn_radius_seed_is_exact(gpu):
         print(f"[radius] scale {scale}: {reruns / len(kth):.3%} searched again")
         if scale == 0.25:
             assert reruns > 0.9 * len(kth)
-        if scale is not None and scale >= 1.0:
+        # a radius at (or just above) the k-th distance proves almost every
+        # tile lane exact; a 4x radius only widens the tile boxes (more LDS
+        # overflows, still exact results)
+        if scale in (1.0, 1.0001):
             assert reruns < 0.05 * len(kth)
 
 
