@@ -245,7 +245,9 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
     ix->built = false;
     ix->counts_valid = false;
     const int cb = ix->code_bits;
-    SX_TRY(sx::fill_u8(ix->counts, 0, 32, st));
+    // unique / octree / non-finite counts; the deferred-query total accumulates
+    // across builds until sx_index_knn_reruns resets it
+    SX_TRY(sx::fill_i32(ix->counts, 0, kCntReruns, st));
     SX_TRY(sx::bbox_launch(d_x, d_y, d_z, n, ix->bounds, ix->counts + kCntBad, ix->ws_bbox, false, st));
     SX_TRY(sx::encode_launch(d_x, d_y, d_z, n, ix->bounds, cb, ix->codes_in, st));
     SX_TRY(sx::sort_pairs_launch(ix->codes_in, nullptr, n, cb, ix->codes_sorted, ix->perm, ix->ws, ix->sort_algo, st));

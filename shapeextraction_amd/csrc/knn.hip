@@ -245,9 +245,13 @@ int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts
 // to a queue that knn_wave_kernel searches, one wave per query.
 // ---------------------------------------------------------------------------
 constexpr int kTileWaves = kKnnBlock / kWave;
-constexpr int kFrontCap = 128;
-constexpr int kRangeCap = 256;
+constexpr int kTileStack = 256;                          // subtrees waiting to be tested against E
+constexpr int kRangeFlush = 64;                          // stream the ranges once this many are pending
+constexpr int kTileRanges = kRangeFlush - 1 + 2 * kWave;  // one expansion adds at most 2 per lane
 constexpr int kWindow = 32;
+constexpr int kSegBudget = 8192;                         // candidates one segment may stream before deferring
+constexpr float kSegGap = 8.0f;                          // segment break: gap > kSegGap x mean radius
+constexpr float kRadiusStretch = 2.25f;                  // learned: accept up to 1.5x the predicted radius
 
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
@@ -264,11 +268,35 @@ __device__ __forceinline__ float wave_sum(float v) {
     for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
     return v;
 }
+// An empty asm that "modifies" v: values derived from the result are not
+// hoisted out of the enclosing loop (keeps per-iteration temporaries out of
+// the registers that stay live across the loop).
+__device__ __forceinline__ float opaque(float v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+__device__ __forceinline__ float wave_min_at(float v, int lane) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1)
+        v = fminf(v, __int_as_float(__builtin_amdgcn_ds_bpermute((lane ^ o) << 2, __float_as_int(v))));
+    return v;
+}
+__device__ __forceinline__ float wave_max_at(float v, int lane) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1)
+        v = fmaxf(v, __int_as_float(__builtin_amdgcn_ds_bpermute((lane ^ o) << 2, __float_as_int(v))));
+    return v;
+}
 __device__ __forceinline__ bool box_meets(float4 lo, float4 hi, float ex0, float ey0, float ez0, float ex1, float ey1,
                                           float ez1) {
     return hi.x >= ex0 && lo.x <= ex1 && hi.y >= ey0 && lo.y <= ey1 && hi.z >= ez0 && lo.z <= ez1;
 }
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniform_f(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
 __device__ __forceinline__ float read_lane(float v, int src) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
 }
@@ -300,6 +328,11 @@ __device__ __forceinline__ uint64_t wave_sort64(uint64_t v) {
         }
     }
     return v;
+}
+// Number of set bits of m below this lane (popcount(m & lanes_below)).
+__device__ __forceinline__ int rank_below(uint64_t m) {
+    return static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
 }
 __device__ __forceinline__ uint64_t make_key(float d, int j) {
     return (static_cast<uint64_t>(__float_as_uint(d)) << 32) | static_cast<uint32_t>(j);
@@ -345,6 +378,60 @@ __device__ __forceinline__ int range_prefix(const int2* rng, int* pre, int nr) {
     return total;
 }
 
+// Broadcast the n candidates staged in cand[0, n) (x, y, z, index bits; two
+// per step, uniform LDS reads) to every lane; a lane inserts a candidate when
+// on, its squared distance is <= tau and it beats the lane's k-th key.
+// cand[n] must hold a point at +inf when n is odd.
+template <int K>
+__device__ __forceinline__ void broadcast_insert(const float4* cand, int n, float4 q, bool on, uint32_t tau,
+                                                 TopK<K>& top) {
+    for (int i = 0; i < n; i += 2) {
+        const float4 c0 = cand[i];
+        const float4 c1 = cand[i + 1];
+        const uint32_t d0 = __float_as_uint(dist2(q.x, q.y, q.z, c0.x, c0.y, c0.z));
+        const uint32_t d1 = __float_as_uint(dist2(q.x, q.y, q.z, c1.x, c1.y, c1.z));
+        const uint64_t k0 = (static_cast<uint64_t>(d0) << 32) | __float_as_uint(c0.w);
+        const uint64_t k1 = (static_cast<uint64_t>(d1) << 32) | __float_as_uint(c1.w);
+        if (on && d0 <= tau && k0 < top.e[K - 1]) top.insert(k0);
+        if (on && d1 <= tau && k1 < top.e[K - 1]) top.insert(k1);
+    }
+}
+
+// Stage the lanes' candidates (p, j) with ok set into cand[0, n), in lane
+// order, followed by one point at +inf; returns n (uniform).
+__device__ __forceinline__ int stage_candidates(float4* cand, float4 p, int j, bool ok) {
+    const uint64_t m = __ballot(ok);
+    const int n = __popcll(m);
+    if (ok) cand[rank_below(m)] = make_float4(p.x, p.y, p.z, __int_as_float(j));
+    if (static_cast<int>(lane_id()) == 0) cand[n] = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    return n;
+}
+
+// Stream the leaf ranges rng[0, nr): 64 candidates per load (one per lane,
+// coalesced); those inside E are staged and broadcast.  Indices in
+// [skip_lo, skip_hi] were seen already and are skipped.
+template <int K>
+__device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, const int2* rng, int* pre, int nr,
+                                              int skip_lo, int skip_hi, float ex0, float ey0, float ez0, float ex1,
+                                              float ey1, float ez1, float4 q, bool on, uint32_t tau, TopK<K>& top,
+                                              int& seen, float4* cand) {
+    const int lane = static_cast<int>(lane_id());
+    const int total = range_prefix(rng, pre, nr);
+    seen += total;
+    for (int f0 = 0; f0 < total; f0 += kWave) {
+        const int f = f0 + lane;
+        int j = f < total ? range_point(rng, pre, nr, f) : -1;
+        if (j >= skip_lo && j <= skip_hi) j = -1;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (j >= 0) p = pts[j];
+        const bool in_e = j >= 0 && p.x >= ex0 && p.x <= ex1 && p.y >= ey0 && p.y <= ey1 && p.z >= ez0 && p.z <= ez1;
+        const int n = stage_candidates(cand, p, j, in_e);
+        broadcast_insert<K>(cand, n, q, on, tau, top);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
 template <int K, bool kLearned>
 __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __restrict__ pts,
                                                              const TravNode* __restrict__ nodes,
@@ -352,20 +439,23 @@ __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __res
                                                              int64_t n_q, const float* __restrict__ radius2,
                                                              int bucket, int32_t* __restrict__ out,
                                                              int32_t* __restrict__ queue, int32_t* queue_count) {
-    __shared__ int s_front[kTileWaves][2][kFrontCap];
-    __shared__ int2 s_range[kTileWaves][kRangeCap];
-    static_assert(2 * kFrontCap >= kRangeCap, "the range prefix reuses the frontier buffers");
+    __shared__ int s_stack[kTileWaves][kTileStack];
+    __shared__ int2 s_range[kTileWaves][kTileRanges];
+    __shared__ int s_pre[kTileWaves][kTileRanges];
+    __shared__ float4 s_cand[kTileWaves][kWave + 1];
     const int w = static_cast<int>(threadIdx.x) / kWave;
     const int lane = static_cast<int>(lane_id());
-    const uint64_t lt = (1ull << lane) - 1ull;
     const int U = d_counts[0];
     int64_t q_end = q_start + n_q;
     if (q_end > U) q_end = U;
-    const int64_t tile0 = q_start + (static_cast<int64_t>(blockIdx.x) * kTileWaves + w) * kWave;
+    const int64_t tile0 = q_start + (static_cast<int64_t>(xcd_block()) * kTileWaves + w) * kWave;
     if (tile0 >= q_end) return;  // whole wave idle (no block barriers below)
     const int64_t qi = tile0 + lane;
     const bool valid = qi < q_end;
     const float4 q = valid ? pts[qi] : make_float4(0.f, 0.f, 0.f, 0.f);
+    int* stk = s_stack[w];
+    int2* rng = s_range[w];
+    int* pre = s_pre[w];
     TopK<K> top;
     top.init();
 
@@ -378,12 +468,9 @@ __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __res
         for (int base = wlo; base <= whi; base += kWave) {
             const int j = base + lane;
             const float4 p = j <= whi ? pts[j] : make_float4(0.f, 0.f, 0.f, 0.f);
-            const int cnt = whi - base + 1 < kWave ? whi - base + 1 : kWave;
-            for (int src = 0; src < cnt; ++src) {
-                const float d = dist2(q.x, q.y, q.z, read_lane(p.x, src), read_lane(p.y, src), read_lane(p.z, src));
-                const uint64_t key = make_key(d, base + src);
-                if (key < top.e[K - 1]) top.insert(key);
-            }
+            const int n = stage_candidates(s_cand[w], p, j, j <= whi);
+            broadcast_insert<K>(s_cand[w], n, q, true, 0xFFFFFFFFu, top);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
         r2bits = top.kth_bits();  // 0xFFFFFFFF when the window holds fewer than K points
     } else {
@@ -398,37 +485,76 @@ __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __res
     // padded radius: fl(rp * rp) > r2; each box bound is stepped one ulp
     // outward so it bounds the exact q -+ rp
     const float rp = r * 1.001f + 1e-19f;
-    const float ex0 = wave_min(join ? nextafterf(q.x - rp, -INFINITY) : INFINITY);
-    const float ey0 = wave_min(join ? nextafterf(q.y - rp, -INFINITY) : INFINITY);
-    const float ez0 = wave_min(join ? nextafterf(q.z - rp, -INFINITY) : INFINITY);
-    const float ex1 = wave_max(join ? nextafterf(q.x + rp, INFINITY) : -INFINITY);
-    const float ey1 = wave_max(join ? nextafterf(q.y + rp, INFINITY) : -INFINITY);
-    const float ez1 = wave_max(join ? nextafterf(q.z + rp, INFINITY) : -INFINITY);
-    const bool any_join = __ballot(join) != 0ull;
 
-    // ---- breadth-first range query for E
-    bool overflow = false;
-    int nr = 0;
-    if (any_join) {
-        int* cur = s_front[w][0];
-        int* nxt = s_front[w][1];
-        int fcount = 0;
+    // ---- segments: runs of joining lanes without a spatial gap larger than
+    // kSegGap mean radii between consecutive ones, so that a tile straddling
+    // a jump of the Morton curve does not search the space in between
+    uint64_t todo = __ballot(join);
+    uint64_t breaks;
+    {
+        const uint64_t below = todo & ((1ull << lane) - 1ull);
+        const int prev = below ? 63 - static_cast<int>(__builtin_clzll(below)) : lane;
+        const float gx = __shfl(q.x, prev, kWave), gy = __shfl(q.y, prev, kWave), gz = __shfl(q.z, prev, kWave);
+        const float gap = kSegGap * mean_r;
+        breaks = __ballot(join && below != 0ull &&
+                          (fabsf(q.x - gx) > gap || fabsf(q.y - gy) > gap || fabsf(q.z - gz) > gap));
+    }
+    bool lost = false;
+    uint32_t lim = 0u;  // squared radius within which the lane's list is proven complete
+    while (todo) {
+        const int s0 = static_cast<int>(__builtin_ctzll(todo));
+        const uint64_t after = s0 < 63 ? breaks & (~0ull << (s0 + 1)) : 0ull;
+        const uint64_t upto = after ? (1ull << __builtin_ctzll(after)) - 1ull : ~0ull;
+        const uint64_t seg = todo & upto;
+        todo &= ~seg;
+        const bool in_seg = (seg >> lane) & 1ull;
+        const float rps = opaque(rp);  // recomputed per segment rather than held live
+        const int ls = opaque(lane);
+        const float ex0 = uniform_f(wave_min_at(in_seg ? nextafterf(q.x - rps, -INFINITY) : INFINITY, ls));
+        const float ey0 = uniform_f(wave_min_at(in_seg ? nextafterf(q.y - rps, -INFINITY) : INFINITY, ls));
+        const float ez0 = uniform_f(wave_min_at(in_seg ? nextafterf(q.z - rps, -INFINITY) : INFINITY, ls));
+        const float ex1 = uniform_f(wave_max_at(in_seg ? nextafterf(q.x + rps, INFINITY) : -INFINITY, ls));
+        const float ey1 = uniform_f(wave_max_at(in_seg ? nextafterf(q.y + rps, INFINITY) : -INFINITY, ls));
+        const float ez1 = uniform_f(wave_max_at(in_seg ? nextafterf(q.z + rps, INFINITY) : -INFINITY, ls));
+        uint32_t tau = 0u;
+        if (in_seg) {
+            tau = r2bits;
+            if (kLearned) {
+                // Every point with dist2 <= rho2 lies inside E: the gap to the
+                // nearest face is shrunk by 2^-20 twice, far more than the
+                // rounding of the gap, the square and dist2 itself.  A lane may
+                // so accept beyond its predicted radius (up to kRadiusStretch).
+                float g = fminf(fminf(q.x - ex0, ex1 - q.x),
+                                fminf(fminf(q.y - ey0, ey1 - q.y), fminf(q.z - ez0, ez1 - q.z)));
+                g = fmaxf(g, 0.0f) * (1.0f - 0x1p-20f);
+                const float rho2 = (g * g) * (1.0f - 0x1p-20f);
+                const uint32_t t = __float_as_uint(fminf(rho2, __uint_as_float(r2bits) * kRadiusStretch));
+                if (t > tau) tau = t;  // the predicted ball itself lies inside E by construction
+            }
+            lim = tau;
+        }
+        // ---- depth-first range query for E, 64 subtrees per step; the
+        // ranges are streamed whenever kRangeFlush of them are pending
+        int sp = 0, nr = 0, seen = 0;
+        bool overflow = false;
         if (U <= bucket) {
-            if (lane == 0) s_range[w][0] = make_int2(0, U - 1);
+            if (lane == 0) rng[0] = make_int2(0, U - 1);
             nr = 1;
         } else {
-            if (lane == 0) cur[0] = 0;
-            fcount = 1;
+            if (lane == 0) stk[0] = 0;
+            sp = 1;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        while (fcount > 0 && !overflow) {
-            int ncount = 0;
-            for (int base = 0; base < fcount; base += kWave) {
-                const int idx = base + lane;
+        while (true) {
+            while (sp > 0 && nr < kRangeFlush) {
+                const int c = sp < kWave ? sp : kWave;
+                sp -= c;
+                const int node = lane < c ? stk[sp + lane] : -1;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 bool nodeL = false, nodeR = false, rangeL = false, rangeR = false;
                 int a = 0, g = 0, b = 0;
-                if (idx < fcount) {
-                    const TravNode n = nodes[cur[idx]];
+                if (node >= 0) {
+                    const TravNode n = nodes[node];
                     a = w_int(n.lo_l.w);
                     g = w_int(n.hi_l.w);
                     b = w_int(n.lo_r.w);
@@ -441,51 +567,37 @@ __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __res
                 }
                 const uint64_t mnl = __ballot(nodeL), mnr = __ballot(nodeR);
                 const uint64_t mrl = __ballot(rangeL), mrr = __ballot(rangeR);
-                const int pnl = ncount + __popcll(mnl & lt);
-                const int pnr = ncount + __popcll(mnl) + __popcll(mnr & lt);
-                const int prl = nr + __popcll(mrl & lt);
-                const int prr = nr + __popcll(mrl) + __popcll(mrr & lt);
-                if (nodeL && pnl < kFrontCap) nxt[pnl] = g;
-                if (nodeR && pnr < kFrontCap) nxt[pnr] = g + 1;
-                if (rangeL && prl < kRangeCap) s_range[w][prl] = make_int2(a, g);
-                if (rangeR && prr < kRangeCap) s_range[w][prr] = make_int2(g + 1, b);
-                ncount += __popcll(mnl) + __popcll(mnr);
+                const int pushes = __popcll(mnl) + __popcll(mnr);
+                if (sp + pushes > kTileStack) {
+                    overflow = true;
+                    sp = 0;
+                    break;
+                }
+                const int pnl = sp + rank_below(mnl);
+                const int pnr = sp + __popcll(mnl) + rank_below(mnr);
+                const int prl = nr + rank_below(mrl);
+                const int prr = nr + __popcll(mrl) + rank_below(mrr);
+                if (nodeL) stk[pnl] = g;
+                if (nodeR) stk[pnr] = g + 1;
+                if (rangeL) rng[prl] = make_int2(a, g);
+                if (rangeR) rng[prr] = make_int2(g + 1, b);
+                sp += pushes;
                 nr += __popcll(mrl) + __popcll(mrr);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
-            overflow = ncount > kFrontCap || nr > kRangeCap;
+            if (overflow || nr == 0) break;
+            stream_ranges<K>(pts, rng, pre, nr, wlo, whi, ex0, ey0, ez0, ex1, ey1, ez1, q, in_seg, tau, top, seen,
+                             s_cand[w]);
+            nr = 0;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            int* t = cur;
-            cur = nxt;
-            nxt = t;
-            fcount = ncount;
+            if (seen > kSegBudget) overflow = true;
+            if (overflow || sp == 0) break;
         }
+        if (overflow) lost = lost || in_seg;  // the segment's lanes are deferred
     }
-
-    // ---- candidate stream: 64 candidates per load, broadcast lane by lane
-    if (any_join && !overflow) {
-        int* pre = s_front[w][0];
-        const int total = range_prefix(s_range[w], pre, nr);
-        for (int f0 = 0; f0 < total; f0 += kWave) {
-            const int f = f0 + lane;
-            int j = f < total ? range_point(s_range[w], pre, nr, f) : -1;
-            if (!kLearned && j >= wlo && j <= whi) j = -1;  // already seeded from the window
-            float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (j >= 0) p = pts[j];
-            const bool in_e = j >= 0 && p.x >= ex0 && p.x <= ex1 && p.y >= ey0 && p.y <= ey1 && p.z >= ez0 &&
-                              p.z <= ez1;
-            uint64_t m = __ballot(in_e);
-            while (m) {
-                const int src = static_cast<int>(__builtin_ctzll(m));
-                m &= m - 1ull;
-                const float d = dist2(q.x, q.y, q.z, read_lane(p.x, src), read_lane(p.y, src), read_lane(p.z, src));
-                const uint32_t db = __float_as_uint(d);
-                const uint64_t key = (static_cast<uint64_t>(db) << 32) |
-                                     static_cast<uint32_t>(__builtin_amdgcn_readlane(j, src));
-                if (join && (kLearned ? db <= r2bits : true) && key < top.e[K - 1]) top.insert(key);
-            }
-        }
-    }
-    const bool exact = join && !overflow && (kLearned ? top.kth_bits() <= r2bits : true);
+    // plain: every point within the window radius was a candidate; learned:
+    // the k-th distance must lie within the radius the segment box proves
+    const bool exact = join && !lost && (kLearned ? top.kth_bits() <= lim : true);
     if (valid && exact) write_row<K>(out + (qi - q_start) * K, top);
     const bool defer = valid && !exact;
     const uint64_t md = __ballot(defer);
@@ -493,7 +605,7 @@ __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __res
         int slot0 = 0;
         if (lane == __ffsll(static_cast<unsigned long long>(md)) - 1) slot0 = atomicAdd(queue_count, __popcll(md));
         slot0 = __shfl(slot0, __ffsll(static_cast<unsigned long long>(md)) - 1, kWave);
-        if (defer) queue[slot0 + __popcll(md & lt)] = static_cast<int32_t>(qi);
+        if (defer) queue[slot0 + rank_below(md)] = static_cast<int32_t>(qi);
     }
 }
 
@@ -521,7 +633,6 @@ __global__ void __launch_bounds__(kKnnBlock) knn_wave_kernel(const float4* __res
     __shared__ int s_pre[kTileWaves][kWaveRanges];
     const int w = static_cast<int>(threadIdx.x) / kWave;
     const int lane = static_cast<int>(lane_id());
-    const uint64_t lt = (1ull << lane) - 1ull;
     const int U = d_counts[0];
     const int n = *queue_count;
     if (blockIdx.x == 0 && threadIdx.x == 0 && deferred_total) atomicAdd(deferred_total, n);
@@ -588,10 +699,10 @@ __global__ void __launch_bounds__(kKnnBlock) knn_wave_kernel(const float4* __res
                     overflow = true;
                     break;
                 }
-                const int pnl = sp + __popcll(mnl & lt);
-                const int pnr = sp + __popcll(mnl) + __popcll(mnr & lt);
-                const int prl = nr + __popcll(mrl & lt);
-                const int prr = nr + __popcll(mrl) + __popcll(mrr & lt);
+                const int pnl = sp + rank_below(mnl);
+                const int pnr = sp + __popcll(mnl) + rank_below(mnr);
+                const int prl = nr + rank_below(mrl);
+                const int prr = nr + __popcll(mrl) + rank_below(mrr);
                 if (nodeL) s_stack[w][pnl] = make_int2(g, static_cast<int>(dlb));
                 if (nodeR) s_stack[w][pnr] = make_int2(g + 1, static_cast<int>(drb));
                 if (rangeL) s_rng[w][prl] = make_int2(a, g);

@@ -41,6 +41,18 @@ inline unsigned grid_for(int64_t n, int block) {
 
 inline size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
 
+// Workgroups are dispatched round-robin over the XCDs (8 on MI355X), each
+// with its own L2.  xcd_block() renumbers them so that every XCD receives a
+// contiguous run of block indices: neighbouring tiles (Morton neighbours,
+// sharing candidate points and tree nodes) then meet in the same L2.  A
+// bijection on [0, gridDim.x) for any grid size.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const unsigned q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+    return x * q + (x < r ? x : r) + b / kXcds;
+}
+
 // Carves 256-byte aligned sub-buffers out of one workspace allocation.  With
 // base == nullptr it only measures.
 struct Carver {
