@@ -158,8 +158,13 @@ namespace sx {
 // Arrays the learned-radius feature kernel reads (mlp.hip).
 int index_feature_arrays(sx_index* ix, const TravNode** trav, const int32_t** leaf_parent, const uint64_t** codes,
                          const int32_t** leaf_oct, const sx_otnode** oct, const float** bounds,
-                         const int32_t** counts, int* code_bits, int64_t* limit) {
+                         const int32_t** counts, const uint8_t** pn, const int32_t** parent, int* code_bits,
+                         int64_t* limit) {
     SX_TRY(check_index(ix));
+    // the compact arrays describe the traversal tree only with the textbook root
+    const bool textbook = ix->root_mode == SX_ROOT_TEXTBOOK;
+    *pn = textbook ? ix->pn : nullptr;
+    *parent = textbook ? ix->parent : nullptr;
     *trav = ix->trav;
     *leaf_parent = ix->leaf_parent;
     *codes = ix->ucodes;

@@ -249,9 +249,11 @@ constexpr int kTileStack = 256;                          // subtrees waiting to 
 constexpr int kRangeFlush = 64;                          // stream the ranges once this many are pending
 constexpr int kTileRanges = kRangeFlush - 1 + 2 * kWave;  // one expansion adds at most 2 per lane
 constexpr int kWindow = 32;
+constexpr int kSeed = 8;                                  // learned: each lane's own +-kSeed Morton neighbours
 constexpr int kSegBudget = 8192;                         // candidates one segment may stream before deferring
 constexpr float kSegGap = 8.0f;                          // segment break: gap > kSegGap x mean radius
 constexpr float kRadiusStretch = 2.25f;                  // learned: accept up to 1.5x the predicted radius
+constexpr float kLearnedPad = 1.1f;                      // learned: box E bounds balls of 1.1x the predicted radius
 
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
@@ -382,18 +384,21 @@ __device__ __forceinline__ int range_prefix(const int2* rng, int* pre, int nr) {
 // per step, uniform LDS reads) to every lane; a lane inserts a candidate when
 // on, its squared distance is <= tau and it beats the lane's k-th key.
 // cand[n] must hold a point at +inf when n is odd.
-template <int K>
+template <int K, bool kLaneSkip>
 __device__ __forceinline__ void broadcast_insert(const float4* cand, int n, float4 q, bool on, uint32_t tau,
-                                                 TopK<K>& top) {
+                                                 int skip_lo, int skip_hi, TopK<K>& top) {
     for (int i = 0; i < n; i += 2) {
         const float4 c0 = cand[i];
         const float4 c1 = cand[i + 1];
         const uint32_t d0 = __float_as_uint(dist2(q.x, q.y, q.z, c0.x, c0.y, c0.z));
         const uint32_t d1 = __float_as_uint(dist2(q.x, q.y, q.z, c1.x, c1.y, c1.z));
-        const uint64_t k0 = (static_cast<uint64_t>(d0) << 32) | __float_as_uint(c0.w);
-        const uint64_t k1 = (static_cast<uint64_t>(d1) << 32) | __float_as_uint(c1.w);
-        if (on && d0 <= tau && k0 < top.e[K - 1]) top.insert(k0);
-        if (on && d1 <= tau && k1 < top.e[K - 1]) top.insert(k1);
+        const int j0 = __float_as_int(c0.w), j1 = __float_as_int(c1.w);
+        const uint64_t k0 = (static_cast<uint64_t>(d0) << 32) | static_cast<uint32_t>(j0);
+        const uint64_t k1 = (static_cast<uint64_t>(d1) << 32) | static_cast<uint32_t>(j1);
+        const bool s0 = kLaneSkip && j0 >= skip_lo && j0 <= skip_hi;  // already in this lane's seed
+        const bool s1 = kLaneSkip && j1 >= skip_lo && j1 <= skip_hi;
+        if (on && !s0 && d0 <= tau && k0 < top.e[K - 1]) top.insert(k0);
+        if (on && !s1 && d1 <= tau && k1 < top.e[K - 1]) top.insert(k1);
     }
 }
 
@@ -410,12 +415,13 @@ __device__ __forceinline__ int stage_candidates(float4* cand, float4 p, int j, b
 
 // Stream the leaf ranges rng[0, nr): 64 candidates per load (one per lane,
 // coalesced); those inside E are staged and broadcast.  Indices in
-// [skip_lo, skip_hi] were seen already and are skipped.
-template <int K>
+// [skip_lo, skip_hi] were seen by every lane already and are not staged;
+// with kLaneSkip, indices in the lane's [lane_lo, lane_hi] are ignored by it.
+template <int K, bool kLaneSkip>
 __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, const int2* rng, int* pre, int nr,
-                                              int skip_lo, int skip_hi, float ex0, float ey0, float ez0, float ex1,
-                                              float ey1, float ez1, float4 q, bool on, uint32_t tau, TopK<K>& top,
-                                              int& seen, float4* cand) {
+                                              int skip_lo, int skip_hi, int lane_lo, int lane_hi, float ex0,
+                                              float ey0, float ez0, float ex1, float ey1, float ez1, float4 q,
+                                              bool on, uint32_t tau, TopK<K>& top, int& seen, float4* cand) {
     const int lane = static_cast<int>(lane_id());
     const int total = range_prefix(rng, pre, nr);
     seen += total;
@@ -427,13 +433,17 @@ __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, co
         if (j >= 0) p = pts[j];
         const bool in_e = j >= 0 && p.x >= ex0 && p.x <= ex1 && p.y >= ey0 && p.y <= ey1 && p.z >= ez0 && p.z <= ez1;
         const int n = stage_candidates(cand, p, j, in_e);
-        broadcast_insert<K>(cand, n, q, on, tau, top);
+        broadcast_insert<K, kLaneSkip>(cand, n, q, on, tau, lane_lo, lane_hi, top);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
 }
 
+// Six waves per SIMD for K <= 16 (the compiler otherwise settles a few
+// registers above the 80 that allow it); larger lists keep their natural
+// allocation.
 template <int K, bool kLearned>
-__global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __restrict__ pts,
+__global__ void __launch_bounds__(kKnnBlock) __attribute__((amdgpu_waves_per_eu(K <= 16 ? 6 : 1, 8)))
+knn_tile_kernel(const float4* __restrict__ pts,
                                                              const TravNode* __restrict__ nodes,
                                                              const int32_t* __restrict__ d_counts, int64_t q_start,
                                                              int64_t n_q, const float* __restrict__ radius2,
@@ -443,6 +453,7 @@ __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __res
     __shared__ int2 s_range[kTileWaves][kTileRanges];
     __shared__ int s_pre[kTileWaves][kTileRanges];
     __shared__ float4 s_cand[kTileWaves][kWave + 1];
+    __shared__ float4 s_seed[kTileWaves][kLearned ? kWave + 2 * kSeed : 1];
     const int w = static_cast<int>(threadIdx.x) / kWave;
     const int lane = static_cast<int>(lane_id());
     const int U = d_counts[0];
@@ -460,7 +471,7 @@ __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __res
     top.init();
 
     // ---- per-lane radius
-    int wlo = 0, whi = -1;
+    int wlo = 0, whi = -1, seed_lo = 0, seed_hi = -1;
     uint32_t r2bits;
     if (!kLearned) {
         wlo = static_cast<int>(tile0 - kWindow > 0 ? tile0 - kWindow : 0);
@@ -469,13 +480,37 @@ __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __res
             const int j = base + lane;
             const float4 p = j <= whi ? pts[j] : make_float4(0.f, 0.f, 0.f, 0.f);
             const int n = stage_candidates(s_cand[w], p, j, j <= whi);
-            broadcast_insert<K>(s_cand[w], n, q, true, 0xFFFFFFFFu, top);
+            broadcast_insert<K, false>(s_cand[w], n, q, true, 0xFFFFFFFFu, 0, -1, top);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
         r2bits = top.kth_bits();  // 0xFFFFFFFF when the window holds fewer than K points
     } else {
         const float r2 = valid ? radius2[qi - q_start] : -1.0f;
         r2bits = (r2 >= 0.0f && r2 < INFINITY) ? __float_as_uint(r2) : 0xFFFFFFFFu;
+        // seed every lane with its own Morton neighbours: all lanes insert at
+        // every step (no wave-wide union of insert events), filling the lists
+        // before the stream, which then only inserts improvements
+        seed_lo = static_cast<int>(qi) - kSeed;
+        seed_hi = static_cast<int>(qi) + kSeed;
+        // the tile's points +-kSeed, staged in LDS by two coalesced loads
+        float4* win = s_seed[w];
+        const int64_t w0 = tile0 - kSeed;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int idx = lane + h * kWave;
+            const int64_t jj = w0 + idx;
+            if (idx < kWave + 2 * kSeed && jj >= 0 && jj < U) win[idx] = pts[jj];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll 1
+        for (int o = -kSeed; o <= kSeed; ++o) {
+            const int jj = static_cast<int>(qi) + o;
+            if (valid && jj >= 0 && jj < U) {
+                const float4 p = win[lane + kSeed + o];
+                const uint64_t key = make_key(dist2(q.x, q.y, q.z, p.x, p.y, p.z), jj);
+                if (key < top.e[K - 1]) top.insert(key);
+            }
+        }
     }
     const bool finite = valid && r2bits < 0x7F800000u;
     const float r = finite ? sqrtf(__uint_as_float(r2bits)) : 0.0f;
@@ -484,7 +519,9 @@ __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __res
     const bool join = finite && r <= 2.0f * mean_r;
     // padded radius: fl(rp * rp) > r2; each box bound is stepped one ulp
     // outward so it bounds the exact q -+ rp
-    const float rp = r * 1.001f + 1e-19f;
+    // (learned: the box is grown by kLearnedPad so that lanes whose radius
+    // was predicted slightly short can still be proven complete inside it)
+    const float rp = r * (kLearned ? 1.001f * kLearnedPad : 1.001f) + 1e-19f;
 
     // ---- segments: runs of joining lanes without a spatial gap larger than
     // kSegGap mean radii between consecutive ones, so that a tile straddling
@@ -586,8 +623,8 @@ __global__ void __launch_bounds__(kKnnBlock) knn_tile_kernel(const float4* __res
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
             if (overflow || nr == 0) break;
-            stream_ranges<K>(pts, rng, pre, nr, wlo, whi, ex0, ey0, ez0, ex1, ey1, ez1, q, in_seg, tau, top, seen,
-                             s_cand[w]);
+            stream_ranges<K, kLearned>(pts, rng, pre, nr, wlo, whi, seed_lo, seed_hi, ex0, ey0, ez0, ex1, ey1, ez1,
+                                       q, in_seg, tau, top, seen, s_cand[w]);
             nr = 0;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             if (seen > kSegBudget) overflow = true;

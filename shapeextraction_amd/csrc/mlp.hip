@@ -401,41 +401,50 @@ __global__ void mlp_adam_kernel(float* __restrict__ p, float* __restrict__ mo, f
 }
 
 // ---------------------------------------------------------------- features
-__device__ __forceinline__ int node_prefix(const TravNode* __restrict__ trav, const uint64_t* __restrict__ codes,
-                                           int node, int code_bits) {
-    const float4 lo_l = trav[node].lo_l, lo_r = trav[node].lo_r;
-    return prefix_len(codes[w_int(lo_l.w)], codes[w_int(lo_r.w)], code_bits);
-}
+// The walk up the radix tree reads the compact prefix-length (1 B) and parent
+// (4 B) arrays when they describe the traversal tree (textbook root), else
+// the traversal records and the codes.
+struct FeatTree {
+    const TravNode* trav;
+    const uint64_t* codes;
+    const uint8_t* pn;      // nullable
+    const int32_t* parent;  // nullable
+    int code_bits;
+    __device__ __forceinline__ int prefix(int node) const {
+        if (pn) return pn[node];
+        const float4 lo_l = trav[node].lo_l, lo_r = trav[node].lo_r;
+        return prefix_len(codes[w_int(lo_l.w)], codes[w_int(lo_r.w)], code_bits);
+    }
+    __device__ __forceinline__ int up(int node) const { return parent ? parent[node] : w_int(trav[node].hi_r.w); }
+};
 
-__global__ void __launch_bounds__(256) features_kernel(const TravNode* __restrict__ trav,
-                                                       const int32_t* __restrict__ leaf_parent,
-                                                       const uint64_t* __restrict__ codes,
+__global__ void __launch_bounds__(256) features_kernel(FeatTree tr, const int32_t* __restrict__ leaf_parent,
                                                        const int32_t* __restrict__ leaf_oct,
                                                        const sx_otnode* __restrict__ oct,
                                                        const float* __restrict__ bounds,
-                                                       const int32_t* __restrict__ d_counts, int code_bits,
-                                                       int64_t q_start, int64_t n_q, float* __restrict__ feats) {
+                                                       const int32_t* __restrict__ d_counts, int64_t q_start,
+                                                       int64_t n_q, float* __restrict__ feats) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n_q) return;
     const int U = d_counts[0];
     const int64_t u = q_start + t;
     float f[kIn] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (u < U && U >= 2) {
-        const int root_level = prefix_len(codes[0], codes[U - 1], code_bits) / 3;
+        const int root_level = tr.prefix(0) / 3;
         int c = leaf_parent[u] >> 1;
-        const int d0 = node_prefix(trav, codes, c, code_bits) / 3;
-        int pc = w_int(trav[c].hi_r.w);
-        int ppn = pc >= 0 ? node_prefix(trav, codes, pc, code_bits) : -1;
+        const int d0 = tr.prefix(c) / 3;
+        int pc = tr.up(c);
+        int ppn = pc >= 0 ? tr.prefix(pc) : -1;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int L = max(d0 - s, root_level);
             // highest ancestor whose prefix still holds 3L bits
             while (pc >= 0 && ppn >= 3 * L) {
                 c = pc;
-                pc = w_int(trav[c].hi_r.w);
-                ppn = pc >= 0 ? node_prefix(trav, codes, pc, code_bits) : -1;
+                pc = tr.up(c);
+                ppn = pc >= 0 ? tr.prefix(pc) : -1;
             }
-            const int cnt = w_int(trav[c].lo_r.w) - w_int(trav[c].lo_l.w) + 1;
+            const int cnt = w_int(tr.trav[c].lo_r.w) - w_int(tr.trav[c].lo_l.w) + 1;
             f[s] = __log2f(static_cast<float>(cnt));
         }
         f[4] = __log2f(bounds[2]) - static_cast<float>(d0);
@@ -504,7 +513,8 @@ struct sx_index;
 namespace sx {
 int index_feature_arrays(sx_index* ix, const TravNode** trav, const int32_t** leaf_parent, const uint64_t** codes,
                          const int32_t** leaf_oct, const sx_otnode** oct, const float** bounds,
-                         const int32_t** counts, int* code_bits, int64_t* limit);
+                         const int32_t** counts, const uint8_t** pn, const int32_t** parent, int* code_bits,
+                         int64_t* limit);
 }
 
 using sx::fail;
@@ -662,19 +672,21 @@ int sx_mlp_adam_step(sx_mlp* m, const float* d_feats, const float* d_targets, in
 // Features of index points [q_start, q_start + n_q) into d_feats (n_q x 8 floats).
 int sx_radius_features(sx_index* index, int64_t q_start, int64_t n_q, float* d_feats, void* stream) {
     const sx::TravNode* trav;
-    const int32_t *lp, *lo, *counts;
+    const int32_t *lp, *lo, *counts, *parent;
     const uint64_t* codes;
+    const uint8_t* pn;
     const sx_otnode* oct;
     const float* bounds;
     int cb;
     int64_t limit;
-    SX_TRY(sx::index_feature_arrays(index, &trav, &lp, &codes, &lo, &oct, &bounds, &counts, &cb, &limit));
+    SX_TRY(sx::index_feature_arrays(index, &trav, &lp, &codes, &lo, &oct, &bounds, &counts, &pn, &parent, &cb,
+                                    &limit));
     if (q_start < 0 || n_q < 0 || q_start + n_q > limit) return fail(SX_EINVAL, "sx_radius_features: bad query range");
     if (n_q == 0) return SX_OK;
     if (!d_feats || reinterpret_cast<uintptr_t>(d_feats) % 16 != 0)
         return fail(SX_EINVAL, "sx_radius_features: output must be 16-byte aligned");
-    sx::features_kernel<<<sx::grid_for(n_q, 256), 256, 0, sx::as_stream(stream)>>>(trav, lp, codes, lo, oct, bounds,
-                                                                                   counts, cb, q_start, n_q, d_feats);
+    sx::features_kernel<<<sx::grid_for(n_q, 256), 256, 0, sx::as_stream(stream)>>>(
+        sx::FeatTree{trav, codes, pn, parent, cb}, lp, lo, oct, bounds, counts, q_start, n_q, d_feats);
     SX_LAUNCH_CHECK("features_kernel");
     return SX_OK;
 }

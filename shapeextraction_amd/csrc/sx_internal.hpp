@@ -128,15 +128,24 @@ __device__ __forceinline__ int64_t scan_count(const int32_t* d_n, int64_t n_max)
     return n < 0 ? 0 : (n < n_max ? n : n_max);
 }
 
+// Element i of a tile is visited by wave i / (kScanItems * kWave) in row
+// (i / kWave) % kScanItems at lane i % kWave: every load and store of a row
+// is one contiguous run (coalesced), and a tile is still kScanTile
+// consecutive elements.
+__device__ __forceinline__ int64_t scan_index(int64_t tile_begin, int row) {
+    const int w = static_cast<int>(threadIdx.x) / kWave;
+    return tile_begin + static_cast<int64_t>(w) * (kScanItems * kWave) + row * kWave + lane_id();
+}
+
 template <typename In>
 __global__ void __launch_bounds__(kScanBlock) scan_reduce_kernel(In in, const int32_t* d_n, int64_t n_max,
                                                                  int* tile_sums) {
     const int64_t n = scan_count(d_n, n_max);
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + static_cast<int64_t>(threadIdx.x) * kScanItems;
+    const int64_t tile_begin = static_cast<int64_t>(blockIdx.x) * kScanTile;
     int s = 0;
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-        int64_t i = base + j;
+        const int64_t i = scan_index(tile_begin, j);
         if (i < n) s += in(i);
     }
     int total;
@@ -165,22 +174,24 @@ __global__ void __launch_bounds__(kScanBlock) scan_apply_kernel(In in, Out out, 
     const int64_t n = scan_count(d_n, n_max);
     const int64_t tile_begin = static_cast<int64_t>(blockIdx.x) * kScanTile;
     if (tile_begin >= n) return;
-    const int64_t base = tile_begin + static_cast<int64_t>(threadIdx.x) * kScanItems;
-    int vals[kScanItems];
-    int s = 0;
+    int vals[kScanItems], excl[kScanItems];
+    int run = 0;  // this wave's running sum over its rows
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-        int64_t i = base + j;
+        const int64_t i = scan_index(tile_begin, j);
         vals[j] = (i < n) ? in(i) : 0;
-        s += vals[j];
+        const int incl = wave_incl_sum(vals[j]);
+        excl[j] = run + incl - vals[j];
+        run += __shfl(incl, kWave - 1, kWave);
     }
+    // the waves' sums in order give each wave its offset inside the tile
     int total;
-    int run = block_excl_sum(s, &total) + tile_prefix[blockIdx.x];
+    const int wave_base = block_excl_sum(lane_id() == kWave - 1 ? run : 0, &total);
+    const int base = __shfl(wave_base, kWave - 1, kWave) + tile_prefix[blockIdx.x];
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-        int64_t i = base + j;
-        if (i < n) out(i, run, vals[j]);
-        run += vals[j];
+        const int64_t i = scan_index(tile_begin, j);
+        if (i < n) out(i, base + excl[j], vals[j]);
     }
 }
 
