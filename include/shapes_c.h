@@ -296,6 +296,64 @@ int sx_mlp_adam_step(sx_mlp* mlp, const float* d_feats, const float* d_targets, 
  * into d_feats (n_q x 8 floats, 16-byte aligned). */
 int sx_radius_features(sx_index* index, int64_t q_start, int64_t n_q, float* d_feats, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Batched clouds (no counterpart in the reference, which builds one cloud at a
+ * time: src/main.cu:80-107).  Cloud c is input points [offsets[c],
+ * offsets[c+1]).  Every stage runs once over all clouds; per-cloud results
+ * equal a standalone sx_index build of that cloud (textbook root).  Cloud c's
+ * unique points occupy slots [cloud_base[c], cloud_base[c] + cloud_size[c])
+ * of the per-slot arrays (cloud_base[c] a multiple of 64); neighbour rows and
+ * normals are per slot, with indices relative to the cloud's first slot, and
+ * rep[] is relative to the cloud's first input point.  kNN is plain (the
+ * learned radius needs octree features, which the batch does not build).
+ * ------------------------------------------------------------------------- */
+typedef struct sx_batch sx_batch;
+
+typedef struct sx_batch_view {
+    int32_t n_clouds;
+    int64_t n_input;
+    int64_t n_slots;            /* per-slot arrays hold n_slots entries       */
+    int64_t n_unique;           /* unique points over all clouds              */
+    const int64_t* cloud_offsets;  /* [n_clouds + 1] input offsets (device)   */
+    const int32_t* cloud_base;     /* [n_clouds] first slot                   */
+    const int32_t* cloud_size;     /* [n_clouds] unique points                */
+    const float* cloud_bounds;     /* [n_clouds][4]: min, max, range, 0       */
+    const uint64_t* codes;         /* [n_slots] unique Morton codes            */
+    const uint32_t* rep;           /* [n_slots] first input index (cloud-local) */
+    const float* points;           /* [n_slots][4] decoded points (x, y, z, 0) */
+    const uint8_t* prefix_n;       /* [n_slots] radix tree, slot numbering     */
+    const int32_t* left_child;
+    const int32_t* parent;
+} sx_batch_view;
+
+int sx_batch_create(sx_batch** out, int64_t max_points, int max_clouds, int code_bits);
+int sx_batch_destroy(sx_batch* batch);
+/* Host offsets [n_clouds + 1], offsets[0] = 0, non-decreasing; copied to the device (synchronous). */
+int sx_batch_set_clouds(sx_batch* batch, const int64_t* h_offsets, int n_clouds, void* stream);
+/* bucket: leaf-range size of the kNN; queue_blocks: deferred-query workgroups (0 = default). */
+int sx_batch_set_tuning(sx_batch* batch, int bucket, int sort_algo, int queue_blocks);
+/* Asynchronous, no host synchronisation, capturable. */
+int sx_batch_build(sx_batch* batch, const float* d_x, const float* d_y, const float* d_z, void* stream);
+/* Synchronises; SX_EDATA when an input coordinate is not finite. */
+int sx_batch_get_view(sx_batch* batch, sx_batch_view* view, void* stream);
+/* d_out: n_slots x k int32 (16-byte aligned); rows of padding slots are not written. */
+int sx_batch_knn(sx_batch* batch, int k, int32_t* d_out, void* stream);
+int sx_batch_normals(sx_batch* batch, const int32_t* d_nn, int k, int precision, float vx, float vy, float vz,
+                     float* d_normals, uint8_t* d_flags, void* stream);
+/* Queries the kNN tiles handed to the per-query search since the last reset. */
+int sx_batch_deferred(sx_batch* batch, int64_t* deferred, int reset, void* stream);
+/* One hipGraph of build + kNN (k) + FP32 normals (viewpoint at the origin) on
+ * the given created stream; launching it reruns the whole batch. */
+int sx_batch_graph_create(sx_batch* batch, const float* d_x, const float* d_y, const float* d_z, int k,
+                          int32_t* d_nn, float* d_normals, void** graph, void* stream);
+int sx_batch_graph_launch(void* graph, void* stream);
+int sx_batch_graph_destroy(void* graph);
+/* Times the built batch's kNN for every (bucket, queue_blocks) candidate and
+ * keeps the fastest.  h_ms (nullable): n_buckets x n_queue times in ms. */
+int sx_batch_autotune(sx_batch* batch, int k, int32_t* d_scratch_nn, const int* h_buckets, int n_buckets,
+                      const int* h_queue_blocks, int n_queue, int reps, float* h_ms, int* best_bucket,
+                      int* best_queue_blocks, void* stream);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

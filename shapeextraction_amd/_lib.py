@@ -72,6 +72,25 @@ class IndexView(C.Structure):
     ]
 
 
+class BatchView(C.Structure):
+    _fields_ = [
+        ("n_clouds", C.c_int32),
+        ("n_input", C.c_int64),
+        ("n_slots", C.c_int64),
+        ("n_unique", C.c_int64),
+        ("cloud_offsets", C.c_void_p),
+        ("cloud_base", C.c_void_p),
+        ("cloud_size", C.c_void_p),
+        ("cloud_bounds", C.c_void_p),
+        ("codes", C.c_void_p),
+        ("rep", C.c_void_p),
+        ("points", C.c_void_p),
+        ("prefix_n", C.c_void_p),
+        ("left_child", C.c_void_p),
+        ("parent", C.c_void_p),
+    ]
+
+
 P = C.c_void_p
 I = C.c_int
 I64 = C.c_int64
@@ -140,6 +159,21 @@ SIGNATURES = {
     "sx_mlp_backward": (I, [P, P, P, I64, P, P, P]),
     "sx_mlp_adam_step": (I, [P, P, P, I64, F, F, P, P]),
     "sx_radius_features": (I, [P, I64, I64, P, P]),
+    # batched clouds
+    "sx_batch_create": (I, [C.POINTER(P), I64, I, I]),
+    "sx_batch_destroy": (I, [P]),
+    "sx_batch_set_clouds": (I, [P, C.POINTER(C.c_int64), I, P]),
+    "sx_batch_set_tuning": (I, [P, I, I, I]),
+    "sx_batch_build": (I, [P, P, P, P, P]),
+    "sx_batch_get_view": (I, [P, C.POINTER(BatchView), P]),
+    "sx_batch_knn": (I, [P, I, P, P]),
+    "sx_batch_normals": (I, [P, P, I, I, F, F, F, P, P, P]),
+    "sx_batch_deferred": (I, [P, C.POINTER(C.c_int64), I, P]),
+    "sx_batch_graph_create": (I, [P, P, P, P, I, P, P, C.POINTER(P), P]),
+    "sx_batch_graph_launch": (I, [P, P]),
+    "sx_batch_graph_destroy": (I, [P]),
+    "sx_batch_autotune": (I, [P, I, P, C.POINTER(C.c_int), I, C.POINTER(C.c_int), I, I, C.POINTER(C.c_float),
+                              C.POINTER(C.c_int), C.POINTER(C.c_int), P]),
 }
 
 _lib = None

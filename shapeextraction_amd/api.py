@@ -320,8 +320,144 @@ class Index:
         return r.value
 
 
+class Batch:
+    """Many clouds in one index (sx_batch): cloud c is input points
+    [offsets[c], offsets[c+1]).  Per-cloud results equal a standalone Index
+    of that cloud; per-slot arrays hold cloud c at [base[c], base[c]+size[c])."""
+
+    def __init__(self, max_points: int, max_clouds: int, code_bits: int = 63):
+        h = C.c_void_p()
+        call("sx_batch_create", C.byref(h), int(max_points), int(max_clouds), code_bits)
+        self.handle = h.value
+        self._lib = _lib.load()
+        self._view = None
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.sx_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_clouds(self, offsets, stream=None) -> "Batch":
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        call("sx_batch_set_clouds", self.handle, off.ctypes.data_as(C.POINTER(C.c_int64)), len(off) - 1, stream)
+        self.offsets = off
+        self._view = None
+        return self
+
+    def set_tuning(self, bucket: int = 16, sort: int = _lib.SX_SORT_ONESWEEP, queue_blocks: int = 0):
+        call("sx_batch_set_tuning", self.handle, bucket, sort, queue_blocks)
+
+    def build(self, x, y, z, stream=None) -> "Batch":
+        dx, dy, dz = _dev(x, np.float32), _dev(y, np.float32), _dev(z, np.float32)
+        self._inputs = (dx, dy, dz)
+        call("sx_batch_build", self.handle, dx.ptr, dy.ptr, dz.ptr, stream)
+        self._view = None
+        return self
+
+    def view(self, stream=None) -> _lib.BatchView:
+        v = _lib.BatchView()
+        call("sx_batch_get_view", self.handle, C.byref(v), stream)
+        self._view = v
+        return v
+
+    def table(self) -> dict:
+        """Per-cloud base slot, unique count, bounds (host copies)."""
+        v = self._view or self.view()
+        n = int(v.n_clouds)
+        return {
+            "n_slots": int(v.n_slots),
+            "n_unique": int(v.n_unique),
+            "base": fetch(v.cloud_base, n, np.int32),
+            "size": fetch(v.cloud_size, n, np.int32),
+            "bounds": fetch(v.cloud_bounds, 4 * n, np.float32).reshape(n, 4),
+        }
+
+    def fetch(self) -> dict:
+        """Copy every per-slot array to the host (tests)."""
+        v = self._view or self.view()
+        S = int(v.n_slots)
+        out = self.table()
+        out.update({
+            "codes": fetch(v.codes, S, np.uint64),
+            "rep": fetch(v.rep, S, np.uint32),
+            "points": fetch(v.points, 4 * S, np.float32).reshape(S, 4)[:, :3].copy(),
+            "prefix_n": fetch(v.prefix_n, S, np.uint8),
+            "left_child": fetch(v.left_child, S, np.int32),
+            "parent": fetch(v.parent, S, np.int32),
+        })
+        return out
+
+    def knn(self, k: int, out: DeviceArray | None = None, stream=None) -> DeviceArray:
+        v = self._view or self.view()
+        if out is None:
+            out = DeviceArray((int(v.n_slots), k), np.int32)
+        call("sx_batch_knn", self.handle, k, out.ptr, stream)
+        return out
+
+    def normals(self, nn: DeviceArray, k: int, precision: int = SX_PREC_FP32, viewpoint=(0.0, 0.0, 0.0),
+                out: DeviceArray | None = None, flags: DeviceArray | None = None, stream=None) -> DeviceArray:
+        v = self._view or self.view()
+        if out is None:
+            out = DeviceArray((int(v.n_slots), 3), np.float32)
+        call("sx_batch_normals", self.handle, nn.ptr, k, precision, *viewpoint, out.ptr,
+             flags.ptr if flags is not None else None, stream)
+        return out
+
+    def deferred(self, reset: bool = True, stream=None) -> int:
+        r = C.c_int64(0)
+        call("sx_batch_deferred", self.handle, C.byref(r), 1 if reset else 0, stream)
+        return r.value
+
+    def graph(self, x: DeviceArray, y: DeviceArray, z: DeviceArray, k: int, nn: DeviceArray, normals: DeviceArray,
+              stream) -> "BatchGraph":
+        g = C.c_void_p()
+        call("sx_batch_graph_create", self.handle, x.ptr, y.ptr, z.ptr, k, nn.ptr, normals.ptr, C.byref(g), stream)
+        self._view = None
+        return BatchGraph(g.value, (x, y, z, nn, normals))
+
+    def autotune(self, k: int, scratch: DeviceArray, buckets=(8, 16, 32), queue_blocks=(0, 2048), reps: int = 2,
+                 stream=None) -> dict:
+        b = (C.c_int * len(buckets))(*buckets)
+        q = (C.c_int * len(queue_blocks))(*queue_blocks)
+        ms = (C.c_float * (len(buckets) * len(queue_blocks)))()
+        bb, bq = C.c_int(0), C.c_int(0)
+        call("sx_batch_autotune", self.handle, k, scratch.ptr, b, len(buckets), q, len(queue_blocks), reps, ms,
+             C.byref(bb), C.byref(bq), stream)
+        return {"bucket": bb.value, "queue_blocks": bq.value,
+                "ms": np.array(list(ms), np.float32).reshape(len(buckets), len(queue_blocks))}
+
+
+class BatchGraph:
+    """A captured batch pipeline (build + kNN + FP32 normals)."""
+
+    def __init__(self, handle, keep):
+        self.handle = handle
+        self._keep = keep
+        self._lib = _lib.load()
+
+    def launch(self, stream):
+        call("sx_batch_graph_launch", self.handle, stream)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.sx_batch_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 __all__ = [
-    "Index", "Workload", "WORKLOADS", "OTNODE_DTYPE", "bbox", "morton_encode", "morton_decode", "sort_pairs",
+    "Index", "Batch", "BatchGraph", "Workload", "WORKLOADS", "OTNODE_DTYPE", "bbox", "morton_encode", "morton_decode", "sort_pairs",
     "unique", "radix_tree", "octree", "check_counts", "gen_sphere", "gen_scene", "gen_gmm_scene",
     "gen_batch_sizes", "gen_degenerate", "fnv64", "SX_KNN_PLAIN", "SX_KNN_RADIUS", "SX_PREC_FP32", "SX_PREC_FP64",
     "SX_ROOT_TEXTBOOK", "SX_ROOT_REFERENCE", "SX_KNN_ALGO_TILES", "SX_KNN_ALGO_LANE", "SX_SORT_ONESWEEP",

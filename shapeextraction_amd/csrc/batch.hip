@@ -1,0 +1,498 @@
+// sx_batch: many independent clouds of uneven size through one set of
+// launches.  Every stage runs once over all clouds (no per-cloud launches), so
+// the whole batch is a short, fixed sequence of kernels that is captured into
+// one hipGraph.  Each cloud's results equal those of a standalone build of
+// that cloud (the reference builds clouds one by one: src/main.cu:80-107).
+//
+//   per-cloud cube       seg_bbox_kernel: one block per cloud
+//   codes                seg_encode_kernel with the point's cloud's cube
+//   segmented sort       stable radix sort by code, then by cloud id: the
+//                        order is (cloud, code, input index), so every cloud's
+//                        points stay in their input block [off[c], off[c+1])
+//   duplicates           removed within a cloud only (scan over the pair)
+//   cloud table          unique points per cloud; cloud c's unique points go
+//                        to slots [base[c], base[c] + size[c]), base[c] a
+//                        multiple of 64, so every 64-query tile is one cloud's
+//   radix trees          karras_kernel per slot with the cloud map: cloud c's
+//                        internal node i sits at slot base[c] + i, exactly a
+//                        standalone tree shifted by base[c]
+//   boxes, kNN, normals  the single-cloud kernels with the cloud map
+//
+// Not built per cloud: the octree (the batched path stops at neighbours and
+// normals; the learned radius needs octree features, so batched kNN is plain).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "sx_internal.hpp"
+
+namespace sx {
+int seg_bbox_launch(const float* x, const float* y, const float* z, const int64_t* off, int n_clouds, float* bounds,
+                    int32_t* d_bad, hipStream_t st);
+int seg_encode_launch(const float* x, const float* y, const float* z, int64_t n, const int64_t* off, int n_clouds,
+                      const float* bounds, int code_bits, uint64_t* codes, hipStream_t st);
+int seg_decode_launch(const uint64_t* codes, const int32_t* d_slots, int64_t n_max, CloudMap cm, const float* bounds,
+                      int code_bits, float4* pts16, hipStream_t st);
+size_t sort_ws_bytes(int64_t n);
+int sort_pairs_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n, int key_bits, uint64_t* keys_out,
+                      uint32_t* vals_out, void* ws, int algo, hipStream_t st);
+int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
+                  uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
+                  int32_t* node_end, int32_t* leaf_parent, TravNode* trav, hipStream_t st, CloudMap cm);
+size_t box_ws_bytes(int64_t n_max);
+int child_boxes_launch(const float4* pts, const int32_t* d_count, int64_t n_max, TravNode* trav, void* ws,
+                       hipStream_t st, CloudMap cm);
+int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
+                     int k, const float* radius2, int bucket, int32_t* out, int32_t* queue, int32_t* queue_count,
+                     int32_t* deferred_total, hipStream_t st, CloudMap cm, int queue_blocks);
+int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn, int64_t q_start, int64_t n_q, int k,
+                   int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st,
+                   CloudMap cm);
+
+// counts[] slots
+enum { kBSlots = 0, kBUnique = 1, kBBad = 2, kBDeferred = 3, kBQueue = 4 };
+
+__global__ void __launch_bounds__(256) cloud_keys_kernel(const uint32_t* __restrict__ perm, int64_t n,
+                                                         const int64_t* __restrict__ off, int n_clouds,
+                                                         uint64_t* __restrict__ keys) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = static_cast<uint64_t>(cloud_of(off, n_clouds, perm[i]));
+}
+
+__global__ void __launch_bounds__(256) gather_codes_kernel(const uint64_t* __restrict__ codes,
+                                                           const uint32_t* __restrict__ perm, int64_t n,
+                                                           uint64_t* __restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = codes[perm[i]];
+}
+
+// First occurrence of a (cloud, code) pair in (cloud, code) order.
+struct BatchUniqueIn {
+    const uint64_t* cloud;
+    const uint64_t* code;
+    __device__ int operator()(int64_t i) const {
+        return (i == 0 || cloud[i] != cloud[i - 1] || code[i] != code[i - 1]) ? 1 : 0;
+    }
+};
+struct BatchUniqueOut {
+    int32_t* uos;  // batch-wide unique id of each sorted position
+    __device__ void operator()(int64_t i, int ex, int v) const { uos[i] = ex + v - 1; }
+};
+
+// One block: unique points per cloud, 64-aligned first slots, the slot total
+// and the cloud of every 64-slot tile.  Cloud c's sorted positions are its
+// input block [off[c], off[c+1]).
+__global__ void __launch_bounds__(kScanBlock) cloud_table_kernel(const int64_t* __restrict__ off, int n_clouds,
+                                                                 const int32_t* __restrict__ uos,
+                                                                 int32_t* __restrict__ size, int32_t* __restrict__ base,
+                                                                 int32_t* __restrict__ counts,
+                                                                 int32_t* __restrict__ tile_cloud) {
+    int carry = 0;
+    for (int c0 = 0; c0 < n_clouds; c0 += kScanBlock) {
+        const int c = c0 + static_cast<int>(threadIdx.x);
+        int u = 0;
+        if (c < n_clouds && off[c + 1] > off[c]) u = uos[off[c + 1] - 1] - uos[off[c]] + 1;
+        const int slots = (u + kCloudAlign - 1) / kCloudAlign * kCloudAlign;
+        int total;
+        const int ex = block_excl_sum(slots, &total);
+        if (c < n_clouds) {
+            size[c] = u;
+            base[c] = carry + ex;
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) {
+        counts[kBSlots] = carry;
+        counts[kBUnique] = n_clouds > 0 ? uos[off[n_clouds] - 1] + 1 : 0;
+    }
+    __syncthreads();  // base[] and size[] of this block are visible to it
+    for (int c = static_cast<int>(threadIdx.x); c < n_clouds; c += kScanBlock) {
+        const int t0 = base[c] / kCloudAlign, t1 = (base[c] + size[c] + kCloudAlign - 1) / kCloudAlign;
+        for (int t = t0; t < t1; ++t) tile_cloud[t] = c;
+    }
+}
+
+// Unique (cloud, code) pairs to their cloud's slots; rep = the first input
+// index of the pair, relative to the cloud's first input point.
+__global__ void __launch_bounds__(256) compact_kernel(int64_t n, const uint64_t* __restrict__ cloud,
+                                                      const uint64_t* __restrict__ code,
+                                                      const uint32_t* __restrict__ perm,
+                                                      const int32_t* __restrict__ uos, const int64_t* __restrict__ off,
+                                                      const int32_t* __restrict__ base, uint64_t* __restrict__ ucodes,
+                                                      uint32_t* __restrict__ rep) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!(i == 0 || cloud[i] != cloud[i - 1] || code[i] != code[i - 1])) return;
+    const int c = static_cast<int>(cloud[i]);
+    const int64_t slot = base[c] + (uos[i] - uos[off[c]]);
+    ucodes[slot] = code[i];
+    rep[slot] = static_cast<uint32_t>(perm[i] - off[c]);
+}
+
+}  // namespace sx
+
+using sx::fail;
+
+struct sx_batch {
+    int64_t cap = 0;        // input points
+    int max_clouds = 0;
+    int64_t slot_cap = 0;   // cap + (kCloudAlign - 1) * max_clouds
+    int code_bits = 63;
+    int bucket = 16;
+    int sort_algo = SX_SORT_ONESWEEP;
+    int queue_blocks = 0;   // deferred-query workers; 0 = default
+    void* mem = nullptr;
+    // per input point / sorted position
+    uint64_t *codes_in = nullptr, *codes_a = nullptr, *keys_c = nullptr, *keys_s = nullptr, *codes_s = nullptr;
+    uint32_t *perm_a = nullptr, *perm = nullptr;
+    int32_t* uos = nullptr;
+    // per cloud
+    int64_t* off = nullptr;
+    float* bounds = nullptr;
+    int32_t *size = nullptr, *base = nullptr;
+    // per slot
+    int32_t* tile_cloud = nullptr;
+    uint64_t* ucodes = nullptr;
+    uint32_t* rep = nullptr;
+    float4* pts16 = nullptr;
+    uint8_t* pn = nullptr;
+    int32_t *lc = nullptr, *parent = nullptr, *queue = nullptr;
+    sx::TravNode* trav = nullptr;
+    int32_t* counts = nullptr;
+    void* ws = nullptr;
+    // host state
+    int n_clouds = 0;
+    int64_t n_input = 0;
+    bool built = false;
+    sx::CloudMap map() const { return sx::CloudMap{tile_cloud, base, size}; }
+};
+
+namespace {
+size_t batch_ws_bytes(int64_t cap, int64_t slot_cap) {
+    size_t ws = std::max(sx::sort_ws_bytes(cap), sx::scan_ws_bytes(cap));
+    return std::max(ws, sx::box_ws_bytes(slot_cap));
+}
+
+size_t batch_layout(sx_batch* b, void* base) {
+    sx::Carver c(base);
+    const int64_t n = b->cap, s = b->slot_cap, mc = b->max_clouds;
+    b->codes_in = c.take<uint64_t>(n);
+    b->codes_a = c.take<uint64_t>(n);
+    b->keys_c = c.take<uint64_t>(n);
+    b->keys_s = c.take<uint64_t>(n);
+    b->codes_s = c.take<uint64_t>(n);
+    b->perm_a = c.take<uint32_t>(n);
+    b->perm = c.take<uint32_t>(n);
+    b->uos = c.take<int32_t>(n);
+    b->off = c.take<int64_t>(mc + 1);
+    b->bounds = c.take<float>(4 * mc);
+    b->size = c.take<int32_t>(mc);
+    b->base = c.take<int32_t>(mc);
+    b->tile_cloud = c.take<int32_t>(s / sx::kCloudAlign + 1);
+    b->ucodes = c.take<uint64_t>(s);
+    b->rep = c.take<uint32_t>(s);
+    b->pts16 = c.take<float4>(s);
+    b->pn = c.take<uint8_t>(s);
+    b->lc = c.take<int32_t>(s);
+    b->parent = c.take<int32_t>(s);
+    b->queue = c.take<int32_t>(s);
+    b->trav = c.take<sx::TravNode>(s);
+    b->counts = c.take<int32_t>(8);
+    b->ws = c.take<char>(batch_ws_bytes(n, s));
+    return c.off;
+}
+
+int check_batch(const sx_batch* b) {
+    if (!b) return fail(SX_EINVAL, "sx_batch: null batch");
+    if (!b->built) return fail(SX_ESTATE, "sx_batch: build the batch first");
+    return SX_OK;
+}
+
+int cloud_bits(int n_clouds) {
+    int bits = 1;
+    while ((1ll << bits) < n_clouds) ++bits;
+    return bits;
+}
+
+int batch_build(sx_batch* b, const float* x, const float* y, const float* z, hipStream_t st) {
+    const int64_t n = b->n_input;
+    const int nc = b->n_clouds;
+    const int cb = b->code_bits;
+    // slot / unique / non-finite counts; the deferred-query total accumulates
+    SX_TRY(sx::fill_i32(b->counts, 0, sx::kBDeferred, st));
+    SX_TRY(sx::seg_bbox_launch(x, y, z, b->off, nc, b->bounds, b->counts + sx::kBBad, st));
+    SX_TRY(sx::seg_encode_launch(x, y, z, n, b->off, nc, b->bounds, cb, b->codes_in, st));
+    // (cloud, code, input index) order: by code, then stably by cloud
+    SX_TRY(sx::sort_pairs_launch(b->codes_in, nullptr, n, cb, b->codes_a, b->perm_a, b->ws, b->sort_algo, st));
+    sx::cloud_keys_kernel<<<sx::grid_for(n, 256), 256, 0, st>>>(b->perm_a, n, b->off, nc, b->keys_c);
+    SX_LAUNCH_CHECK("cloud_keys_kernel");
+    SX_TRY(sx::sort_pairs_launch(b->keys_c, b->perm_a, n, cloud_bits(nc), b->keys_s, b->perm, b->ws, b->sort_algo,
+                                 st));
+    sx::gather_codes_kernel<<<sx::grid_for(n, 256), 256, 0, st>>>(b->codes_in, b->perm, n, b->codes_s);
+    SX_LAUNCH_CHECK("gather_codes_kernel");
+    SX_TRY(sx::device_scan(sx::BatchUniqueIn{b->keys_s, b->codes_s}, sx::BatchUniqueOut{b->uos}, nullptr, n, nullptr,
+                           b->ws, st));
+    sx::cloud_table_kernel<<<1, sx::kScanBlock, 0, st>>>(b->off, nc, b->uos, b->size, b->base, b->counts,
+                                                         b->tile_cloud);
+    SX_LAUNCH_CHECK("cloud_table_kernel");
+    sx::compact_kernel<<<sx::grid_for(n, 256), 256, 0, st>>>(n, b->keys_s, b->codes_s, b->perm, b->uos, b->off,
+                                                              b->base, b->ucodes, b->rep);
+    SX_LAUNCH_CHECK("compact_kernel");
+    const int32_t* slots = b->counts + sx::kBSlots;
+    SX_TRY(sx::seg_decode_launch(b->ucodes, slots, b->slot_cap, b->map(), b->bounds, cb, b->pts16, st));
+    SX_TRY(sx::karras_launch(b->ucodes, slots, b->slot_cap, cb, SX_ROOT_TEXTBOOK, b->pn, b->lc, nullptr, nullptr,
+                             b->parent, nullptr, nullptr, b->trav, st, b->map()));
+    SX_TRY(sx::child_boxes_launch(b->pts16, slots, b->slot_cap, b->trav, b->ws, st, b->map()));
+    return SX_OK;
+}
+
+int batch_knn(sx_batch* b, int k, int32_t* out, hipStream_t st) {
+    return sx::knn_tiles_launch(b->pts16, b->trav, b->counts + sx::kBSlots, 0, b->slot_cap, k, nullptr, b->bucket,
+                                out, b->queue, b->counts + sx::kBQueue, b->counts + sx::kBDeferred, st, b->map(),
+                                b->queue_blocks);
+}
+
+int batch_normals(sx_batch* b, const int32_t* nn, int k, int precision, float vx, float vy, float vz, float* normals,
+                  uint8_t* flags, hipStream_t st) {
+    return sx::normals_launch(b->pts16, b->counts + sx::kBSlots, nn, 0, b->slot_cap, k, precision, vx, vy, vz,
+                              normals, flags, st, b->map());
+}
+
+bool valid_k(int k) { return k == 8 || k == 16 || k == 24 || k == 32; }
+
+struct BatchGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+}  // namespace
+
+extern "C" {
+
+int sx_batch_create(sx_batch** out, int64_t max_points, int max_clouds, int code_bits) {
+    if (!out) return fail(SX_EINVAL, "sx_batch_create: null output");
+    *out = nullptr;
+    if (max_points < 1 || max_points >= (1ll << 30)) return fail(SX_EINVAL, "sx_batch_create: max_points must be in [1, 2^30)");
+    if (max_clouds < 1 || max_clouds > (1 << 16)) return fail(SX_EINVAL, "sx_batch_create: max_clouds must be in [1, 65536]");
+    if (code_bits != 63 && code_bits != 30) return fail(SX_EINVAL, "sx_batch_create: code_bits must be 30 or 63");
+    auto* b = new sx_batch();
+    b->cap = max_points;
+    b->max_clouds = max_clouds;
+    b->slot_cap = max_points + static_cast<int64_t>(sx::kCloudAlign - 1) * max_clouds;
+    b->code_bits = code_bits;
+    const size_t bytes = batch_layout(b, nullptr);
+    if (hipMalloc(&b->mem, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        delete b;
+        return fail(SX_ENOMEM, "sx_batch_create: device allocation failed");
+    }
+    batch_layout(b, b->mem);
+    int rc = sx::fill_i32(b->counts, 0, 8, nullptr);
+    if (rc == SX_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(SX_EHIP, "sx_batch_create: sync failed");
+    if (rc != SX_OK) {
+        (void)hipFree(b->mem);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return SX_OK;
+}
+
+int sx_batch_destroy(sx_batch* b) {
+    if (!b) return SX_OK;
+    if (b->mem) SX_HIP(hipFree(b->mem));
+    delete b;
+    return SX_OK;
+}
+
+int sx_batch_set_clouds(sx_batch* b, const int64_t* h_offsets, int n_clouds, void* stream) {
+    if (!b || !h_offsets) return fail(SX_EINVAL, "sx_batch_set_clouds: null argument");
+    if (n_clouds < 1 || n_clouds > b->max_clouds) return fail(SX_EINVAL, "sx_batch_set_clouds: bad cloud count");
+    if (h_offsets[0] != 0) return fail(SX_EINVAL, "sx_batch_set_clouds: offsets must start at 0");
+    for (int c = 0; c < n_clouds; ++c)
+        if (h_offsets[c + 1] < h_offsets[c]) return fail(SX_EINVAL, "sx_batch_set_clouds: offsets must not decrease");
+    const int64_t n = h_offsets[n_clouds];
+    if (n < 1 || n > b->cap) return fail(SX_ERANGE, "sx_batch_set_clouds: point total outside [1, max_points]");
+    hipStream_t st = sx::as_stream(stream);
+    SX_HIP(hipMemcpyAsync(b->off, h_offsets, sizeof(int64_t) * (n_clouds + 1), hipMemcpyHostToDevice, st));
+    SX_HIP(hipStreamSynchronize(st));
+    b->n_clouds = n_clouds;
+    b->n_input = n;
+    b->built = false;
+    return SX_OK;
+}
+
+int sx_batch_set_tuning(sx_batch* b, int bucket, int sort_algo, int queue_blocks) {
+    if (!b) return fail(SX_EINVAL, "sx_batch_set_tuning: null batch");
+    if (bucket < 1 || bucket > 256) return fail(SX_EINVAL, "sx_batch_set_tuning: bucket must be in [1, 256]");
+    if (sort_algo != SX_SORT_ONESWEEP && sort_algo != SX_SORT_THREE_KERNEL)
+        return fail(SX_EINVAL, "sx_batch_set_tuning: bad sort algorithm");
+    if (queue_blocks < 0 || queue_blocks > (1 << 16)) return fail(SX_EINVAL, "sx_batch_set_tuning: bad queue_blocks");
+    b->bucket = bucket;
+    b->sort_algo = sort_algo;
+    b->queue_blocks = queue_blocks;
+    return SX_OK;
+}
+
+int sx_batch_build(sx_batch* b, const float* d_x, const float* d_y, const float* d_z, void* stream) {
+    if (!b) return fail(SX_EINVAL, "sx_batch_build: null batch");
+    if (b->n_clouds < 1) return fail(SX_ESTATE, "sx_batch_build: set the clouds first");
+    if (!d_x || !d_y || !d_z) return fail(SX_EINVAL, "sx_batch_build: null coordinates");
+    SX_TRY(batch_build(b, d_x, d_y, d_z, sx::as_stream(stream)));
+    b->built = true;
+    return SX_OK;
+}
+
+int sx_batch_get_view(sx_batch* b, sx_batch_view* view, void* stream) {
+    SX_TRY(check_batch(b));
+    if (!view) return fail(SX_EINVAL, "sx_batch_get_view: null view");
+    hipStream_t st = sx::as_stream(stream);
+    int32_t h[8];
+    SX_HIP(hipMemcpyAsync(h, b->counts, sizeof(h), hipMemcpyDeviceToHost, st));
+    SX_HIP(hipStreamSynchronize(st));
+    if (h[sx::kBBad] != 0) return fail(SX_EDATA, "sx_batch_get_view: non-finite input coordinates");
+    view->n_clouds = b->n_clouds;
+    view->n_input = b->n_input;
+    view->n_slots = h[sx::kBSlots];
+    view->n_unique = h[sx::kBUnique];
+    view->cloud_offsets = b->off;
+    view->cloud_base = b->base;
+    view->cloud_size = b->size;
+    view->cloud_bounds = b->bounds;
+    view->codes = b->ucodes;
+    view->rep = b->rep;
+    view->points = reinterpret_cast<const float*>(b->pts16);
+    view->prefix_n = b->pn;
+    view->left_child = b->lc;
+    view->parent = b->parent;
+    return SX_OK;
+}
+
+int sx_batch_knn(sx_batch* b, int k, int32_t* d_out, void* stream) {
+    SX_TRY(check_batch(b));
+    if (!valid_k(k)) return fail(SX_EINVAL, "sx_batch_knn: k must be 8, 16, 24 or 32");
+    if (!d_out || reinterpret_cast<uintptr_t>(d_out) % 16 != 0)
+        return fail(SX_EINVAL, "sx_batch_knn: output must be a 16-byte aligned device buffer");
+    return batch_knn(b, k, d_out, sx::as_stream(stream));
+}
+
+int sx_batch_normals(sx_batch* b, const int32_t* d_nn, int k, int precision, float vx, float vy, float vz,
+                     float* d_normals, uint8_t* d_flags, void* stream) {
+    SX_TRY(check_batch(b));
+    if (!valid_k(k)) return fail(SX_EINVAL, "sx_batch_normals: k must be 8, 16, 24 or 32");
+    if (!d_nn || !d_normals) return fail(SX_EINVAL, "sx_batch_normals: null buffer");
+    return batch_normals(b, d_nn, k, precision, vx, vy, vz, d_normals, d_flags, sx::as_stream(stream));
+}
+
+int sx_batch_deferred(sx_batch* b, int64_t* deferred, int reset, void* stream) {
+    if (!b || !deferred) return fail(SX_EINVAL, "sx_batch_deferred: null argument");
+    hipStream_t st = sx::as_stream(stream);
+    int32_t h = 0;
+    SX_HIP(hipMemcpyAsync(&h, b->counts + sx::kBDeferred, 4, hipMemcpyDeviceToHost, st));
+    SX_HIP(hipStreamSynchronize(st));
+    *deferred = h;
+    if (reset) SX_TRY(sx::fill_i32(b->counts + sx::kBDeferred, 0, 1, st));
+    return SX_OK;
+}
+
+int sx_batch_graph_create(sx_batch* b, const float* d_x, const float* d_y, const float* d_z, int k, int32_t* d_nn,
+                          float* d_normals, void** graph, void* stream) {
+    if (!b || !graph) return fail(SX_EINVAL, "sx_batch_graph_create: null argument");
+    *graph = nullptr;
+    if (b->n_clouds < 1) return fail(SX_ESTATE, "sx_batch_graph_create: set the clouds first");
+    if (!valid_k(k)) return fail(SX_EINVAL, "sx_batch_graph_create: k must be 8, 16, 24 or 32");
+    if (!d_x || !d_y || !d_z || !d_nn || !d_normals || reinterpret_cast<uintptr_t>(d_nn) % 16 != 0)
+        return fail(SX_EINVAL, "sx_batch_graph_create: null or misaligned buffer");
+    hipStream_t st = sx::as_stream(stream);
+    if (!st) return fail(SX_EINVAL, "sx_batch_graph_create: capture needs a created stream, not the null stream");
+    auto* g = new BatchGraph();
+    SX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = batch_build(b, d_x, d_y, d_z, st);
+    if (rc == SX_OK) rc = batch_knn(b, k, d_nn, st);
+    if (rc == SX_OK) rc = batch_normals(b, d_nn, k, SX_PREC_FP32, 0.0f, 0.0f, 0.0f, d_normals, nullptr, st);
+    hipError_t e = hipStreamEndCapture(st, &g->graph);
+    if (rc == SX_OK && e != hipSuccess) rc = fail(SX_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    if (rc == SX_OK) {
+        e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = fail(SX_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    if (rc != SX_OK) {
+        (void)hipGetLastError();
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g;
+        return rc;
+    }
+    b->built = true;
+    *graph = g;
+    return SX_OK;
+}
+
+int sx_batch_graph_launch(void* graph, void* stream) {
+    if (!graph) return fail(SX_EINVAL, "sx_batch_graph_launch: null graph");
+    SX_HIP(hipGraphLaunch(static_cast<BatchGraph*>(graph)->exec, sx::as_stream(stream)));
+    return SX_OK;
+}
+
+int sx_batch_graph_destroy(void* graph) {
+    if (!graph) return SX_OK;
+    auto* g = static_cast<BatchGraph*>(graph);
+    if (g->exec) SX_HIP(hipGraphExecDestroy(g->exec));
+    if (g->graph) SX_HIP(hipGraphDestroy(g->graph));
+    delete g;
+    return SX_OK;
+}
+
+// Times the kNN of the built batch for every (bucket, deferred-worker grid)
+// pair in the candidate lists (reps runs each after one warm-up), keeps the
+// fastest on the batch and reports it.  ms (nullable) receives n_buckets x
+// n_queue times in milliseconds, row-major.
+int sx_batch_autotune(sx_batch* b, int k, int32_t* d_scratch_nn, const int* h_buckets, int n_buckets,
+                      const int* h_queue_blocks, int n_queue, int reps, float* h_ms, int* best_bucket,
+                      int* best_queue_blocks, void* stream) {
+    SX_TRY(check_batch(b));
+    if (!valid_k(k)) return fail(SX_EINVAL, "sx_batch_autotune: k must be 8, 16, 24 or 32");
+    if (!d_scratch_nn || !h_buckets || !h_queue_blocks || n_buckets < 1 || n_queue < 1 || reps < 1)
+        return fail(SX_EINVAL, "sx_batch_autotune: bad argument");
+    hipStream_t st = sx::as_stream(stream);
+    hipEvent_t e0, e1;
+    SX_HIP(hipEventCreate(&e0));
+    SX_HIP(hipEventCreate(&e1));
+    const int keep_bucket = b->bucket, keep_queue = b->queue_blocks;
+    float best = 1e30f;
+    int bb = keep_bucket, bq = keep_queue;
+    int rc = SX_OK;
+    for (int i = 0; i < n_buckets && rc == SX_OK; ++i) {
+        for (int j = 0; j < n_queue && rc == SX_OK; ++j) {
+            rc = sx_batch_set_tuning(b, h_buckets[i], b->sort_algo, h_queue_blocks[j]);
+            if (rc == SX_OK) rc = batch_knn(b, k, d_scratch_nn, st);  // warm-up
+            float ms = 0.0f;
+            if (rc == SX_OK && hipEventRecord(e0, st) != hipSuccess) rc = fail(SX_EHIP, "sx_batch_autotune: event");
+            for (int r = 0; r < reps && rc == SX_OK; ++r) rc = batch_knn(b, k, d_scratch_nn, st);
+            if (rc == SX_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = fail(SX_EHIP, "sx_batch_autotune: timing failed");
+            ms /= static_cast<float>(reps);
+            if (h_ms) h_ms[i * n_queue + j] = ms;
+            if (rc == SX_OK && ms < best) {
+                best = ms;
+                bb = h_buckets[i];
+                bq = h_queue_blocks[j];
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != SX_OK) {
+        b->bucket = keep_bucket;
+        b->queue_blocks = keep_queue;
+        return rc;
+    }
+    b->bucket = bb;
+    b->queue_blocks = bq;
+    if (best_bucket) *best_bucket = bb;
+    if (best_queue_blocks) *best_queue_blocks = bq;
+    return SX_OK;
+}
+
+}  // extern "C"

@@ -34,10 +34,10 @@ int unique_launch(const uint64_t* sorted, const uint32_t* vals, int64_t n, uint6
                   int32_t* uos, int32_t* uoi, int32_t* d_count, void* ws, hipStream_t st);
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
-                  int32_t* node_end, int32_t* leaf_parent, TravNode* trav, hipStream_t st);
+                  int32_t* node_end, int32_t* leaf_parent, TravNode* trav, hipStream_t st, CloudMap cm = CloudMap());
 size_t box_ws_bytes(int64_t n_max);
 int child_boxes_launch(const float4* pts, const int32_t* d_count, int64_t n_max, TravNode* trav, void* ws,
-                       hipStream_t st);
+                       hipStream_t st, CloudMap cm = CloudMap());
 size_t octree_ws_bytes(int64_t n_max);
 int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, const float* bounds,
                   const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
@@ -45,13 +45,14 @@ int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
                   void* ws, hipStream_t st);
 int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
-               hipStream_t st);
+               hipStream_t st, CloudMap cm = CloudMap());
 int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                      int k, const float* radius2, int bucket, int32_t* out, int32_t* queue, int32_t* queue_count,
-                     int32_t* deferred_total, hipStream_t st);
+                     int32_t* deferred_total, hipStream_t st, CloudMap cm = CloudMap(), int queue_blocks = 0);
 int scatter_i32_launch(const int32_t* src, const uint32_t* idx, int64_t n, int32_t* dst, hipStream_t st);
 int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn, int64_t q_start, int64_t n_q, int k,
-                   int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st);
+                   int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st,
+                   CloudMap cm = CloudMap());
 }  // namespace sx
 
 // counts[] slots: unique points, octree nodes, non-finite inputs, queries the

@@ -68,6 +68,22 @@ __device__ __forceinline__ void scan_range(const float4* __restrict__ pts, int l
     }
 }
 
+// Row of K neighbour indices relative to the cloud's first point `base`;
+// missing entries (fewer than K points) repeat the nearest.
+template <int K>
+__device__ __forceinline__ void write_row(int32_t* row, const TopK<K>& top, int base) {
+    const int32_t first = static_cast<int32_t>(top.e[0] & 0xFFFFFFFFull) - base;
+#pragma unroll
+    for (int s = 0; s < K; s += 4) {
+        int4 v;
+        v.x = top.e[s] == ~0ull ? first : static_cast<int32_t>(top.e[s] & 0xFFFFFFFFull) - base;
+        v.y = top.e[s + 1] == ~0ull ? first : static_cast<int32_t>(top.e[s + 1] & 0xFFFFFFFFull) - base;
+        v.z = top.e[s + 2] == ~0ull ? first : static_cast<int32_t>(top.e[s + 2] & 0xFFFFFFFFull) - base;
+        v.w = top.e[s + 3] == ~0ull ? first : static_cast<int32_t>(top.e[s + 3] & 0xFFFFFFFFull) - base;
+        *reinterpret_cast<int4*>(row + s) = v;
+    }
+}
+
 // One query's exact search.  State machine with one record load per step:
 //   down at `node`: pick the nearer child; a bucket child is scanned (next
 //     step) and counts as returned-from; an internal child is entered.
@@ -76,13 +92,14 @@ __device__ __forceinline__ void scan_range(const float4* __restrict__ pts, int l
 //     otherwise p is finished and the walk returns to p's parent.
 // `from` is compared with the split gamma of p: the left child's index is
 // gamma and the right child's gamma+1, for leaves and internal nodes alike.
+// The cloud spans points (and radix nodes) [base, base + U); its root is node base.
 template <int K>
-__device__ void knn_search(const float4* __restrict__ pts, const TravNode* __restrict__ nodes, int U, float qx,
-                           float qy, float qz, uint32_t r2bits, int bucket, TopK<K>& top) {
+__device__ void knn_search(const float4* __restrict__ pts, const TravNode* __restrict__ nodes, int base, int U,
+                           float qx, float qy, float qz, uint32_t r2bits, int bucket, TopK<K>& top) {
     bool scan = U <= bucket;
     bool done = scan;
-    int slo = 0, shi = U - 1;
-    int node = 0, p = -1, from = -1;
+    int slo = base, shi = base + U - 1;
+    int node = base, p = -1, from = -1;
     bool down = true;
     while (true) {
         if (scan) {
@@ -149,11 +166,13 @@ __global__ void __launch_bounds__(kKnnBlock) knn_kernel(const float4* __restrict
                                                         const int32_t* __restrict__ d_counts, int64_t q_start,
                                                         int64_t n_q, const float* __restrict__ queries,
                                                         const float* __restrict__ radius2, int bucket,
-                                                        int32_t* __restrict__ out, int32_t* __restrict__ reruns) {
+                                                        int32_t* __restrict__ out, int32_t* __restrict__ reruns,
+                                                        CloudMap cm) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n_q) return;
-    const int U = d_counts[0];
-    if (U <= 0) return;
+    const int total = d_counts[0];
+    if (total <= 0) return;
+    int base = 0, U = total;
     float qx, qy, qz;
     if (kExternal) {
         qx = queries[3 * t];
@@ -161,7 +180,11 @@ __global__ void __launch_bounds__(kKnnBlock) knn_kernel(const float4* __restrict
         qz = queries[3 * t + 2];
     } else {
         const int64_t qi = q_start + t;
-        if (qi >= U) return;
+        if (qi >= total) return;
+        const int2 cloud = cm.at(qi, total);
+        base = cloud.x;
+        U = cloud.y;
+        if (qi - base >= U) return;  // padding slot of a batched cloud
         const float4 q = pts[qi];
         qx = q.x;
         qy = q.y;
@@ -177,35 +200,25 @@ __global__ void __launch_bounds__(kKnnBlock) knn_kernel(const float4* __restrict
     // A seeded search that finds fewer than K points within R is not proven
     // exact: it is repeated unseeded (one loop so the search is inlined once).
     for (int attempt = 0; attempt < 2; ++attempt) {
-        knn_search<K>(pts, nodes, U, qx, qy, qz, r2bits, bucket, top);
+        knn_search<K>(pts, nodes, base, U, qx, qy, qz, r2bits, bucket, top);
         if (r2bits == 0xFFFFFFFFu || top.kth_bits() <= r2bits) break;
         if (reruns) atomicAdd(reruns, 1);
         top.init();
         r2bits = 0xFFFFFFFFu;
     }
-    int32_t* row = out + t * K;
-    const int32_t first = static_cast<int32_t>(top.e[0] & 0xFFFFFFFFull);
-#pragma unroll
-    for (int s = 0; s < K; s += 4) {
-        int4 v;
-        v.x = top.e[s] == ~0ull ? first : static_cast<int32_t>(top.e[s] & 0xFFFFFFFFull);
-        v.y = top.e[s + 1] == ~0ull ? first : static_cast<int32_t>(top.e[s + 1] & 0xFFFFFFFFull);
-        v.z = top.e[s + 2] == ~0ull ? first : static_cast<int32_t>(top.e[s + 2] & 0xFFFFFFFFull);
-        v.w = top.e[s + 3] == ~0ull ? first : static_cast<int32_t>(top.e[s + 3] & 0xFFFFFFFFull);
-        *reinterpret_cast<int4*>(row + s) = v;
-    }
+    write_row<K>(out + t * K, top, base);
 }
 
 template <bool kExternal>
 int knn_dispatch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                  const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
-                 hipStream_t st) {
+                 hipStream_t st, CloudMap cm) {
     const unsigned g = grid_for(n_q, kKnnBlock);
     switch (k) {
-        case 8: knn_kernel<8, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns); break;
-        case 16: knn_kernel<16, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns); break;
-        case 24: knn_kernel<24, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns); break;
-        case 32: knn_kernel<32, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns); break;
+        case 8: knn_kernel<8, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns, cm); break;
+        case 16: knn_kernel<16, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns, cm); break;
+        case 24: knn_kernel<24, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns, cm); break;
+        case 32: knn_kernel<32, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns, cm); break;
         default: return fail(SX_EINVAL, "knn: k must be 8, 16, 24 or 32");
     }
     SX_LAUNCH_CHECK("knn_kernel");
@@ -216,12 +229,13 @@ int knn_dispatch(const float4* pts, const TravNode* nodes, const int32_t* d_coun
 // queries; otherwise n_q external points (12-byte AoS).
 int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
-               hipStream_t st) {
+               hipStream_t st, CloudMap cm = CloudMap()) {
     if (n_q <= 0) return SX_OK;
     if (reinterpret_cast<uintptr_t>(out) % 16 != 0) return fail(SX_EINVAL, "knn: output must be 16-byte aligned");
     if (queries)
-        return knn_dispatch<true>(pts, nodes, d_counts, q_start, n_q, queries, k, radius2, bucket, out, reruns, st);
-    return knn_dispatch<false>(pts, nodes, d_counts, q_start, n_q, nullptr, k, radius2, bucket, out, reruns, st);
+        return knn_dispatch<true>(pts, nodes, d_counts, q_start, n_q, queries, k, radius2, bucket, out, reruns, st,
+                                  CloudMap());
+    return knn_dispatch<false>(pts, nodes, d_counts, q_start, n_q, nullptr, k, radius2, bucket, out, reruns, st, cm);
 }
 
 
@@ -340,20 +354,6 @@ __device__ __forceinline__ uint64_t make_key(float d, int j) {
     return (static_cast<uint64_t>(__float_as_uint(d)) << 32) | static_cast<uint32_t>(j);
 }
 
-template <int K>
-__device__ __forceinline__ void write_row(int32_t* row, const TopK<K>& top) {
-    const int32_t first = static_cast<int32_t>(top.e[0] & 0xFFFFFFFFull);
-#pragma unroll
-    for (int s = 0; s < K; s += 4) {
-        int4 v;
-        v.x = top.e[s] == ~0ull ? first : static_cast<int32_t>(top.e[s] & 0xFFFFFFFFull);
-        v.y = top.e[s + 1] == ~0ull ? first : static_cast<int32_t>(top.e[s + 1] & 0xFFFFFFFFull);
-        v.z = top.e[s + 2] == ~0ull ? first : static_cast<int32_t>(top.e[s + 2] & 0xFFFFFFFFull);
-        v.w = top.e[s + 3] == ~0ull ? first : static_cast<int32_t>(top.e[s + 3] & 0xFFFFFFFFull);
-        *reinterpret_cast<int4*>(row + s) = v;
-    }
-}
-
 // Flattened position f (< total) of a list of inclusive index ranges with
 // inclusive size prefix `pre` -> point index.
 __device__ __forceinline__ int range_point(const int2* rng, const int* pre, int nr, int f) {
@@ -448,7 +448,8 @@ knn_tile_kernel(const float4* __restrict__ pts,
                                                              const int32_t* __restrict__ d_counts, int64_t q_start,
                                                              int64_t n_q, const float* __restrict__ radius2,
                                                              int bucket, int32_t* __restrict__ out,
-                                                             int32_t* __restrict__ queue, int32_t* queue_count) {
+                                                             int32_t* __restrict__ queue, int32_t* queue_count,
+                                                             CloudMap cm) {
     __shared__ int s_stack[kTileWaves][kTileStack];
     __shared__ int2 s_range[kTileWaves][kTileRanges];
     __shared__ int s_pre[kTileWaves][kTileRanges];
@@ -456,13 +457,16 @@ knn_tile_kernel(const float4* __restrict__ pts,
     __shared__ float4 s_seed[kTileWaves][kLearned ? kWave + 2 * kSeed : 1];
     const int w = static_cast<int>(threadIdx.x) / kWave;
     const int lane = static_cast<int>(lane_id());
-    const int U = d_counts[0];
+    const int total = d_counts[0];
     int64_t q_end = q_start + n_q;
-    if (q_end > U) q_end = U;
+    if (q_end > total) q_end = total;
     const int64_t tile0 = q_start + (static_cast<int64_t>(xcd_block()) * kTileWaves + w) * kWave;
     if (tile0 >= q_end) return;  // whole wave idle (no block barriers below)
+    // the tile's cloud: points and radix nodes [cbase, cbase + U)
+    const int2 cloud = cm.at(tile0, total);
+    const int cbase = cloud.x, U = cloud.y, cend = cloud.x + cloud.y;
     const int64_t qi = tile0 + lane;
-    const bool valid = qi < q_end;
+    const bool valid = qi < q_end && qi < cend;
     const float4 q = valid ? pts[qi] : make_float4(0.f, 0.f, 0.f, 0.f);
     int* stk = s_stack[w];
     int2* rng = s_range[w];
@@ -474,8 +478,8 @@ knn_tile_kernel(const float4* __restrict__ pts,
     int wlo = 0, whi = -1, seed_lo = 0, seed_hi = -1;
     uint32_t r2bits;
     if (!kLearned) {
-        wlo = static_cast<int>(tile0 - kWindow > 0 ? tile0 - kWindow : 0);
-        whi = static_cast<int>(tile0 + kWave - 1 + kWindow < U - 1 ? tile0 + kWave - 1 + kWindow : U - 1);
+        wlo = static_cast<int>(tile0 - kWindow > cbase ? tile0 - kWindow : cbase);
+        whi = static_cast<int>(tile0 + kWave - 1 + kWindow < cend - 1 ? tile0 + kWave - 1 + kWindow : cend - 1);
         for (int base = wlo; base <= whi; base += kWave) {
             const int j = base + lane;
             const float4 p = j <= whi ? pts[j] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -499,13 +503,13 @@ knn_tile_kernel(const float4* __restrict__ pts,
         for (int h = 0; h < 2; ++h) {
             const int idx = lane + h * kWave;
             const int64_t jj = w0 + idx;
-            if (idx < kWave + 2 * kSeed && jj >= 0 && jj < U) win[idx] = pts[jj];
+            if (idx < kWave + 2 * kSeed && jj >= cbase && jj < cend) win[idx] = pts[jj];
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 #pragma unroll 1
         for (int o = -kSeed; o <= kSeed; ++o) {
             const int jj = static_cast<int>(qi) + o;
-            if (valid && jj >= 0 && jj < U) {
+            if (valid && jj >= cbase && jj < cend) {
                 const float4 p = win[lane + kSeed + o];
                 const uint64_t key = make_key(dist2(q.x, q.y, q.z, p.x, p.y, p.z), jj);
                 if (key < top.e[K - 1]) top.insert(key);
@@ -575,10 +579,10 @@ knn_tile_kernel(const float4* __restrict__ pts,
         int sp = 0, nr = 0, seen = 0;
         bool overflow = false;
         if (U <= bucket) {
-            if (lane == 0) rng[0] = make_int2(0, U - 1);
+            if (lane == 0) rng[0] = make_int2(cbase, cend - 1);
             nr = 1;
         } else {
-            if (lane == 0) stk[0] = 0;
+            if (lane == 0) stk[0] = cbase;  // the cloud's root
             sp = 1;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -635,7 +639,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
     // plain: every point within the window radius was a candidate; learned:
     // the k-th distance must lie within the radius the segment box proves
     const bool exact = join && !lost && (kLearned ? top.kth_bits() <= lim : true);
-    if (valid && exact) write_row<K>(out + (qi - q_start) * K, top);
+    if (valid && exact) write_row<K>(out + (qi - q_start) * K, top, cbase);
     const bool defer = valid && !exact;
     const uint64_t md = __ballot(defer);
     if (md) {
@@ -664,28 +668,32 @@ __global__ void __launch_bounds__(kKnnBlock) knn_wave_kernel(const float4* __res
                                                              const int32_t* __restrict__ d_counts, int64_t q_start,
                                                              const int32_t* __restrict__ queue,
                                                              const int32_t* __restrict__ queue_count, int bucket,
-                                                             int32_t* __restrict__ out, int32_t* deferred_total) {
+                                                             int32_t* __restrict__ out, int32_t* deferred_total,
+                                                             CloudMap cm) {
     __shared__ int2 s_stack[kTileWaves][kStackCap];  // (node, box distance bits)
     __shared__ int2 s_rng[kTileWaves][kWaveRanges];
     __shared__ int s_pre[kTileWaves][kWaveRanges];
     const int w = static_cast<int>(threadIdx.x) / kWave;
     const int lane = static_cast<int>(lane_id());
-    const int U = d_counts[0];
+    const int total = d_counts[0];
     const int n = *queue_count;
     if (blockIdx.x == 0 && threadIdx.x == 0 && deferred_total) atomicAdd(deferred_total, n);
     const int n_waves = static_cast<int>(gridDim.x) * kTileWaves;
     for (int t = static_cast<int>(blockIdx.x) * kTileWaves + w; t < n; t += n_waves) {
         const int qi = uniform(queue[t]);
         const float4 q = pts[qi];
+        // the query's cloud: points and radix nodes [cbase, cend)
+        const int2 cloud = cm.at(qi, total);
+        const int cbase = cloud.x, U = cloud.y, cend = cloud.x + cloud.y;
         // seed: the 64 Morton neighbours of the query
         int wbase = qi - kWave / 2;
-        if (wbase > U - kWave) wbase = U - kWave;
-        if (wbase < 0) wbase = 0;
+        if (wbase > cend - kWave) wbase = cend - kWave;
+        if (wbase < cbase) wbase = cbase;
         const int wend = wbase + kWave - 1;
         uint64_t key = ~0ull;
         {
             const int j = wbase + lane;
-            if (j < U) {
+            if (j < cend) {
                 const float4 p = pts[j];
                 key = make_key(dist2(q.x, q.y, q.z, p.x, p.y, p.z), j);
             }
@@ -698,10 +706,10 @@ __global__ void __launch_bounds__(kKnnBlock) knn_wave_kernel(const float4* __res
         if (U <= kWave) {
             sp = 0;  // the seed window holds every point
         } else if (U <= bucket) {
-            if (lane == 0) s_rng[w][0] = make_int2(0, U - 1);
+            if (lane == 0) s_rng[w][0] = make_int2(cbase, cend - 1);
             nr = 1;
         } else {
-            if (lane == 0) s_stack[w][0] = make_int2(0, 0);
+            if (lane == 0) s_stack[w][0] = make_int2(cbase, 0);  // the cloud's root
             sp = 1;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -781,33 +789,34 @@ __global__ void __launch_bounds__(kKnnBlock) knn_wave_kernel(const float4* __res
             TopK<K> top;
             top.init();
             if (lane == 0) {
-                knn_search<K>(pts, nodes, U, q.x, q.y, q.z, 0xFFFFFFFFu, bucket, top);
-                write_row<K>(row, top);
+                knn_search<K>(pts, nodes, cbase, U, q.x, q.y, q.z, 0xFFFFFFFFu, bucket, top);
+                write_row<K>(row, top, cbase);
             }
             continue;
         }
-        const int first = __builtin_amdgcn_readfirstlane(static_cast<int>(key));
-        if (lane < K) row[lane] = key == ~0ull ? first : static_cast<int32_t>(key & 0xFFFFFFFFull);
+        const int first = __builtin_amdgcn_readfirstlane(static_cast<int>(key)) - cbase;
+        if (lane < K) row[lane] = key == ~0ull ? first : static_cast<int32_t>(key & 0xFFFFFFFFull) - cbase;
     }
 }
 
 template <int K>
 int knn_tiles_k(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                 const float* radius2, int bucket, int32_t* out, int32_t* queue, int32_t* queue_count,
-                int32_t* deferred_total, hipStream_t st) {
+                int32_t* deferred_total, hipStream_t st, CloudMap cm, int queue_blocks) {
     const int64_t tiles = (n_q + kWave - 1) / kWave;
     const unsigned g = static_cast<unsigned>((tiles + kTileWaves - 1) / kTileWaves);
     if (radius2)
         knn_tile_kernel<K, true><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out,
-                                                         queue, queue_count);
+                                                         queue, queue_count, cm);
     else
         knn_tile_kernel<K, false><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, nullptr, bucket, out,
-                                                          queue, queue_count);
+                                                          queue, queue_count, cm);
     SX_LAUNCH_CHECK("knn_tile_kernel");
     int64_t gq = (n_q + kTileWaves - 1) / kTileWaves;  // one wave per query at most
     if (gq > 8192) gq = 8192;
+    if (queue_blocks > 0 && queue_blocks < gq) gq = queue_blocks;
     knn_wave_kernel<K><<<static_cast<unsigned>(gq), kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, queue,
-                                                                      queue_count, bucket, out, deferred_total);
+                                                                      queue_count, bucket, out, deferred_total, cm);
     SX_LAUNCH_CHECK("knn_wave_kernel");
     return SX_OK;
 }
@@ -816,15 +825,15 @@ int knn_tiles_k(const float4* pts, const TravNode* nodes, const int32_t* d_count
 // cleared here; deferred_total accumulates the number of queued queries.
 int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                      int k, const float* radius2, int bucket, int32_t* out, int32_t* queue, int32_t* queue_count,
-                     int32_t* deferred_total, hipStream_t st) {
+                     int32_t* deferred_total, hipStream_t st, CloudMap cm = CloudMap(), int queue_blocks = 0) {
     if (n_q <= 0) return SX_OK;
     if (reinterpret_cast<uintptr_t>(out) % 16 != 0) return fail(SX_EINVAL, "knn: output must be 16-byte aligned");
     SX_TRY(fill_i32(queue_count, 0, 1, st));
     switch (k) {
-        case 8: return knn_tiles_k<8>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st);
-        case 16: return knn_tiles_k<16>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st);
-        case 24: return knn_tiles_k<24>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st);
-        case 32: return knn_tiles_k<32>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st);
+        case 8: return knn_tiles_k<8>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks);
+        case 16: return knn_tiles_k<16>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks);
+        case 24: return knn_tiles_k<24>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks);
+        case 32: return knn_tiles_k<32>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks);
         default: return fail(SX_EINVAL, "knn: k must be 8, 16, 24 or 32");
     }
 }

@@ -209,6 +209,94 @@ int decode_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
     return SX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Batched clouds: cloud c holds input points [off[c], off[c+1]).
+// ---------------------------------------------------------------------------
+// One block per cloud: bounds[4c] = (min, max, range) over its three axes,
+// the same reduction as bbox_kernel; non-finite coordinates add to *d_bad.
+__global__ void __launch_bounds__(kBBoxBlock) seg_bbox_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                              const float* __restrict__ z,
+                                                              const int64_t* __restrict__ off, float* bounds,
+                                                              int32_t* d_bad) {
+    const int c = blockIdx.x;
+    float mn = INFINITY, mx = -INFINITY;
+    int bad = 0;
+    for (int64_t i = off[c] + threadIdx.x; i < off[c + 1]; i += blockDim.x) {
+        bbox_acc(x[i], mn, mx, bad);
+        bbox_acc(y[i], mn, mx, bad);
+        bbox_acc(z[i], mn, mx, bad);
+    }
+    block_minmax(mn, mx, bad);
+    if (threadIdx.x == 0) {
+        bounds[4 * c] = mn;
+        bounds[4 * c + 1] = mx;
+        bounds[4 * c + 2] = __fsub_rn(mx, mn);
+        bounds[4 * c + 3] = 0.0f;
+        if (bad) atomicAdd(d_bad, bad);
+    }
+}
+
+int seg_bbox_launch(const float* x, const float* y, const float* z, const int64_t* off, int n_clouds, float* bounds,
+                    int32_t* d_bad, hipStream_t st) {
+    if (n_clouds <= 0) return SX_OK;
+    seg_bbox_kernel<<<n_clouds, kBBoxBlock, 0, st>>>(x, y, z, off, bounds, d_bad);
+    SX_LAUNCH_CHECK("seg_bbox_kernel");
+    return SX_OK;
+}
+
+__global__ void __launch_bounds__(kEncodeBlock) seg_encode_kernel(const float* __restrict__ x,
+                                                                  const float* __restrict__ y,
+                                                                  const float* __restrict__ z, int64_t n,
+                                                                  const int64_t* __restrict__ off, int n_clouds,
+                                                                  const float* __restrict__ bounds, int code_bits,
+                                                                  uint64_t* __restrict__ codes) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cloud_of(off, n_clouds, i);
+    const float fscale = static_cast<float>(axis_scale(code_bits));
+    codes[i] = encode_one(x[i], y[i], z[i], bounds[4 * c], bounds[4 * c + 2], fscale);
+}
+
+int seg_encode_launch(const float* x, const float* y, const float* z, int64_t n, const int64_t* off, int n_clouds,
+                      const float* bounds, int code_bits, uint64_t* codes, hipStream_t st) {
+    if (n <= 0) return SX_OK;
+    seg_encode_kernel<<<grid_for(n, kEncodeBlock), kEncodeBlock, 0, st>>>(x, y, z, n, off, n_clouds, bounds,
+                                                                            code_bits, codes);
+    SX_LAUNCH_CHECK("seg_encode_kernel");
+    return SX_OK;
+}
+
+// Points of every slot [0, *d_slots): a cloud's unique codes with its own
+// cube; the padding slots after a cloud's last point are zero.
+__global__ void __launch_bounds__(256) seg_decode_kernel(const uint64_t* __restrict__ codes, const int32_t* d_slots,
+                                                         int64_t n_max, CloudMap cm, const int32_t* __restrict__ cloud_of_tile,
+                                                         const float* __restrict__ bounds, int code_bits,
+                                                         float4* __restrict__ pts16) {
+    const int64_t n = scan_count(d_slots, n_max);
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int2 cloud = cm.at(i, static_cast<int>(n));
+    if (i - cloud.x >= cloud.y) {
+        pts16[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const int c = cloud_of_tile[i / kCloudAlign];
+    const float mn = bounds[4 * c], range = bounds[4 * c + 2];
+    const float fscale = static_cast<float>(axis_scale(code_bits));
+    const uint64_t v = codes[i];
+    pts16[i] = make_float4(dequantize(compact3(v), fscale, range, mn), dequantize(compact3(v >> 1), fscale, range, mn),
+                           dequantize(compact3(v >> 2), fscale, range, mn), 0.0f);
+}
+
+int seg_decode_launch(const uint64_t* codes, const int32_t* d_slots, int64_t n_max, CloudMap cm, const float* bounds,
+                      int code_bits, float4* pts16, hipStream_t st) {
+    if (n_max <= 0) return SX_OK;
+    seg_decode_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(codes, d_slots, n_max, cm, cm.tile_cloud, bounds,
+                                                            code_bits, pts16);
+    SX_LAUNCH_CHECK("seg_decode_kernel");
+    return SX_OK;
+}
+
 inline bool valid_code_bits(int b) { return b == 63 || b == 30; }
 
 }  // namespace sx

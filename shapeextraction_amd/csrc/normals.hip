@@ -141,19 +141,23 @@ __global__ void __launch_bounds__(kNormalsBlock) normals_kernel(const float4* __
                                                                 const int32_t* __restrict__ nn, int64_t q_start,
                                                                 int64_t n_q, int precision, float vx, float vy,
                                                                 float vz, float* __restrict__ normals,
-                                                                uint8_t* __restrict__ flags) {
+                                                                uint8_t* __restrict__ flags, CloudMap cm) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n_q) return;
-    const int U = d_counts[0];
+    const int total = d_counts[0];
     const int64_t qi = q_start + t;
-    if (qi >= U) return;
+    if (qi >= total) return;
+    // rows hold indices relative to the query's cloud
+    const int2 cloud = cm.at(qi, total);
+    if (qi - cloud.x >= cloud.y) return;  // padding slot of a batched cloud
     const float4 q = pts[qi];
+    const float4* cpts = pts + cloud.x;
     const int32_t* row = nn + t * K;
     float dx[K], dy[K], dz[K];
 #pragma unroll
     for (int s = 0; s < K; s += 4) {
         const int4 id = *reinterpret_cast<const int4*>(row + s);
-        const float4 p0 = pts[id.x], p1 = pts[id.y], p2 = pts[id.z], p3 = pts[id.w];
+        const float4 p0 = cpts[id.x], p1 = cpts[id.y], p2 = cpts[id.z], p3 = cpts[id.w];
         dx[s] = p0.x - q.x; dy[s] = p0.y - q.y; dz[s] = p0.z - q.z;
         dx[s + 1] = p1.x - q.x; dy[s + 1] = p1.y - q.y; dz[s + 1] = p1.z - q.z;
         dx[s + 2] = p2.x - q.x; dy[s + 2] = p2.y - q.y; dz[s + 2] = p2.z - q.z;
@@ -203,16 +207,17 @@ __global__ void __launch_bounds__(kNormalsBlock) normals_kernel(const float4* __
 }
 
 int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn, int64_t q_start, int64_t n_q, int k,
-                   int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st) {
+                   int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st,
+                   CloudMap cm = CloudMap()) {
     if (n_q <= 0) return SX_OK;
     if (reinterpret_cast<uintptr_t>(nn) % 16 != 0) return fail(SX_EINVAL, "normals: neighbour rows must be 16-byte aligned");
     if (precision != SX_PREC_FP32 && precision != SX_PREC_FP64) return fail(SX_EINVAL, "normals: bad precision mode");
     const unsigned g = grid_for(n_q, kNormalsBlock);
     switch (k) {
-        case 8: normals_kernel<8><<<g, kNormalsBlock, 0, st>>>(pts, d_counts, nn, q_start, n_q, precision, vx, vy, vz, normals, flags); break;
-        case 16: normals_kernel<16><<<g, kNormalsBlock, 0, st>>>(pts, d_counts, nn, q_start, n_q, precision, vx, vy, vz, normals, flags); break;
-        case 24: normals_kernel<24><<<g, kNormalsBlock, 0, st>>>(pts, d_counts, nn, q_start, n_q, precision, vx, vy, vz, normals, flags); break;
-        case 32: normals_kernel<32><<<g, kNormalsBlock, 0, st>>>(pts, d_counts, nn, q_start, n_q, precision, vx, vy, vz, normals, flags); break;
+        case 8: normals_kernel<8><<<g, kNormalsBlock, 0, st>>>(pts, d_counts, nn, q_start, n_q, precision, vx, vy, vz, normals, flags, cm); break;
+        case 16: normals_kernel<16><<<g, kNormalsBlock, 0, st>>>(pts, d_counts, nn, q_start, n_q, precision, vx, vy, vz, normals, flags, cm); break;
+        case 24: normals_kernel<24><<<g, kNormalsBlock, 0, st>>>(pts, d_counts, nn, q_start, n_q, precision, vx, vy, vz, normals, flags, cm); break;
+        case 32: normals_kernel<32><<<g, kNormalsBlock, 0, st>>>(pts, d_counts, nn, q_start, n_q, precision, vx, vy, vz, normals, flags, cm); break;
         default: return fail(SX_EINVAL, "normals: k must be 8, 16, 24 or 32");
     }
     SX_LAUNCH_CHECK("normals_kernel");

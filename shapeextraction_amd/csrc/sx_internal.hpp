@@ -41,6 +41,38 @@ inline unsigned grid_for(int64_t n, int block) {
 
 inline size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
 
+// ---------------------------------------------------------------------------
+// Batched clouds.  Cloud c's unique points occupy slots [base[c], base[c] +
+// size[c]) of the index arrays, base[c] a multiple of kCloudAlign, so every
+// 64-slot tile of queries lies in one cloud and a cloud's internal radix
+// nodes carry the same numbers as a standalone build shifted by base[c].  The
+// default (all null) is one cloud of U points at slot 0.
+// ---------------------------------------------------------------------------
+constexpr int kCloudAlign = 64;
+struct CloudMap {
+    const int32_t* tile_cloud = nullptr;  // cloud of each kCloudAlign-slot tile
+    const int32_t* base = nullptr;
+    const int32_t* size = nullptr;
+    // (first slot, unique points) of the cloud holding `slot`; U is the point
+    // count of the single-cloud case
+    __device__ __forceinline__ int2 at(int64_t slot, int U) const {
+        if (!tile_cloud) return make_int2(0, U);
+        const int c = tile_cloud[slot / kCloudAlign];
+        return make_int2(base[c], size[c]);
+    }
+};
+
+// Cloud of input point i when cloud c holds points [off[c], off[c+1]): the
+// last cloud whose first point is <= i (empty clouds are skipped).
+__device__ __forceinline__ int cloud_of(const int64_t* __restrict__ off, int n_clouds, int64_t i) {
+    int lo = 0, hi = n_clouds - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Workgroups are dispatched round-robin over the XCDs (8 on MI355X), each
 // with its own L2.  xcd_block() renumbers them so that every XCD receives a
 // contiguous run of block indices: neighbouring tiles (Morton neighbours,

@@ -54,12 +54,19 @@ __global__ void __launch_bounds__(256) karras_kernel(const uint64_t* __restrict_
                                                      int64_t n_max, int code_bits, int root_mode,
                                                      uint8_t* prefix_n, int32_t* left_child, uint8_t* has_leaf_left,
                                                      uint8_t* has_leaf_right, int32_t* parent, int32_t* node_end,
-                                                     int32_t* leaf_parent, TravNode* trav) {
-    const int64_t U = scan_count(d_count, n_max);
+                                                     int32_t* leaf_parent, TravNode* trav, CloudMap cm) {
+    // g: slot; the cloud holding it spans slots [base, base + U) and its
+    // internal node i (local) is stored at slot base + i
+    const int64_t total = scan_count(d_count, n_max);
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const int2 cloud = cm.at(g, static_cast<int>(total));
+    const int64_t base = cloud.x, U = cloud.y;
     const int64_t N = U - 1;  // internal nodes; leaves are 0..N
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i == 0 && parent) parent[0] = -1;
+    const int64_t i = g - base;
+    if (i == 0 && parent) parent[g] = -1;
     if (i >= N) return;
+    codes += base;
     const uint64_t ci = codes[i];
     int d;
     if (i == 0) {
@@ -99,39 +106,42 @@ __global__ void __launch_bounds__(256) karras_kernel(const uint64_t* __restrict_
     const int64_t gamma = i + s * d + (d < 0 ? d : 0);
     const int64_t a = i < j ? i : j, b = i < j ? j : i;
     const bool hl = (a == gamma), hr = (b == gamma + 1);
-    if (prefix_n) prefix_n[i] = static_cast<uint8_t>(dnode);
-    if (left_child) left_child[i] = static_cast<int32_t>(gamma);
-    if (has_leaf_left) has_leaf_left[i] = hl ? 1 : 0;
-    if (has_leaf_right) has_leaf_right[i] = hr ? 1 : 0;
+    // global (slot) numbers from here on
+    const int64_t G = base + gamma;
+    if (prefix_n) prefix_n[g] = static_cast<uint8_t>(dnode);
+    if (left_child) left_child[g] = static_cast<int32_t>(G);
+    if (has_leaf_left) has_leaf_left[g] = hl ? 1 : 0;
+    if (has_leaf_right) has_leaf_right[g] = hr ? 1 : 0;
     if (parent) {
-        if (!hl) parent[gamma] = static_cast<int32_t>(i);
-        if (!hr) parent[gamma + 1] = static_cast<int32_t>(i);
+        if (!hl) parent[G] = static_cast<int32_t>(g);
+        if (!hr) parent[G + 1] = static_cast<int32_t>(g);
     }
-    if (node_end) node_end[i] = static_cast<int32_t>(j);
+    if (node_end) node_end[g] = static_cast<int32_t>(base + j);
     if (leaf_parent) {
-        if (hl) leaf_parent[gamma] = static_cast<int32_t>(2 * i);
-        if (hr) leaf_parent[gamma + 1] = static_cast<int32_t>(2 * i + 1);
+        if (hl) leaf_parent[G] = static_cast<int32_t>(2 * g);
+        if (hr) leaf_parent[G + 1] = static_cast<int32_t>(2 * g + 1);
     }
     if (trav) {
-        TravNode* t = trav + i;
-        t->lo_l.w = __int_as_float(static_cast<int>(a));
-        t->hi_l.w = __int_as_float(static_cast<int>(gamma));
-        t->lo_r.w = __int_as_float(static_cast<int>(b));
+        TravNode* t = trav + g;
+        t->lo_l.w = __int_as_float(static_cast<int>(base + a));
+        t->hi_l.w = __int_as_float(static_cast<int>(G));
+        t->lo_r.w = __int_as_float(static_cast<int>(base + b));
         if (i == 0) t->hi_r.w = __int_as_float(-1);
-        if (!hl) trav[gamma].hi_r.w = __int_as_float(static_cast<int>(i));
-        if (!hr) trav[gamma + 1].hi_r.w = __int_as_float(static_cast<int>(i));
+        if (!hl) trav[G].hi_r.w = __int_as_float(static_cast<int>(g));
+        if (!hr) trav[G + 1].hi_r.w = __int_as_float(static_cast<int>(g));
     }
 }
 
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
-                  int32_t* node_end, int32_t* leaf_parent, TravNode* trav, hipStream_t st) {
+                  int32_t* node_end, int32_t* leaf_parent, TravNode* trav, hipStream_t st, CloudMap cm = CloudMap()) {
     if (n_max <= 0) return SX_OK;
     // Nodes that no other node claims (only possible under the reference root
     // rule) keep parent -1.
     if (parent) SX_TRY(fill_i32(parent, -1, n_max, st));
     karras_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(codes, d_count, n_max, code_bits, root_mode, prefix_n,
-                                                        left_child, hll, hlr, parent, node_end, leaf_parent, trav);
+                                                        left_child, hll, hlr, parent, node_end, leaf_parent, trav,
+                                                        cm);
     SX_LAUNCH_CHECK("karras_kernel");
     return SX_OK;
 }
@@ -226,10 +236,13 @@ __device__ Box3 range_box(const float4* __restrict__ pts, const BoxLevels& lv, i
 }
 
 __global__ void __launch_bounds__(256) child_box_kernel(const float4* __restrict__ pts, BoxLevels lv,
-                                                        const int32_t* d_count, int64_t n_max, TravNode* trav) {
-    const int64_t U = scan_count(d_count, n_max);
+                                                        const int32_t* d_count, int64_t n_max, TravNode* trav,
+                                                        CloudMap cm) {
+    const int64_t total = scan_count(d_count, n_max);
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= U - 1) return;
+    if (i >= total) return;
+    const int2 cloud = cm.at(i, static_cast<int>(total));
+    if (i - cloud.x >= cloud.y - 1) return;  // not an internal node of its cloud
     TravNode* t = trav + i;
     const int a = w_int(t->lo_l.w), g = w_int(t->hi_l.w), b = w_int(t->lo_r.w);
     const Box3 l = range_box(pts, lv, a, g);
@@ -258,7 +271,7 @@ size_t box_ws_bytes(int64_t n_max) {
 }
 
 int child_boxes_launch(const float4* pts, const int32_t* d_count, int64_t n_max, TravNode* trav, void* ws,
-                       hipStream_t st) {
+                       hipStream_t st, CloudMap cm = CloudMap()) {
     if (n_max < 2) return SX_OK;
     BoxLevels lv{};
     lv.n_levels = box_levels_for(n_max);
@@ -271,7 +284,7 @@ int child_boxes_launch(const float4* pts, const int32_t* d_count, int64_t n_max,
         box_level_kernel<<<grid_for(cnt, 256), 256, 0, st>>>(pts, lv, L, d_count, n_max);
         SX_LAUNCH_CHECK("box_level_kernel");
     }
-    child_box_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(pts, lv, d_count, n_max, trav);
+    child_box_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(pts, lv, d_count, n_max, trav, cm);
     SX_LAUNCH_CHECK("child_box_kernel");
     return SX_OK;
 }

@@ -1,0 +1,219 @@
+"""Batched clouds (sx_batch): every cloud of a batch must give exactly what a
+standalone build of that cloud gives, and small clouds exactly what the CPU
+oracle gives.
+
+Checked per cloud: unique Morton codes, representatives (cloud-local input
+index), decoded points, radix-tree prefix lengths / splits / parents (slot
+numbering = standalone numbering + the cloud's first slot), kNN rows (bit
+exact) and FP32 normals (bit exact: same kernels on the same rows).  Also:
+a captured hipGraph replays to the eager result, the autotuner keeps results
+unchanged, and the full 512-cloud configuration (about 20M points) holds the
+same per-cloud equality on a sample of clouds.
+"""
+import numpy as np
+import pytest
+
+from conftest import rng
+
+pytestmark = pytest.mark.gpu
+
+K = 16
+
+
+def _mixed_clouds(sx, seed):
+    """Clouds of awkward sizes: 1, 2, sub-bucket, tile edges, duplicates, scenes."""
+    g = rng(seed)
+    clouds = []
+    for n in (1, 2, 15, 16, 17, 63, 64, 65, 200, 1000):
+        clouds.append(g.normal(size=(n, 3)).astype(np.float32) * 3.0)
+    # heavy duplicates: points on a coarse lattice (many identical codes)
+    lat = g.integers(0, 6, size=(5000, 3)).astype(np.float32) * 0.5
+    clouds.append(lat)
+    clouds.append(np.zeros((7, 3), np.float32))  # one unique point
+    for i, n in enumerate((3000, 20000, 45000)):
+        x, y, z = sx.gen_scene(seed * 10 + i, n)
+        clouds.append(np.stack([x, y, z], 1))
+    x, y, z = sx.gen_sphere(seed + 5, 30000)
+    clouds.append(np.stack([x, y, z], 1))
+    order = g.permutation(len(clouds))
+    return [clouds[i] for i in order]
+
+
+def _batch(sx, clouds):
+    off = np.zeros(len(clouds) + 1, np.int64)
+    off[1:] = np.cumsum([len(c) for c in clouds])
+    allp = np.concatenate(clouds, 0)
+    b = sx.Batch(int(off[-1]), len(clouds))
+    b.set_clouds(off)
+    b.build(allp[:, 0].copy(), allp[:, 1].copy(), allp[:, 2].copy())
+    return b, off
+
+
+def _single(sx, c, k=K):
+    ix = sx.Index(len(c), 63).build(c[:, 0].copy(), c[:, 1].copy(), c[:, 2].copy())
+    f = ix.fetch()
+    nn = ix.knn(k)
+    f["nn"] = nn.numpy()
+    f["normals"] = ix.normals(nn, k).numpy()
+    return f
+
+
+def _compare_cloud(tab, bf, nn_b, nrm_b, c, s, k=K):
+    base, U = int(tab["base"][c]), int(tab["size"][c])
+    assert U == s["n_unique"], c
+    sl = slice(base, base + U)
+    assert np.array_equal(bf["codes"][sl], s["codes"]), c
+    assert np.array_equal(bf["rep"][sl], s["rep"]), c
+    assert np.array_equal(bf["points"][sl], s["points"]), c
+    assert np.allclose(tab["bounds"][c, :3], s["bounds"], rtol=0, atol=0), c
+    if U >= 2:
+        inner = slice(base, base + U - 1)
+        assert np.array_equal(bf["prefix_n"][inner], s["prefix_n"][: U - 1]), c
+        assert np.array_equal(bf["left_child"][inner] - base, s["left_child"][: U - 1]), c
+        par = bf["parent"][inner].astype(np.int64)
+        par = np.where(par >= 0, par - base, -1)
+        assert np.array_equal(par, s["parent"][: U - 1]), c
+    assert np.array_equal(nn_b[sl], s["nn"]), c
+    assert np.array_equal(nrm_b[sl].view(np.uint32), s["normals"].view(np.uint32)), c
+
+
+def test_batch_equals_standalone_clouds(gpu):
+    sx = gpu
+    clouds = _mixed_clouds(sx, 3)
+    b, off = _batch(sx, clouds)
+    tab = b.table()
+    assert tab["n_slots"] % 64 == 0 and np.all(tab["base"] % 64 == 0)
+    bf = b.fetch()
+    nn = b.knn(K)
+    nrm = b.normals(nn, K).numpy()
+    nn = nn.numpy()
+    print(f"[batch] {len(clouds)} clouds, {int(off[-1])} points, {tab['n_unique']} unique, "
+          f"{tab['n_slots']} slots, deferred {b.deferred()}")
+    for c, cl in enumerate(clouds):
+        _compare_cloud(tab, bf, nn, nrm, c, _single(sx, cl))
+
+
+def test_batch_small_clouds_vs_oracle(gpu, O):
+    sx = gpu
+    g = rng(11)
+    clouds = [g.uniform(-5, 5, size=(n, 3)).astype(np.float32) for n in (40, 333, 1500, 64, 65)]
+    b, off = _batch(sx, clouds)
+    tab = b.table()
+    bf = b.fetch()
+    nn = b.knn(K).numpy()
+    for c, cl in enumerate(clouds):
+        x, y, z = cl[:, 0], cl[:, 1], cl[:, 2]
+        bounds, _ = O.bbox(x, y, z)
+        codes = O.encode(x, y, z, bounds[0], bounds[2], 63)
+        ks, perm = O.sort_pairs(codes)
+        uq, rep, _ = O.unique(ks, perm)
+        pts = O.decode(uq, bounds[0], bounds[2], 63)
+        base, U = int(tab["base"][c]), int(tab["size"][c])
+        assert U == len(uq)
+        assert np.array_equal(bf["codes"][base:base + U], uq)
+        assert np.array_equal(bf["rep"][base:base + U], rep)
+        assert np.array_equal(bf["points"][base:base + U], pts)
+        assert np.array_equal(nn[base:base + U], O.knn_brute(pts, K)), c
+
+
+def test_batch_graph_replays_match_eager(gpu):
+    sx = gpu
+    clouds = _mixed_clouds(sx, 5)
+    off = np.zeros(len(clouds) + 1, np.int64)
+    off[1:] = np.cumsum([len(c) for c in clouds])
+    allp = np.concatenate(clouds, 0)
+    D = sx.DeviceArray
+    dx, dy, dz = (D.from_numpy(allp[:, i].copy()) for i in range(3))
+    b = sx.Batch(int(off[-1]), len(clouds))
+    b.set_clouds(off)
+    b.build(dx, dy, dz)
+    tab = b.table()
+    nn_e = b.knn(K)
+    nrm_e = b.normals(nn_e, K).numpy()
+    nn_e = nn_e.numpy()
+    S = tab["n_slots"]
+    nn_g = D((S, K), np.int32)
+    nrm_g = D((S, 3), np.float32)
+    st = sx.Stream()
+    graph = b.graph(dx, dy, dz, K, nn_g, nrm_g, st.handle)
+    valid = np.zeros(S, bool)
+    for c in range(len(clouds)):
+        valid[tab["base"][c]:tab["base"][c] + tab["size"][c]] = True
+    for rep in range(3):
+        nn_g.fill_(0xFF)
+        graph.launch(st.handle)
+        st.sync()
+        assert np.array_equal(nn_g.numpy()[valid], nn_e[valid]), rep
+        assert np.array_equal(nrm_g.numpy()[valid].view(np.uint32), nrm_e[valid].view(np.uint32)), rep
+        # eager work between replays
+        b.knn(K, out=D((S, K), np.int32))
+        sx.synchronize()
+    graph.close()
+
+
+def test_batch_autotune_keeps_results(gpu):
+    sx = gpu
+    clouds = _mixed_clouds(sx, 7)
+    b, off = _batch(sx, clouds)
+    tab = b.table()
+    nn0 = b.knn(K).numpy()
+    scratch = sx.DeviceArray((tab["n_slots"], K), np.int32)
+    res = b.autotune(K, scratch, buckets=(8, 16, 32), queue_blocks=(0, 512), reps=1)
+    print(f"[autotune] {res}")
+    assert res["bucket"] in (8, 16, 32) and res["queue_blocks"] in (0, 512)
+    assert np.all(res["ms"] > 0)
+    nn1 = b.knn(K).numpy()
+    valid = np.zeros(tab["n_slots"], bool)
+    for c in range(len(clouds)):
+        valid[tab["base"][c]:tab["base"][c] + tab["size"][c]] = True
+    assert np.array_equal(nn0[valid], nn1[valid])
+
+
+def test_batch_validation(gpu):
+    sx = gpu
+    b = sx.Batch(1000, 4)
+    with pytest.raises(sx.SxError):
+        b.set_clouds(np.array([0, 10, 5], np.int64))  # decreasing
+    with pytest.raises(sx.SxError):
+        b.set_clouds(np.array([1, 10], np.int64))  # not starting at 0
+    with pytest.raises(sx.SxError):
+        b.set_clouds(np.array([0, 600, 1200], np.int64))  # over capacity
+    with pytest.raises(sx.SxError):
+        b.knn(K)  # not built
+    x = np.ones(10, np.float32)
+    x[3] = np.nan
+    b.set_clouds(np.array([0, 4, 10], np.int64))
+    b.build(x, np.ones(10, np.float32), np.ones(10, np.float32))
+    with pytest.raises(sx.SxError):
+        b.view()
+
+
+def test_batch_512_clouds(gpu):
+    """The batched configuration: 512 scenes, sizes log-uniform in [1k, 200k]."""
+    sx = gpu
+    sizes = sx.gen_batch_sizes(4, 512)
+    clouds = []
+    for i, n in enumerate(sizes):
+        x, y, z = sx.gen_scene(4 * 1000 + i, int(n))
+        clouds.append(np.stack([x, y, z], 1))
+    b, off = _batch(sx, clouds)
+    tab = b.table()
+    print(f"[batch512] points {int(off[-1])}, unique {tab['n_unique']}, slots {tab['n_slots']}")
+    assert 15_000_000 < off[-1] < 25_000_000
+    bf = b.fetch()
+    nn = b.knn(K)
+    nrm = b.normals(nn, K).numpy()
+    nn = nn.numpy()
+    g = rng(4)
+    sample = sorted(set(g.choice(512, 6, replace=False).tolist()) | {int(np.argmin(sizes)), int(np.argmax(sizes))})
+    for c in sample:
+        _compare_cloud(tab, bf, nn, nrm, c, _single(sx, clouds[c]))
+    # every valid row starts with its own point and is sorted by distance
+    for c in range(0, 512, 37):
+        base, U = int(tab["base"][c]), int(tab["size"][c])
+        rows = nn[base:base + U]
+        P = bf["points"][base:base + U]
+        assert np.array_equal(rows[:, 0], np.arange(U))
+        D = P[rows] - P[:, None, :]  # float32, the kernel's operation order
+        d = (D[..., 0] * D[..., 0] + D[..., 1] * D[..., 1]) + D[..., 2] * D[..., 2]
+        assert np.all(np.diff(d, axis=1) >= 0)
