@@ -47,23 +47,25 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(workload_n: int, k: int) -> dict:
+def cpu_baseline(w, workload_n: int, k: int) -> dict:
+    """The oracle's restatement of the reference pipeline on a bounded sample
+    of the same workload (same generator and seed), on the host cores."""
     from oracle import oracle as O
 
     import shapeextraction_amd as sx
 
-    n = int(os.environ.get("SX_CPU_SAMPLE", 1_048_576))
-    x, y, z = sx.gen_scene(2, n)
+    n = min(int(os.environ.get("SX_CPU_SAMPLE", 1_048_576)), workload_n)
+    x, y, z = sx.Workload(w.name, n, w.k, w.code_bits, w.seed, w.kind).generate()
     t0 = time.perf_counter()
-    U, _, stage = O.pipeline(x, y, z, 63, k, knn_mode=0)
+    U, _, stage = O.pipeline(x, y, z, w.code_bits, k, knn_mode=0)
     dt = time.perf_counter() - t0
     return {
         "value": n / dt,
         "unit": "points/s",
         "cores": O.threads(),
         "kind": "port",
-        "sample": f"{n}-point scene from the same generator and seed as the {workload_n}-point workload; "
-                  f"reference pipeline restated on the host (32-slot queue kNN, FP32 Jacobi), "
+        "sample": f"{n}-point {w.kind} cloud from the same generator and seed as the {workload_n}-point "
+                  f"workload; reference pipeline restated on the host (32-slot queue kNN, FP32 Jacobi), "
                   f"{dt:.1f} s, stage ms {[round(float(s), 1) for s in stage]}",
     }
 
@@ -79,6 +81,10 @@ def main() -> int:
     ap.add_argument("--knn", choices=["plain", "learned", "both"], default="both",
                     help="learned/both use shapeextraction_amd/models/radius_k{k}.bin (trained here if missing)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="scene8m",
+                    help="scene8m (default, the headline), gmm16m, degenerate4m or sphere100k")
+    ap.add_argument("--batched", action="store_true",
+                    help="time the batched workload (512 scenes, sizes log-uniform in [1k, 200k]) instead")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run three device-to-device copies of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -104,13 +110,26 @@ def main() -> int:
         sx.synchronize()
         log(f"copied {nb} bytes x3")
         return 0
-    n, k = args.n, args.k
+    if args.batched:
+        return bench_batched(sx, args, dev)
+    if args.workload not in sx.WORKLOADS:
+        print(f"unknown workload {args.workload}; choose from {sorted(sx.WORKLOADS)}", file=sys.stderr)
+        return 2
+    w = sx.WORKLOADS[args.workload]
+    headline = w.name == "scene8m"
+    n, k, bits = (args.n, args.k, 63) if headline else (w.n, w.k, w.code_bits)
     t = time.perf_counter()
-    x, y, z = sx.gen_scene(2, n)
+    cls = dirs = None
+    if headline:
+        x, y, z = sx.gen_scene(2, n)
+    elif w.kind == "degenerate":
+        x, y, z, cls, dirs = sx.gen_degenerate(w.seed, n)
+    else:
+        x, y, z = w.generate()
     log(f"generated {n} points in {time.perf_counter() - t:.1f} s")
     st = sx.Stream()
     dx, dy, dz = (sx.DeviceArray.from_numpy(a) for a in (x, y, z))
-    ix = sx.Index(n, 63)
+    ix = sx.Index(n, bits)
     ix.set_bucket(args.bucket)
 
     # first build to learn U (the only host read of a count; untimed)
@@ -125,7 +144,9 @@ def main() -> int:
 
     model = None
     model_path = learned.default_model_path(k)
-    if args.knn in ("learned", "both"):
+    if args.knn in ("learned", "both") and not headline and not model_path.exists():
+        log(f"no trained radius model for k={k}: plain kNN only")
+    elif args.knn in ("learned", "both"):
         if not model_path.exists():
             log(f"no trained radius model at {model_path}; training one on this GPU")
             model = learned.train(k, log=log)
@@ -222,9 +243,11 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"scene8m: {n} float32 points, 200 random primitives (planes/spheres/cylinders, "
-                        f"sizes 1-20) in [0,100]^3, noise 0.01, 1% outliers, seed 2; 63-bit Morton codes; "
-                        f"k={k}; FP32 normals",
+            "workload": (f"scene8m: {n} float32 points, 200 random primitives (planes/spheres/cylinders, "
+                         f"sizes 1-20) in [0,100]^3, noise 0.01, 1% outliers, seed 2; 63-bit Morton codes; "
+                         f"k={k}; FP32 normals") if headline else
+                        (f"{w.name}: {n} float32 points, {w.kind} generator, seed {w.seed}; {bits}-bit Morton "
+                         f"codes; k={k}; FP32 normals"),
             "knn": head,
             "bucket": args.bucket,
             "n_unique": U,
@@ -251,12 +274,126 @@ def main() -> int:
             "frac": pipe_bytes / (ms_per_step / 1000.0) / 1e9 / HBM_PEAK_GBS,
         },
     }
+    if cls is not None:
+        line["per_class_angular_error_deg"] = per_class_errors(sx, ix, nn, normals, cls, dirs, k, st)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(n, k)
+        line["cpu_baseline"] = cpu_baseline(w, n, k)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def per_class_errors(sx, ix, nn, normals, cls, dirs, k, st) -> dict:
+    """Per structure class: FP32 normals against the FP64 path (FP64
+    covariance + FP64 Jacobi on the GPU, itself pinned to the CPU FP64 PCA in
+    the tests).  A line's normal is only defined up to a rotation about the
+    line, so for lines the error is the deviation from perpendicular."""
+    n32 = normals.numpy().astype(np.float64)
+    n64 = ix.normals(nn, k, precision=sx.SX_PREC_FP64, stream=st.handle).numpy().astype(np.float64)
+    rep = ix.fetch()["rep"]
+    ucls, udir = cls[rep], dirs[rep].astype(np.float64)
+    vs64 = np.degrees(np.arccos(np.clip(np.abs(np.sum(n32 * n64, axis=1)), 0.0, 1.0)))
+    perp = np.degrees(np.arcsin(np.clip(np.abs(np.sum(n32 * udir, axis=1)), 0.0, 1.0)))
+    out = {}
+    for c, name in enumerate(("lines", "corners", "slabs")):
+        m = ucls == c
+        if not m.any():
+            continue
+        a = perp[m] if c == 0 else vs64[m]
+        out[name] = {"n": int(m.sum()), "measure": "deviation from perpendicular" if c == 0 else "vs FP64",
+                     "median": float(np.median(a)), "p99": float(np.percentile(a, 99)), "max": float(a.max())}
+    return out
+
+
+def bench_batched(sx, args, dev) -> int:
+    """512 independent scenes (seed 4*1000+i, sizes log-uniform in [1k, 200k])
+    through one captured graph: build + kNN (k=16) + FP32 normals."""
+    k, n_clouds = 16, 512
+    sizes = sx.gen_batch_sizes(4, n_clouds)
+    off = np.zeros(n_clouds + 1, np.int64)
+    off[1:] = np.cumsum(sizes)
+    n = int(off[-1])
+    xs, ys, zs = np.empty(n, np.float32), np.empty(n, np.float32), np.empty(n, np.float32)
+    t = time.perf_counter()
+    for i in range(n_clouds):
+        x, y, z = sx.gen_scene(4 * 1000 + i, int(sizes[i]))
+        xs[off[i]:off[i + 1]], ys[off[i]:off[i + 1]], zs[off[i]:off[i + 1]] = x, y, z
+    log(f"generated {n_clouds} clouds, {n} points in {time.perf_counter() - t:.1f} s")
+    st = sx.Stream()
+    dx, dy, dz = (sx.DeviceArray.from_numpy(a) for a in (xs, ys, zs))
+    b = sx.Batch(n, n_clouds)
+    b.set_clouds(off)
+    b.build(dx, dy, dz, stream=st.handle)
+    tab = b.table(stream=st.handle)
+    S = tab["n_slots"]
+    nn = sx.DeviceArray((S, k), np.int32)
+    normals = sx.DeviceArray((S, 3), np.float32)
+    tune = b.autotune(k, nn, buckets=(8, 16, 32), queue_blocks=(0, 2048), reps=2, stream=st.handle)
+    log(f"autotune: bucket {tune['bucket']}, queue blocks {tune['queue_blocks']}, ms {tune['ms'].round(3).tolist()}")
+
+    def eager():  # the same three calls as the graph, no host synchronisation
+        sx._lib.call("sx_batch_build", b.handle, dx.ptr, dy.ptr, dz.ptr, st.handle)
+        sx._lib.call("sx_batch_knn", b.handle, k, nn.ptr, S, st.handle)
+        sx._lib.call("sx_batch_normals", b.handle, nn.ptr, k, sx.SX_PREC_FP32, 0.0, 0.0, 0.0, normals.ptr, None, S,
+                     st.handle)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        st.sync()
+        e0, e1 = sx.Event(), sx.Event()
+        e0.record(st.handle)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        e1.record(st.handle)
+        st.sync()
+        return e0.elapsed_ms(e1) / steps, (time.perf_counter() - t0) * 1000.0 / steps
+
+    eager_ms, eager_wall = timed(eager, args.steps, args.warmup)
+    graph = b.graph(dx, dy, dz, k, nn, normals, st.handle)
+    graph_ms, graph_wall = timed(lambda: graph.launch(st.handle), args.steps, args.warmup)
+    b.deferred(reset=True, stream=st.handle)
+    graph.launch(st.handle)
+    deferred = b.deferred(reset=True, stream=st.handle)
+    nrm = normals.numpy()
+    valid = np.zeros(S, bool)
+    for c in range(n_clouds):
+        valid[tab["base"][c]:tab["base"][c] + tab["size"][c]] = True
+    assert np.isfinite(nrm[valid]).all()
+    log(f"eager {eager_ms:.2f} ms (wall {eager_wall:.2f}), graph {graph_ms:.2f} ms (wall {graph_wall:.2f}), "
+        f"deferred {deferred}")
+    line = {
+        "metric": "points/sec batched normals, 512 clouds (log-uniform 1k-200k points), k=16",
+        "value": n / (graph_wall / 1000.0),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": graph_wall,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batch512: {n_clouds} scenes (200 primitives, noise 0.01, 1% outliers, seeds 4000+i), "
+                        f"sizes log-uniform in [1000, 200000], {n} points; 63-bit codes; k={k}; FP32 normals",
+            "n_points": n, "n_unique": tab["n_unique"], "n_slots": S,
+            "cloud_sizes": {"min": int(sizes.min()), "median": int(np.median(sizes)), "max": int(sizes.max())},
+            "device": dev,
+        },
+        "per_batch_ms": {"graph_events": graph_ms, "graph_wall": graph_wall, "eager_events": eager_ms,
+                         "eager_wall": eager_wall},
+        "graph_speedup_vs_eager": eager_wall / graph_wall,
+        "autotune": {"bucket": tune["bucket"], "queue_blocks": tune["queue_blocks"],
+                     "candidates_ms": tune["ms"].tolist()},
+        "deferred_queries": deferred,
+    }
+    print(json.dumps(line), flush=True)
+    graph.close()
     return 0
 
 

@@ -217,7 +217,8 @@ int sx_index_destroy(sx_index* index);
 /* Builds the index from device SoA coordinates; asynchronous on `stream`. */
 int sx_index_build(sx_index* index, const float* d_x, const float* d_y, const float* d_z, int64_t n,
                    void* stream);
-/* Synchronises `stream`, validates counts and fills *view. */
+/* Orders `stream` after the last sx_index_build (whatever stream it ran on),
+ * synchronises it, validates counts and fills *view. */
 int sx_index_get_view(sx_index* index, sx_index_view* view, void* stream);
 /* Bucket size of the kNN descent: a child holding at most `bucket` points is
  * scanned linearly instead of descended (default 16; tuning knob, no effect
@@ -334,23 +335,26 @@ int sx_batch_set_clouds(sx_batch* batch, const int64_t* h_offsets, int n_clouds,
 int sx_batch_set_tuning(sx_batch* batch, int bucket, int sort_algo, int queue_blocks);
 /* Asynchronous, no host synchronisation, capturable. */
 int sx_batch_build(sx_batch* batch, const float* d_x, const float* d_y, const float* d_z, void* stream);
-/* Synchronises; SX_EDATA when an input coordinate is not finite. */
+/* Waits for the last sx_batch_build (whatever its stream); SX_EDATA when an
+ * input coordinate is not finite. */
 int sx_batch_get_view(sx_batch* batch, sx_batch_view* view, void* stream);
-/* d_out: n_slots x k int32 (16-byte aligned); rows of padding slots are not written. */
-int sx_batch_knn(sx_batch* batch, int k, int32_t* d_out, void* stream);
+/* d_out: n_rows x k int32 (16-byte aligned); rows are written for slots
+ * < min(n_rows, n_slots) (rows of padding slots are not written); n_rows is
+ * the row capacity of the caller's buffers (n_slots from the view). */
+int sx_batch_knn(sx_batch* batch, int k, int32_t* d_out, int64_t n_rows, void* stream);
 int sx_batch_normals(sx_batch* batch, const int32_t* d_nn, int k, int precision, float vx, float vy, float vz,
-                     float* d_normals, uint8_t* d_flags, void* stream);
+                     float* d_normals, uint8_t* d_flags, int64_t n_rows, void* stream);
 /* Queries the kNN tiles handed to the per-query search since the last reset. */
 int sx_batch_deferred(sx_batch* batch, int64_t* deferred, int reset, void* stream);
 /* One hipGraph of build + kNN (k) + FP32 normals (viewpoint at the origin) on
  * the given created stream; launching it reruns the whole batch. */
 int sx_batch_graph_create(sx_batch* batch, const float* d_x, const float* d_y, const float* d_z, int k,
-                          int32_t* d_nn, float* d_normals, void** graph, void* stream);
+                          int32_t* d_nn, float* d_normals, int64_t n_rows, void** graph, void* stream);
 int sx_batch_graph_launch(void* graph, void* stream);
 int sx_batch_graph_destroy(void* graph);
 /* Times the built batch's kNN for every (bucket, queue_blocks) candidate and
  * keeps the fastest.  h_ms (nullable): n_buckets x n_queue times in ms. */
-int sx_batch_autotune(sx_batch* batch, int k, int32_t* d_scratch_nn, const int* h_buckets, int n_buckets,
+int sx_batch_autotune(sx_batch* batch, int k, int32_t* d_scratch_nn, int64_t n_rows, const int* h_buckets, int n_buckets,
                       const int* h_queue_blocks, int n_queue, int reps, float* h_ms, int* best_bucket,
                       int* best_queue_blocks, void* stream);
 

@@ -166,13 +166,13 @@ SIGNATURES = {
     "sx_batch_set_tuning": (I, [P, I, I, I]),
     "sx_batch_build": (I, [P, P, P, P, P]),
     "sx_batch_get_view": (I, [P, C.POINTER(BatchView), P]),
-    "sx_batch_knn": (I, [P, I, P, P]),
-    "sx_batch_normals": (I, [P, P, I, I, F, F, F, P, P, P]),
+    "sx_batch_knn": (I, [P, I, P, I64, P]),
+    "sx_batch_normals": (I, [P, P, I, I, F, F, F, P, P, I64, P]),
     "sx_batch_deferred": (I, [P, C.POINTER(C.c_int64), I, P]),
-    "sx_batch_graph_create": (I, [P, P, P, P, I, P, P, C.POINTER(P), P]),
+    "sx_batch_graph_create": (I, [P, P, P, P, I, P, P, I64, C.POINTER(P), P]),
     "sx_batch_graph_launch": (I, [P, P]),
     "sx_batch_graph_destroy": (I, [P]),
-    "sx_batch_autotune": (I, [P, I, P, C.POINTER(C.c_int), I, C.POINTER(C.c_int), I, I, C.POINTER(C.c_float),
+    "sx_batch_autotune": (I, [P, I, P, I64, C.POINTER(C.c_int), I, C.POINTER(C.c_int), I, I, C.POINTER(C.c_float),
                               C.POINTER(C.c_int), C.POINTER(C.c_int), P]),
 }
 

@@ -366,9 +366,9 @@ class Batch:
         self._view = v
         return v
 
-    def table(self) -> dict:
+    def table(self, stream=None) -> dict:
         """Per-cloud base slot, unique count, bounds (host copies)."""
-        v = self._view or self.view()
+        v = self._view or self.view(stream)
         n = int(v.n_clouds)
         return {
             "n_slots": int(v.n_slots),
@@ -394,19 +394,20 @@ class Batch:
         return out
 
     def knn(self, k: int, out: DeviceArray | None = None, stream=None) -> DeviceArray:
-        v = self._view or self.view()
+        v = self._view or self.view(stream)
         if out is None:
             out = DeviceArray((int(v.n_slots), k), np.int32)
-        call("sx_batch_knn", self.handle, k, out.ptr, stream)
+        call("sx_batch_knn", self.handle, k, out.ptr, out.shape[0], stream)
         return out
 
     def normals(self, nn: DeviceArray, k: int, precision: int = SX_PREC_FP32, viewpoint=(0.0, 0.0, 0.0),
                 out: DeviceArray | None = None, flags: DeviceArray | None = None, stream=None) -> DeviceArray:
-        v = self._view or self.view()
+        v = self._view or self.view(stream)
         if out is None:
             out = DeviceArray((int(v.n_slots), 3), np.float32)
+        rows = min(out.shape[0], nn.shape[0])
         call("sx_batch_normals", self.handle, nn.ptr, k, precision, *viewpoint, out.ptr,
-             flags.ptr if flags is not None else None, stream)
+             flags.ptr if flags is not None else None, rows, stream)
         return out
 
     def deferred(self, reset: bool = True, stream=None) -> int:
@@ -417,7 +418,9 @@ class Batch:
     def graph(self, x: DeviceArray, y: DeviceArray, z: DeviceArray, k: int, nn: DeviceArray, normals: DeviceArray,
               stream) -> "BatchGraph":
         g = C.c_void_p()
-        call("sx_batch_graph_create", self.handle, x.ptr, y.ptr, z.ptr, k, nn.ptr, normals.ptr, C.byref(g), stream)
+        rows = min(nn.shape[0], normals.shape[0])
+        call("sx_batch_graph_create", self.handle, x.ptr, y.ptr, z.ptr, k, nn.ptr, normals.ptr, rows, C.byref(g),
+             stream)
         self._view = None
         return BatchGraph(g.value, (x, y, z, nn, normals))
 
@@ -427,8 +430,8 @@ class Batch:
         q = (C.c_int * len(queue_blocks))(*queue_blocks)
         ms = (C.c_float * (len(buckets) * len(queue_blocks)))()
         bb, bq = C.c_int(0), C.c_int(0)
-        call("sx_batch_autotune", self.handle, k, scratch.ptr, b, len(buckets), q, len(queue_blocks), reps, ms,
-             C.byref(bb), C.byref(bq), stream)
+        call("sx_batch_autotune", self.handle, k, scratch.ptr, scratch.shape[0], b, len(buckets), q,
+             len(queue_blocks), reps, ms, C.byref(bb), C.byref(bq), stream)
         return {"bucket": bb.value, "queue_blocks": bq.value,
                 "ms": np.array(list(ms), np.float32).reshape(len(buckets), len(queue_blocks))}
 

@@ -143,6 +143,7 @@ struct sx_batch {
     int sort_algo = SX_SORT_ONESWEEP;
     int queue_blocks = 0;   // deferred-query workers; 0 = default
     void* mem = nullptr;
+    hipEvent_t built_event = nullptr;  // recorded after every eager build
     // per input point / sorted position
     uint64_t *codes_in = nullptr, *codes_a = nullptr, *keys_c = nullptr, *keys_s = nullptr, *codes_s = nullptr;
     uint32_t *perm_a = nullptr, *perm = nullptr;
@@ -247,15 +248,17 @@ int batch_build(sx_batch* b, const float* x, const float* y, const float* z, hip
     return SX_OK;
 }
 
-int batch_knn(sx_batch* b, int k, int32_t* out, hipStream_t st) {
-    return sx::knn_tiles_launch(b->pts16, b->trav, b->counts + sx::kBSlots, 0, b->slot_cap, k, nullptr, b->bucket,
+// Rows are written for slots < min(n_rows, slot count): n_rows is the row
+// capacity of the caller's buffers.
+int batch_knn(sx_batch* b, int k, int32_t* out, int64_t n_rows, hipStream_t st) {
+    return sx::knn_tiles_launch(b->pts16, b->trav, b->counts + sx::kBSlots, 0, n_rows, k, nullptr, b->bucket,
                                 out, b->queue, b->counts + sx::kBQueue, b->counts + sx::kBDeferred, st, b->map(),
                                 b->queue_blocks);
 }
 
 int batch_normals(sx_batch* b, const int32_t* nn, int k, int precision, float vx, float vy, float vz, float* normals,
-                  uint8_t* flags, hipStream_t st) {
-    return sx::normals_launch(b->pts16, b->counts + sx::kBSlots, nn, 0, b->slot_cap, k, precision, vx, vy, vz,
+                  uint8_t* flags, int64_t n_rows, hipStream_t st) {
+    return sx::normals_launch(b->pts16, b->counts + sx::kBSlots, nn, 0, n_rows, k, precision, vx, vy, vz,
                               normals, flags, st, b->map());
 }
 
@@ -288,8 +291,11 @@ int sx_batch_create(sx_batch** out, int64_t max_points, int max_clouds, int code
     }
     batch_layout(b, b->mem);
     int rc = sx::fill_i32(b->counts, 0, 8, nullptr);
+    if (rc == SX_OK && hipEventCreateWithFlags(&b->built_event, hipEventDisableTiming) != hipSuccess)
+        rc = fail(SX_EHIP, "sx_batch_create: event creation failed");
     if (rc == SX_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(SX_EHIP, "sx_batch_create: sync failed");
     if (rc != SX_OK) {
+        if (b->built_event) (void)hipEventDestroy(b->built_event);
         (void)hipFree(b->mem);
         delete b;
         return rc;
@@ -300,6 +306,7 @@ int sx_batch_create(sx_batch** out, int64_t max_points, int max_clouds, int code
 
 int sx_batch_destroy(sx_batch* b) {
     if (!b) return SX_OK;
+    if (b->built_event) SX_HIP(hipEventDestroy(b->built_event));
     if (b->mem) SX_HIP(hipFree(b->mem));
     delete b;
     return SX_OK;
@@ -339,6 +346,7 @@ int sx_batch_build(sx_batch* b, const float* d_x, const float* d_y, const float*
     if (b->n_clouds < 1) return fail(SX_ESTATE, "sx_batch_build: set the clouds first");
     if (!d_x || !d_y || !d_z) return fail(SX_EINVAL, "sx_batch_build: null coordinates");
     SX_TRY(batch_build(b, d_x, d_y, d_z, sx::as_stream(stream)));
+    SX_HIP(hipEventRecord(b->built_event, sx::as_stream(stream)));
     b->built = true;
     return SX_OK;
 }
@@ -347,6 +355,7 @@ int sx_batch_get_view(sx_batch* b, sx_batch_view* view, void* stream) {
     SX_TRY(check_batch(b));
     if (!view) return fail(SX_EINVAL, "sx_batch_get_view: null view");
     hipStream_t st = sx::as_stream(stream);
+    SX_HIP(hipEventSynchronize(b->built_event));  // whichever stream the build ran on
     int32_t h[8];
     SX_HIP(hipMemcpyAsync(h, b->counts, sizeof(h), hipMemcpyDeviceToHost, st));
     SX_HIP(hipStreamSynchronize(st));
@@ -368,20 +377,23 @@ int sx_batch_get_view(sx_batch* b, sx_batch_view* view, void* stream) {
     return SX_OK;
 }
 
-int sx_batch_knn(sx_batch* b, int k, int32_t* d_out, void* stream) {
+int sx_batch_knn(sx_batch* b, int k, int32_t* d_out, int64_t n_rows, void* stream) {
     SX_TRY(check_batch(b));
     if (!valid_k(k)) return fail(SX_EINVAL, "sx_batch_knn: k must be 8, 16, 24 or 32");
     if (!d_out || reinterpret_cast<uintptr_t>(d_out) % 16 != 0)
         return fail(SX_EINVAL, "sx_batch_knn: output must be a 16-byte aligned device buffer");
-    return batch_knn(b, k, d_out, sx::as_stream(stream));
+    if (n_rows < 0 || n_rows > b->slot_cap) return fail(SX_EINVAL, "sx_batch_knn: n_rows outside [0, slot capacity]");
+    return batch_knn(b, k, d_out, n_rows, sx::as_stream(stream));
 }
 
 int sx_batch_normals(sx_batch* b, const int32_t* d_nn, int k, int precision, float vx, float vy, float vz,
-                     float* d_normals, uint8_t* d_flags, void* stream) {
+                     float* d_normals, uint8_t* d_flags, int64_t n_rows, void* stream) {
     SX_TRY(check_batch(b));
     if (!valid_k(k)) return fail(SX_EINVAL, "sx_batch_normals: k must be 8, 16, 24 or 32");
     if (!d_nn || !d_normals) return fail(SX_EINVAL, "sx_batch_normals: null buffer");
-    return batch_normals(b, d_nn, k, precision, vx, vy, vz, d_normals, d_flags, sx::as_stream(stream));
+    if (n_rows < 0 || n_rows > b->slot_cap)
+        return fail(SX_EINVAL, "sx_batch_normals: n_rows outside [0, slot capacity]");
+    return batch_normals(b, d_nn, k, precision, vx, vy, vz, d_normals, d_flags, n_rows, sx::as_stream(stream));
 }
 
 int sx_batch_deferred(sx_batch* b, int64_t* deferred, int reset, void* stream) {
@@ -396,20 +408,22 @@ int sx_batch_deferred(sx_batch* b, int64_t* deferred, int reset, void* stream) {
 }
 
 int sx_batch_graph_create(sx_batch* b, const float* d_x, const float* d_y, const float* d_z, int k, int32_t* d_nn,
-                          float* d_normals, void** graph, void* stream) {
+                          float* d_normals, int64_t n_rows, void** graph, void* stream) {
     if (!b || !graph) return fail(SX_EINVAL, "sx_batch_graph_create: null argument");
     *graph = nullptr;
     if (b->n_clouds < 1) return fail(SX_ESTATE, "sx_batch_graph_create: set the clouds first");
     if (!valid_k(k)) return fail(SX_EINVAL, "sx_batch_graph_create: k must be 8, 16, 24 or 32");
     if (!d_x || !d_y || !d_z || !d_nn || !d_normals || reinterpret_cast<uintptr_t>(d_nn) % 16 != 0)
         return fail(SX_EINVAL, "sx_batch_graph_create: null or misaligned buffer");
+    if (n_rows < 0 || n_rows > b->slot_cap)
+        return fail(SX_EINVAL, "sx_batch_graph_create: n_rows outside [0, slot capacity]");
     hipStream_t st = sx::as_stream(stream);
     if (!st) return fail(SX_EINVAL, "sx_batch_graph_create: capture needs a created stream, not the null stream");
     auto* g = new BatchGraph();
     SX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = batch_build(b, d_x, d_y, d_z, st);
-    if (rc == SX_OK) rc = batch_knn(b, k, d_nn, st);
-    if (rc == SX_OK) rc = batch_normals(b, d_nn, k, SX_PREC_FP32, 0.0f, 0.0f, 0.0f, d_normals, nullptr, st);
+    if (rc == SX_OK) rc = batch_knn(b, k, d_nn, n_rows, st);
+    if (rc == SX_OK) rc = batch_normals(b, d_nn, k, SX_PREC_FP32, 0.0f, 0.0f, 0.0f, d_normals, nullptr, n_rows, st);
     hipError_t e = hipStreamEndCapture(st, &g->graph);
     if (rc == SX_OK && e != hipSuccess) rc = fail(SX_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     if (rc == SX_OK) {
@@ -447,12 +461,13 @@ int sx_batch_graph_destroy(void* graph) {
 // pair in the candidate lists (reps runs each after one warm-up), keeps the
 // fastest on the batch and reports it.  ms (nullable) receives n_buckets x
 // n_queue times in milliseconds, row-major.
-int sx_batch_autotune(sx_batch* b, int k, int32_t* d_scratch_nn, const int* h_buckets, int n_buckets,
+int sx_batch_autotune(sx_batch* b, int k, int32_t* d_scratch_nn, int64_t n_rows, const int* h_buckets, int n_buckets,
                       const int* h_queue_blocks, int n_queue, int reps, float* h_ms, int* best_bucket,
                       int* best_queue_blocks, void* stream) {
     SX_TRY(check_batch(b));
     if (!valid_k(k)) return fail(SX_EINVAL, "sx_batch_autotune: k must be 8, 16, 24 or 32");
-    if (!d_scratch_nn || !h_buckets || !h_queue_blocks || n_buckets < 1 || n_queue < 1 || reps < 1)
+    if (!d_scratch_nn || !h_buckets || !h_queue_blocks || n_buckets < 1 || n_queue < 1 || reps < 1 || n_rows < 0 ||
+        n_rows > b->slot_cap)
         return fail(SX_EINVAL, "sx_batch_autotune: bad argument");
     hipStream_t st = sx::as_stream(stream);
     hipEvent_t e0, e1;
@@ -465,10 +480,10 @@ int sx_batch_autotune(sx_batch* b, int k, int32_t* d_scratch_nn, const int* h_bu
     for (int i = 0; i < n_buckets && rc == SX_OK; ++i) {
         for (int j = 0; j < n_queue && rc == SX_OK; ++j) {
             rc = sx_batch_set_tuning(b, h_buckets[i], b->sort_algo, h_queue_blocks[j]);
-            if (rc == SX_OK) rc = batch_knn(b, k, d_scratch_nn, st);  // warm-up
+            if (rc == SX_OK) rc = batch_knn(b, k, d_scratch_nn, n_rows, st);  // warm-up
             float ms = 0.0f;
             if (rc == SX_OK && hipEventRecord(e0, st) != hipSuccess) rc = fail(SX_EHIP, "sx_batch_autotune: event");
-            for (int r = 0; r < reps && rc == SX_OK; ++r) rc = batch_knn(b, k, d_scratch_nn, st);
+            for (int r = 0; r < reps && rc == SX_OK; ++r) rc = batch_knn(b, k, d_scratch_nn, n_rows, st);
             if (rc == SX_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
                 rc = fail(SX_EHIP, "sx_batch_autotune: timing failed");

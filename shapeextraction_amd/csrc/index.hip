@@ -99,6 +99,7 @@ struct sx_index {
     int64_t n_input = 0;
     bool built = false;
     bool counts_valid = false;
+    hipEvent_t built_event = nullptr;  // recorded after every build
     int64_t n_unique = 0;
     int64_t n_oct = 0;
     float host_bounds[3] = {0, 0, 0};
@@ -206,8 +207,11 @@ int sx_index_create(sx_index** out, int64_t capacity, int code_bits, int root_mo
     index_layout(ix, ix->mem);
     int rc = sx::fill_u8(ix->bounds, 0, 256, nullptr);                    // bounds + counts
     if (rc == SX_OK) rc = sx::fill_u8(ix->ws_bbox, 0, sx::bbox_ws_bytes(), nullptr);  // bbox counter
+    if (rc == SX_OK && hipEventCreateWithFlags(&ix->built_event, hipEventDisableTiming) != hipSuccess)
+        rc = fail(SX_EHIP, "sx_index_create: event creation failed");
     if (rc == SX_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(SX_EHIP, "sx_index_create: sync failed");
     if (rc != SX_OK) {
+        if (ix->built_event) (void)hipEventDestroy(ix->built_event);
         (void)hipFree(ix->mem);
         delete ix;
         return rc;
@@ -221,6 +225,8 @@ int sx_index_destroy(sx_index* ix) {
     int rc = SX_OK;
     if (ix->nn_buf && hipFree(ix->nn_buf) != hipSuccess) rc = fail(SX_EHIP, "sx_index_destroy: hipFree failed");
     if (ix->mem && hipFree(ix->mem) != hipSuccess) rc = fail(SX_EHIP, "sx_index_destroy: hipFree failed");
+    if (ix->built_event && hipEventDestroy(ix->built_event) != hipSuccess)
+        rc = fail(SX_EHIP, "sx_index_destroy: hipEventDestroy failed");
     delete ix;
     return rc;
 }
@@ -273,6 +279,7 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
     SX_TRY(sx::octree_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, ix->bounds, ix->pn, ix->lc, ix->hll, ix->hlr,
                              ix->parent, ix->oct, ix->oct_cap, ix->counts + kCntOct, ix->leaf_oct, ix->ws, st));
     ix->n_input = n;
+    SX_HIP(hipEventRecord(ix->built_event, sx::as_stream(stream)));
     ix->built = true;
     return SX_OK;
 }
@@ -280,6 +287,8 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
 int sx_index_get_view(sx_index* ix, sx_index_view* view, void* stream) {
     SX_TRY(check_index(ix));
     if (!view) return fail(SX_EINVAL, "sx_index_get_view: null view");
+    // order this stream after the build, whichever stream the build ran on
+    SX_HIP(hipStreamWaitEvent(sx::as_stream(stream), ix->built_event, 0));
     // the input-order map is only built when a caller asks for the view
     SX_TRY(sx::scatter_i32_launch(ix->uos, ix->perm, ix->n_input, ix->uoi, sx::as_stream(stream)));
     int64_t nu = 0, no = 0;

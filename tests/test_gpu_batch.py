@@ -217,3 +217,50 @@ def test_batch_512_clouds(gpu):
         D = P[rows] - P[:, None, :]  # float32, the kernel's operation order
         d = (D[..., 0] * D[..., 0] + D[..., 1] * D[..., 1]) + D[..., 2] * D[..., 2]
         assert np.all(np.diff(d, axis=1) >= 0)
+
+
+def test_batch_view_waits_for_build_on_another_stream(gpu):
+    """The view of a batch built on a created (non-blocking) stream, read
+    through the null stream, must see the finished build."""
+    sx = gpu
+    clouds = _mixed_clouds(sx, 9)
+    off = np.zeros(len(clouds) + 1, np.int64)
+    off[1:] = np.cumsum([len(c) for c in clouds])
+    allp = np.concatenate(clouds, 0)
+    D = sx.DeviceArray
+    dx, dy, dz = (D.from_numpy(allp[:, i].copy()) for i in range(3))
+    st = sx.Stream()
+    b = sx.Batch(int(off[-1]), len(clouds))
+    b.set_clouds(off)
+    b.build(dx, dy, dz, stream=st.handle)
+    tab = b.table()  # null stream
+    st.sync()
+    b2 = sx.Batch(int(off[-1]), len(clouds))
+    b2.set_clouds(off)
+    b2.build(dx, dy, dz)
+    ref = b2.table()
+    assert tab["n_slots"] == ref["n_slots"] > 0
+    assert np.array_equal(tab["base"], ref["base"]) and np.array_equal(tab["size"], ref["size"])
+
+
+def test_batch_knn_respects_row_capacity(gpu):
+    """Rows are written only below the row capacity the caller passes."""
+    sx = gpu
+    clouds = _mixed_clouds(sx, 13)
+    b, off = _batch(sx, clouds)
+    tab = b.table()
+    S = tab["n_slots"]
+    rows = S // 3
+    out = sx.DeviceArray((S, K), np.int32)
+    out.fill_(0x7F)
+    sx._lib.call("sx_batch_knn", b.handle, K, out.ptr, rows, None)
+    sx.synchronize()
+    got = out.numpy()
+    assert np.all(got[rows:] == 0x7F7F7F7F)
+    full = b.knn(K).numpy()
+    valid = np.zeros(S, bool)
+    for c in range(len(clouds)):
+        valid[tab["base"][c]:tab["base"][c] + tab["size"][c]] = True
+    keep = valid.copy()
+    keep[rows:] = False
+    assert np.array_equal(got[keep], full[keep])
