@@ -28,8 +28,9 @@
 #include "sx_internal.hpp"
 
 namespace sx {
+size_t seg_bbox_ws_bytes(int max_clouds);
 int seg_bbox_launch(const float* x, const float* y, const float* z, const int64_t* off, int n_clouds, float* bounds,
-                    int32_t* d_bad, hipStream_t st);
+                    int32_t* d_bad, float* partial, hipStream_t st);
 int seg_encode_launch(const float* x, const float* y, const float* z, int64_t n, const int64_t* off, int n_clouds,
                       const float* bounds, int code_bits, uint64_t* codes, hipStream_t st);
 int seg_decode_launch(const uint64_t* codes, const int32_t* d_slots, int64_t n_max, CloudMap cm, const float* bounds,
@@ -80,14 +81,12 @@ struct BatchUniqueOut {
     __device__ void operator()(int64_t i, int ex, int v) const { uos[i] = ex + v - 1; }
 };
 
-// One block: unique points per cloud, 64-aligned first slots, the slot total
-// and the cloud of every 64-slot tile.  Cloud c's sorted positions are its
-// input block [off[c], off[c+1]).
+// One block: unique points per cloud, 64-aligned first slots and the slot
+// total.  Cloud c's sorted positions are its input block [off[c], off[c+1]).
 __global__ void __launch_bounds__(kScanBlock) cloud_table_kernel(const int64_t* __restrict__ off, int n_clouds,
                                                                  const int32_t* __restrict__ uos,
                                                                  int32_t* __restrict__ size, int32_t* __restrict__ base,
-                                                                 int32_t* __restrict__ counts,
-                                                                 int32_t* __restrict__ tile_cloud) {
+                                                                 int32_t* __restrict__ counts) {
     int carry = 0;
     for (int c0 = 0; c0 < n_clouds; c0 += kScanBlock) {
         const int c = c0 + static_cast<int>(threadIdx.x);
@@ -106,11 +105,20 @@ __global__ void __launch_bounds__(kScanBlock) cloud_table_kernel(const int64_t* 
         counts[kBSlots] = carry;
         counts[kBUnique] = n_clouds > 0 ? uos[off[n_clouds] - 1] + 1 : 0;
     }
-    __syncthreads();  // base[] and size[] of this block are visible to it
-    for (int c = static_cast<int>(threadIdx.x); c < n_clouds; c += kScanBlock) {
-        const int t0 = base[c] / kCloudAlign, t1 = (base[c] + size[c] + kCloudAlign - 1) / kCloudAlign;
-        for (int t = t0; t < t1; ++t) tile_cloud[t] = c;
+}
+
+// Cloud of every 64-slot tile: the last cloud whose first slot is <= the tile's.
+__global__ void __launch_bounds__(256) tile_cloud_kernel(const int32_t* __restrict__ base, int n_clouds,
+                                                         const int32_t* __restrict__ counts, int64_t max_tiles,
+                                                         int32_t* __restrict__ tile_cloud) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= max_tiles || t * kCloudAlign >= counts[kBSlots]) return;
+    int lo = 0, hi = n_clouds - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (static_cast<int64_t>(base[mid]) <= t * kCloudAlign) lo = mid; else hi = mid - 1;
     }
+    tile_cloud[t] = lo;
 }
 
 // Unique (cloud, code) pairs to their cloud's slots; rep = the first input
@@ -151,6 +159,7 @@ struct sx_batch {
     // per cloud
     int64_t* off = nullptr;
     float* bounds = nullptr;
+    float* bbox_partial = nullptr;
     int32_t *size = nullptr, *base = nullptr;
     // per slot
     int32_t* tile_cloud = nullptr;
@@ -188,6 +197,7 @@ size_t batch_layout(sx_batch* b, void* base) {
     b->uos = c.take<int32_t>(n);
     b->off = c.take<int64_t>(mc + 1);
     b->bounds = c.take<float>(4 * mc);
+    b->bbox_partial = c.take<float>(sx::seg_bbox_ws_bytes(static_cast<int>(mc)) / sizeof(float));
     b->size = c.take<int32_t>(mc);
     b->base = c.take<int32_t>(mc);
     b->tile_cloud = c.take<int32_t>(s / sx::kCloudAlign + 1);
@@ -222,7 +232,7 @@ int batch_build(sx_batch* b, const float* x, const float* y, const float* z, hip
     const int cb = b->code_bits;
     // slot / unique / non-finite counts; the deferred-query total accumulates
     SX_TRY(sx::fill_i32(b->counts, 0, sx::kBDeferred, st));
-    SX_TRY(sx::seg_bbox_launch(x, y, z, b->off, nc, b->bounds, b->counts + sx::kBBad, st));
+    SX_TRY(sx::seg_bbox_launch(x, y, z, b->off, nc, b->bounds, b->counts + sx::kBBad, b->bbox_partial, st));
     SX_TRY(sx::seg_encode_launch(x, y, z, n, b->off, nc, b->bounds, cb, b->codes_in, st));
     // (cloud, code, input index) order: by code, then stably by cloud
     SX_TRY(sx::sort_pairs_launch(b->codes_in, nullptr, n, cb, b->codes_a, b->perm_a, b->ws, b->sort_algo, st));
@@ -234,9 +244,12 @@ int batch_build(sx_batch* b, const float* x, const float* y, const float* z, hip
     SX_LAUNCH_CHECK("gather_codes_kernel");
     SX_TRY(sx::device_scan(sx::BatchUniqueIn{b->keys_s, b->codes_s}, sx::BatchUniqueOut{b->uos}, nullptr, n, nullptr,
                            b->ws, st));
-    sx::cloud_table_kernel<<<1, sx::kScanBlock, 0, st>>>(b->off, nc, b->uos, b->size, b->base, b->counts,
-                                                         b->tile_cloud);
+    sx::cloud_table_kernel<<<1, sx::kScanBlock, 0, st>>>(b->off, nc, b->uos, b->size, b->base, b->counts);
     SX_LAUNCH_CHECK("cloud_table_kernel");
+    const int64_t max_tiles = b->slot_cap / sx::kCloudAlign + 1;
+    sx::tile_cloud_kernel<<<sx::grid_for(max_tiles, 256), 256, 0, st>>>(b->base, nc, b->counts, max_tiles,
+                                                                        b->tile_cloud);
+    SX_LAUNCH_CHECK("tile_cloud_kernel");
     sx::compact_kernel<<<sx::grid_for(n, 256), 256, 0, st>>>(n, b->keys_s, b->codes_s, b->perm, b->uos, b->off,
                                                               b->base, b->ucodes, b->rep);
     SX_LAUNCH_CHECK("compact_kernel");

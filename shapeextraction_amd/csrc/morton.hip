@@ -212,35 +212,57 @@ int decode_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
 // ---------------------------------------------------------------------------
 // Batched clouds: cloud c holds input points [off[c], off[c+1]).
 // ---------------------------------------------------------------------------
-// One block per cloud: bounds[4c] = (min, max, range) over its three axes,
-// the same reduction as bbox_kernel; non-finite coordinates add to *d_bad.
+// kSegBBoxSplit blocks per cloud reduce strided shares of its points into
+// partials; one thread per cloud then gives bounds[4c] = (min, max, range)
+// over its three axes (the same min/max as bbox_kernel, so the same bits);
+// non-finite coordinates add to *d_bad.
+constexpr int kSegBBoxSplit = 16;
+
 __global__ void __launch_bounds__(kBBoxBlock) seg_bbox_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                                               const float* __restrict__ z,
-                                                              const int64_t* __restrict__ off, float* bounds,
+                                                              const int64_t* __restrict__ off, float* partial,
                                                               int32_t* d_bad) {
-    const int c = blockIdx.x;
+    const int c = blockIdx.x / kSegBBoxSplit, part = blockIdx.x % kSegBBoxSplit;
     float mn = INFINITY, mx = -INFINITY;
     int bad = 0;
-    for (int64_t i = off[c] + threadIdx.x; i < off[c + 1]; i += blockDim.x) {
+    const int64_t stride = static_cast<int64_t>(kSegBBoxSplit) * blockDim.x;
+    for (int64_t i = off[c] + static_cast<int64_t>(part) * blockDim.x + threadIdx.x; i < off[c + 1]; i += stride) {
         bbox_acc(x[i], mn, mx, bad);
         bbox_acc(y[i], mn, mx, bad);
         bbox_acc(z[i], mn, mx, bad);
     }
     block_minmax(mn, mx, bad);
     if (threadIdx.x == 0) {
-        bounds[4 * c] = mn;
-        bounds[4 * c + 1] = mx;
-        bounds[4 * c + 2] = __fsub_rn(mx, mn);
-        bounds[4 * c + 3] = 0.0f;
+        partial[2 * blockIdx.x] = mn;
+        partial[2 * blockIdx.x + 1] = mx;
         if (bad) atomicAdd(d_bad, bad);
     }
 }
 
+__global__ void __launch_bounds__(256) seg_bbox_final_kernel(const float* __restrict__ partial, int n_clouds,
+                                                             float* bounds) {
+    const int c = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n_clouds) return;
+    float mn = INFINITY, mx = -INFINITY;
+    for (int p = 0; p < kSegBBoxSplit; ++p) {
+        mn = fminf(mn, partial[2 * (c * kSegBBoxSplit + p)]);
+        mx = fmaxf(mx, partial[2 * (c * kSegBBoxSplit + p) + 1]);
+    }
+    bounds[4 * c] = mn;
+    bounds[4 * c + 1] = mx;
+    bounds[4 * c + 2] = __fsub_rn(mx, mn);
+    bounds[4 * c + 3] = 0.0f;
+}
+
+size_t seg_bbox_ws_bytes(int max_clouds) { return sizeof(float) * 2 * kSegBBoxSplit * max_clouds; }
+
 int seg_bbox_launch(const float* x, const float* y, const float* z, const int64_t* off, int n_clouds, float* bounds,
-                    int32_t* d_bad, hipStream_t st) {
+                    int32_t* d_bad, float* partial, hipStream_t st) {
     if (n_clouds <= 0) return SX_OK;
-    seg_bbox_kernel<<<n_clouds, kBBoxBlock, 0, st>>>(x, y, z, off, bounds, d_bad);
+    seg_bbox_kernel<<<n_clouds * kSegBBoxSplit, kBBoxBlock, 0, st>>>(x, y, z, off, partial, d_bad);
     SX_LAUNCH_CHECK("seg_bbox_kernel");
+    seg_bbox_final_kernel<<<grid_for(n_clouds, 256), 256, 0, st>>>(partial, n_clouds, bounds);
+    SX_LAUNCH_CHECK("seg_bbox_final_kernel");
     return SX_OK;
 }
 

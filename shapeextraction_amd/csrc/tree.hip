@@ -235,6 +235,17 @@ __device__ Box3 range_box(const float4* __restrict__ pts, const BoxLevels& lv, i
     return bx;
 }
 
+// One range query per internal node: node i computes the box of its own
+// range [a, b] and writes it into its parent's record (the parent's left
+// child is node gamma_p, its right child gamma_p + 1), and writes the point
+// box of each leaf child into its own record.  The fields written by a node,
+// its parent and its sibling are disjoint.
+__device__ __forceinline__ void set_box(float4* lo, float4* hi, float lx, float ly, float lz, float hx, float hy,
+                                        float hz) {
+    lo->x = lx; lo->y = ly; lo->z = lz;  // .w (range / split / parent) untouched
+    hi->x = hx; hi->y = hy; hi->z = hz;
+}
+
 __global__ void __launch_bounds__(256) child_box_kernel(const float4* __restrict__ pts, BoxLevels lv,
                                                         const int32_t* d_count, int64_t n_max, TravNode* trav,
                                                         CloudMap cm) {
@@ -244,13 +255,23 @@ __global__ void __launch_bounds__(256) child_box_kernel(const float4* __restrict
     const int2 cloud = cm.at(i, static_cast<int>(total));
     if (i - cloud.x >= cloud.y - 1) return;  // not an internal node of its cloud
     TravNode* t = trav + i;
-    const int a = w_int(t->lo_l.w), g = w_int(t->hi_l.w), b = w_int(t->lo_r.w);
-    const Box3 l = range_box(pts, lv, a, g);
-    const Box3 r = range_box(pts, lv, g + 1, b);
-    t->lo_l.x = l.lx; t->lo_l.y = l.ly; t->lo_l.z = l.lz;
-    t->hi_l.x = l.hx; t->hi_l.y = l.hy; t->hi_l.z = l.hz;
-    t->lo_r.x = r.lx; t->lo_r.y = r.ly; t->lo_r.z = r.lz;
-    t->hi_r.x = r.hx; t->hi_r.y = r.hy; t->hi_r.z = r.hz;
+    const int a = w_int(t->lo_l.w), g = w_int(t->hi_l.w), b = w_int(t->lo_r.w), p = w_int(t->hi_r.w);
+    if (g == a) {
+        const float4 q = pts[a];
+        set_box(&t->lo_l, &t->hi_l, q.x, q.y, q.z, q.x, q.y, q.z);
+    }
+    if (b == g + 1) {
+        const float4 q = pts[b];
+        set_box(&t->lo_r, &t->hi_r, q.x, q.y, q.z, q.x, q.y, q.z);
+    }
+    if (p >= 0) {
+        const Box3 bx = range_box(pts, lv, a, b);
+        TravNode* pt = trav + p;
+        if (w_int(pt->hi_l.w) == static_cast<int>(i))
+            set_box(&pt->lo_l, &pt->hi_l, bx.lx, bx.ly, bx.lz, bx.hx, bx.hy, bx.hz);
+        else
+            set_box(&pt->lo_r, &pt->hi_r, bx.lx, bx.ly, bx.lz, bx.hx, bx.hy, bx.hz);
+    }
 }
 
 int box_levels_for(int64_t n_max) {

@@ -30,6 +30,7 @@ def _mixed_clouds(sx, seed):
     lat = g.integers(0, 6, size=(5000, 3)).astype(np.float32) * 0.5
     clouds.append(lat)
     clouds.append(np.zeros((7, 3), np.float32))  # one unique point
+    clouds += [np.zeros((0, 3), np.float32)] * 2  # empty clouds
     for i, n in enumerate((3000, 20000, 45000)):
         x, y, z = sx.gen_scene(seed * 10 + i, n)
         clouds.append(np.stack([x, y, z], 1))
@@ -90,6 +91,9 @@ def test_batch_equals_standalone_clouds(gpu):
     print(f"[batch] {len(clouds)} clouds, {int(off[-1])} points, {tab['n_unique']} unique, "
           f"{tab['n_slots']} slots, deferred {b.deferred()}")
     for c, cl in enumerate(clouds):
+        if len(cl) == 0:
+            assert tab["size"][c] == 0
+            continue
         _compare_cloud(tab, bf, nn, nrm, c, _single(sx, cl))
 
 
