@@ -2,6 +2,8 @@
 and rejects bad arguments before touching a device (no GPU needed)."""
 import ctypes as C
 import re
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -36,8 +38,65 @@ def test_otnode_layout():
     assert _lib.OTNode.corner.offset == 32 and _lib.OTNode.cell_size.offset == 44
 
 
+def _c_layout(tmp_path, structs):
+    """sizeof and offsetof of each (struct, fields) pair, as a C compiler lays
+    out the header's declarations."""
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "shapes_c.h"', "int main(void) {"]
+    for name, fields in structs.items():
+        lines.append(f'printf("{name} sizeof %zu\\n", sizeof({name}));')
+        for f in fields:
+            lines.append(f'printf("{name} {f} %zu\\n", offsetof({name}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", str(_lib.HEADER_PATH.parent), str(src), "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    table = {}
+    for line in out.splitlines():
+        s, f, v = line.split()
+        table[(s, f)] = int(v)
+    return table
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs a C compiler")
+def test_ctypes_structs_match_c_layout(tmp_path):
+    mirrors = {"sx_otnode": _lib.OTNode, "sx_index_view": _lib.IndexView, "sx_batch_view": _lib.BatchView}
+    structs = {n: [f[0] for f in m._fields_] for n, m in mirrors.items()}
+    table = _c_layout(tmp_path, structs)
+    for name, mirror in mirrors.items():
+        assert C.sizeof(mirror) == table[(name, "sizeof")], name
+        for field in structs[name]:
+            assert getattr(mirror, field).offset == table[(name, field)], (name, field)
+
+
 def _rc(name, *args):
     return getattr(_lib.load(), name)(*args)
+
+
+def test_batch_arguments_rejected_without_device():
+    h = C.c_void_p()
+    assert _rc("sx_batch_create", None, 100, 4, 63) == _lib.SX_EINVAL
+    assert _rc("sx_batch_create", C.byref(h), 0, 4, 63) == _lib.SX_EINVAL
+    assert h.value is None
+    assert _rc("sx_batch_create", C.byref(h), 1 << 30, 4, 63) == _lib.SX_EINVAL
+    assert _rc("sx_batch_create", C.byref(h), 100, 0, 63) == _lib.SX_EINVAL
+    assert _rc("sx_batch_create", C.byref(h), 100, (1 << 16) + 1, 63) == _lib.SX_EINVAL
+    assert _rc("sx_batch_create", C.byref(h), 100, 4, 31) == _lib.SX_EINVAL
+    assert b"code_bits" in _lib.load().sx_last_error()
+    offs = (C.c_int64 * 3)(0, 5, 9)
+    assert _rc("sx_batch_set_clouds", None, offs, 2, None) == _lib.SX_EINVAL
+    assert _rc("sx_batch_set_tuning", None, 16, 0, 0) == _lib.SX_EINVAL
+    assert _rc("sx_batch_build", None, None, None, None, None) == _lib.SX_EINVAL
+    assert _rc("sx_batch_get_view", None, None, None) == _lib.SX_EINVAL
+    assert _rc("sx_batch_knn", None, 16, None, 0, None) == _lib.SX_EINVAL
+    assert _rc("sx_batch_normals", None, None, 16, 0, 0.0, 0.0, 0.0, None, None, 0, None) == _lib.SX_EINVAL
+    d = C.c_int64()
+    assert _rc("sx_batch_deferred", None, C.byref(d), 0, None) == _lib.SX_EINVAL
+    assert _rc("sx_batch_graph_launch", None, None) == _lib.SX_EINVAL
+    assert _rc("sx_batch_graph_destroy", None) == _lib.SX_OK
+    assert _rc("sx_batch_destroy", None) == _lib.SX_OK
 
 
 def test_invalid_arguments_rejected_without_device():
