@@ -354,6 +354,66 @@ __device__ __forceinline__ uint64_t make_key(float d, int j) {
     return (static_cast<uint64_t>(__float_as_uint(d)) << 32) | static_cast<uint32_t>(j);
 }
 
+// ---------------------------------------------------------------------------
+// Lane-parallel selection by sorting networks.  A wave-wide broadcast insert
+// costs every lane a full K-slot shift whenever ANY lane improves, which in a
+// dense window of candidates is nearly every step.  Instead each lane sorts a
+// chunk of C keys with a fixed network and merges it into its list: the cost
+// is the same for every lane and does not depend on who accepts.
+// ---------------------------------------------------------------------------
+// Batcher's odd-even merge sort on N (a power of two) elements, as a list of
+// compare-exchange pairs (a < b): 19 for N = 8, 63 for N = 16.
+template <int N>
+struct OddEvenNet {
+    static constexpr int kMax = 8 * N;
+    int n = 0;
+    unsigned char a[kMax] = {}, b[kMax] = {};
+    constexpr OddEvenNet() {
+        for (int p = 1; p < N; p <<= 1)
+            for (int k = p; k >= 1; k >>= 1)
+                for (int j = k % p; j + k < N; j += 2 * k)
+                    for (int i = 0; i < k && i < N - j - k; ++i)
+                        if ((i + j) / (2 * p) == (i + j + k) / (2 * p)) {
+                            a[n] = static_cast<unsigned char>(i + j);
+                            b[n] = static_cast<unsigned char>(i + j + k);
+                            ++n;
+                        }
+    }
+};
+
+// One compare and four selects.  The empty asm hides that the selects pick
+// the compared values; otherwise they become a 64-bit min and max, each with
+// a compare of its own.
+__device__ __forceinline__ void cswap(uint64_t& x, uint64_t& y) {
+    const bool s = y < x;
+    uint64_t xo = x, yo = y;
+    asm("" : "+v"(xo), "+v"(yo));
+    x = s ? yo : xo;
+    y = s ? xo : yo;
+}
+
+template <int N>
+__device__ __forceinline__ void sort_keys(uint64_t (&x)[N]) {
+    constexpr OddEvenNet<N> net{};
+#pragma unroll
+    for (int c = 0; c < net.n; ++c) cswap(x[net.a[c]], x[net.b[c]]);
+}
+
+// Merge an ascending chunk c[0, C) into the ascending list (C <= K, K a power
+// of two): e[K-1-t] = min(e[K-1-t], c[t]) leaves the K smallest of both as a
+// bitonic sequence, which half-cleaners then sort.
+template <int K, int C>
+__device__ __forceinline__ void merge_chunk(uint64_t (&e)[K], const uint64_t (&c)[C]) {
+    static_assert(C <= K && (K & (K - 1)) == 0, "merge_chunk: C <= K, K a power of two");
+#pragma unroll
+    for (int t = 0; t < C; ++t) e[K - 1 - t] = c[t] < e[K - 1 - t] ? c[t] : e[K - 1 - t];
+#pragma unroll
+    for (int d = K / 2; d >= 1; d >>= 1)
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if ((j & d) == 0) cswap(e[j], e[j + d]);
+}
+
 // Flattened position f (< total) of a list of inclusive index ranges with
 // inclusive size prefix `pre` -> point index.
 __device__ __forceinline__ int range_point(const int2* rng, const int* pre, int nr, int f) {
@@ -399,6 +459,29 @@ __device__ __forceinline__ void broadcast_insert(const float4* cand, int n, floa
         const bool s1 = kLaneSkip && j1 >= skip_lo && j1 <= skip_hi;
         if (on && !s0 && d0 <= tau && k0 < top.e[K - 1]) top.insert(k0);
         if (on && !s1 && d1 <= tau && k1 < top.e[K - 1]) top.insert(k1);
+    }
+}
+
+// Lists of a power-of-two length take the window through sorting networks.
+template <int K>
+constexpr bool kNetWindow = (K & (K - 1)) == 0;
+
+// Merge the staged points cand[0, n) (n <= kWave; slots up to the next chunk
+// boundary hold points at +inf) into every lane's list, C keys at a time.
+template <int K>
+__device__ __forceinline__ void window_merge(const float4* cand, int n, float4 q, TopK<K>& top) {
+    constexpr int C = K <= 16 ? 8 : 16;  // K = 16: an 8-key chunk keeps the list and chunk within 80 VGPRs
+    static_assert(kWave % C == 0, "window_merge: chunks tile the staged points");
+#pragma unroll 1
+    for (int c0 = 0; c0 < n; c0 += C) {
+        uint64_t x[C];
+#pragma unroll
+        for (int t = 0; t < C; ++t) {
+            const float4 p = cand[c0 + t];
+            x[t] = make_key(dist2(q.x, q.y, q.z, p.x, p.y, p.z), __float_as_int(p.w));
+        }
+        sort_keys<C>(x);
+        merge_chunk<K, C>(top.e, x);
     }
 }
 
@@ -482,12 +565,23 @@ knn_tile_kernel(const float4* __restrict__ pts,
         whi = static_cast<int>(tile0 + kWave - 1 + kWindow < cend - 1 ? tile0 + kWave - 1 + kWindow : cend - 1);
         for (int base = wlo; base <= whi; base += kWave) {
             const int j = base + lane;
-            const float4 p = j <= whi ? pts[j] : make_float4(0.f, 0.f, 0.f, 0.f);
-            const int n = stage_candidates(s_cand[w], p, j, j <= whi);
-            broadcast_insert<K, false>(s_cand[w], n, q, true, 0xFFFFFFFFu, 0, -1, top);
+            if constexpr (kNetWindow<K>) {
+                // points past the window sit at +inf: their keys (bits
+                // 0x7F800000..) follow every finite one, so they reach the
+                // list only when the window holds fewer than K points, and
+                // such a lane never joins (its k-th distance is not finite)
+                const float4 p = j <= whi ? pts[j] : make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+                s_cand[w][lane] = make_float4(p.x, p.y, p.z, __int_as_float(j));
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                window_merge<K>(s_cand[w], whi - base + 1 < kWave ? whi - base + 1 : kWave, q, top);
+            } else {
+                const float4 p = j <= whi ? pts[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const int n = stage_candidates(s_cand[w], p, j, j <= whi);
+                broadcast_insert<K, false>(s_cand[w], n, q, true, 0xFFFFFFFFu, 0, -1, top);
+            }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
-        r2bits = top.kth_bits();  // 0xFFFFFFFF when the window holds fewer than K points
+        r2bits = top.kth_bits();  // >= 0x7F800000 when the window holds fewer than K points
     } else {
         const float r2 = valid ? radius2[qi - q_start] : -1.0f;
         r2bits = (r2 >= 0.0f && r2 < INFINITY) ? __float_as_uint(r2) : 0xFFFFFFFFu;
