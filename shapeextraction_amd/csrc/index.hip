@@ -34,10 +34,9 @@ int unique_launch(const uint64_t* sorted, const uint32_t* vals, int64_t n, uint6
                   int32_t* uos, int32_t* uoi, int32_t* d_count, void* ws, hipStream_t st);
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
-                  int32_t* node_end, int32_t* leaf_parent, TravNode* trav, hipStream_t st, CloudMap cm = CloudMap());
+                  int32_t* node_end, int32_t* leaf_parent, TravNode* trav, const float4* pts, void* box_ws,
+                  hipStream_t st, CloudMap cm = CloudMap());
 size_t box_ws_bytes(int64_t n_max);
-int child_boxes_launch(const float4* pts, const int32_t* d_count, int64_t n_max, TravNode* trav, void* ws,
-                       hipStream_t st, CloudMap cm = CloudMap());
 size_t octree_ws_bytes(int64_t n_max);
 int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, const float* bounds,
                   const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
@@ -268,14 +267,14 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
     SX_TRY(sx::decode_launch(ix->ucodes, ix->counts + kCntUnique, n, ix->bounds, cb, ix->pts12, ix->pts16, st));
     if (ix->root_mode == SX_ROOT_TEXTBOOK) {
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, ix->pn, ix->lc, ix->hll,
-                                 ix->hlr, ix->parent, nullptr, ix->leaf_parent, ix->trav, st));
+                                 ix->hlr, ix->parent, nullptr, ix->leaf_parent, ix->trav, ix->pts16, ix->ws, st));
     } else {
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_REFERENCE, ix->pn, ix->lc,
-                                 ix->hll, ix->hlr, ix->parent, nullptr, nullptr, nullptr, st));
+                                 ix->hll, ix->hlr, ix->parent, nullptr, nullptr, nullptr, nullptr, nullptr, st));
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, nullptr, nullptr,
-                                 nullptr, nullptr, nullptr, nullptr, ix->leaf_parent, ix->trav, st));
+                                 nullptr, nullptr, nullptr, nullptr, ix->leaf_parent, ix->trav, ix->pts16, ix->ws,
+                                 st));
     }
-    SX_TRY(sx::child_boxes_launch(ix->pts16, ix->counts + kCntUnique, n, ix->trav, ix->ws, st));
     SX_TRY(sx::octree_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, ix->bounds, ix->pn, ix->lc, ix->hll, ix->hlr,
                              ix->parent, ix->oct, ix->oct_cap, ix->counts + kCntOct, ix->leaf_oct, ix->ws, st));
     ix->n_input = n;

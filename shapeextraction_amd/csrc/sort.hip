@@ -24,8 +24,8 @@
 //   3. radix_scatter_kernel  stable rank of every key inside its tile (wave
 //                            ballot match on the digit + per-wave running
 //                            digit counters in LDS), then scatter.
-// A tile is 2048 keys: 4 waves x 8 rounds x 64 lanes, each wave owning a
-// contiguous 512-key span so that ranks follow input order.
+// A tile is 4096 keys: 4 waves x 16 rounds x 64 lanes, each wave owning a
+// contiguous 1024-key span so that ranks follow input order.
 #include <hip/hip_runtime.h>
 
 #include "sx_internal.hpp"
@@ -35,7 +35,7 @@ namespace sx {
 
 constexpr int kSortBlock = 256;
 constexpr int kSortWaves = kSortBlock / kWave;
-constexpr int kSortItems = 8;
+constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortBlock * kSortItems;
 constexpr int kWaveSpan = kSortItems * kWave;
 constexpr int kRadix = 256;
