@@ -259,7 +259,7 @@ def main() -> int:
                              if "learned" in results else None),
         "roofline": {
             "kernel": f"sx_knn ({head}): knn_tile_kernel<{k},{'true' if head == 'learned' else 'false'}> + "
-                      f"knn_wave_kernel<{k}>, HIP events around the call on its stream",
+                      f"knn_group_kernel<{k}>, HIP events around the call on its stream",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

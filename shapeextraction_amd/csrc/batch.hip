@@ -44,7 +44,7 @@ int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
                   hipStream_t st, CloudMap cm);
 size_t box_ws_bytes(int64_t n_max);
 int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
-                     int k, const float* radius2, int bucket, int32_t* out, int32_t* queue, int32_t* queue_count,
+                     int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
                      int32_t* deferred_total, hipStream_t st, CloudMap cm, int queue_blocks);
 int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn, int64_t q_start, int64_t n_q, int k,
                    int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st,
@@ -166,7 +166,8 @@ struct sx_batch {
     uint32_t* rep = nullptr;
     float4* pts16 = nullptr;
     uint8_t* pn = nullptr;
-    int32_t *lc = nullptr, *parent = nullptr, *queue = nullptr;
+    int32_t *lc = nullptr, *parent = nullptr;
+    int2* queue = nullptr;  // deferred queries: (slot, k-th distance bound bits)
     sx::TravNode* trav = nullptr;
     int32_t* counts = nullptr;
     void* ws = nullptr;
@@ -206,7 +207,7 @@ size_t batch_layout(sx_batch* b, void* base) {
     b->pn = c.take<uint8_t>(s);
     b->lc = c.take<int32_t>(s);
     b->parent = c.take<int32_t>(s);
-    b->queue = c.take<int32_t>(s);
+    b->queue = c.take<int2>(s);
     b->trav = c.take<sx::TravNode>(s);
     b->counts = c.take<int32_t>(8);
     b->ws = c.take<char>(batch_ws_bytes(n, s));

@@ -46,7 +46,7 @@ int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts
                const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
                hipStream_t st, CloudMap cm = CloudMap());
 int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
-                     int k, const float* radius2, int bucket, int32_t* out, int32_t* queue, int32_t* queue_count,
+                     int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
                      int32_t* deferred_total, hipStream_t st, CloudMap cm = CloudMap(), int queue_blocks = 0);
 int scatter_i32_launch(const int32_t* src, const uint32_t* idx, int64_t n, int32_t* dst, hipStream_t st);
 int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn, int64_t q_start, int64_t n_q, int k,
@@ -87,7 +87,7 @@ struct sx_index {
     int32_t* parent = nullptr;
     int32_t* leaf_parent = nullptr;
     int32_t* leaf_oct = nullptr;
-    int32_t* knn_queue = nullptr;
+    int2* knn_queue = nullptr;  // deferred queries: (index, k-th distance bound bits)
     sx::TravNode* trav = nullptr;
     sx_otnode* oct = nullptr;
     void* ws_bbox = nullptr;
@@ -127,7 +127,7 @@ size_t index_layout(sx_index* ix, void* base) {
     ix->parent = c.take<int32_t>(n);
     ix->leaf_parent = c.take<int32_t>(n);
     ix->leaf_oct = c.take<int32_t>(n);
-    ix->knn_queue = c.take<int32_t>(n);
+    ix->knn_queue = c.take<int2>(n);
     ix->trav = c.take<sx::TravNode>(n);
     ix->oct = c.take<sx_otnode>(static_cast<size_t>(ix->oct_cap));
     ix->ws_bbox = c.take<char>(sx::bbox_ws_bytes());

@@ -3,7 +3,8 @@
 //   knn_tile_kernel  64 Morton-consecutive queries per wave share one range
 //                    query and one candidate stream (the default for index
 //                    queries);
-//   knn_wave_kernel  one wave per query, for the queries a tile defers.
+//   knn_group_kernel one 16-lane group per query, for the queries a tile
+//                    defers.
 //
 // Replaces knnSearchKernel<k> and its two per-thread min-max heaps in local
 // memory (src/Octree.hpp:136-193, src/dequeue.hpp).  The reference search
@@ -256,7 +257,7 @@ int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts
 // list.  A lane's list is exact when (plain) it joined the search, or
 // (learned) it also found at least K points within its radius.  Other lanes
 // - and whole tiles whose frontier or range list overflows LDS - are appended
-// to a queue that knn_wave_kernel searches, one wave per query.
+// to a queue that knn_group_kernel searches, one 16-lane group per query.
 // ---------------------------------------------------------------------------
 constexpr int kTileWaves = kKnnBlock / kWave;
 constexpr int kTileStack = 256;                          // subtrees waiting to be tested against E
@@ -531,7 +532,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
                                                              const int32_t* __restrict__ d_counts, int64_t q_start,
                                                              int64_t n_q, const float* __restrict__ radius2,
                                                              int bucket, int32_t* __restrict__ out,
-                                                             int32_t* __restrict__ queue, int32_t* queue_count,
+                                                             int2* __restrict__ queue, int32_t* queue_count,
                                                              CloudMap cm) {
     __shared__ int s_stack[kTileWaves][kTileStack];
     __shared__ int2 s_range[kTileWaves][kTileRanges];
@@ -740,84 +741,200 @@ knn_tile_kernel(const float4* __restrict__ pts,
         int slot0 = 0;
         if (lane == __ffsll(static_cast<unsigned long long>(md)) - 1) slot0 = atomicAdd(queue_count, __popcll(md));
         slot0 = __shfl(slot0, __ffsll(static_cast<unsigned long long>(md)) - 1, kWave);
-        if (defer) queue[slot0 + rank_below(md)] = static_cast<int32_t>(qi);
+        // with the k-th key of its list, an upper bound on the k-th distance
+        // when the list holds K real points (+inf bits otherwise)
+        if (defer) queue[slot0 + rank_below(md)] = make_int2(static_cast<int32_t>(qi), static_cast<int>(top.kth_bits()));
     }
 }
 
 // ---------------------------------------------------------------------------
-// Queued queries, one wave each.  The list is seeded with the 64 index
-// points nearest in Morton order (wave-sorted; lane i holds the i-th key),
-// then subtrees are expanded 64 at a time from an LDS stack, pruned by the
-// current k-th distance (strict, as in knn_search), and their leaf ranges
-// streamed one point per lane.  A candidate that beats the k-th key is
-// inserted into the distributed list by a ballot for its rank and a shift.
-// A query whose stack would overflow is searched by lane 0 with knn_search.
+// Queued queries, one 16-lane group each (four per wave).  A query arrives
+// with an upper bound on its k-th squared distance: the k-th key of the list
+// its tile built from real points (+inf when that list held fewer than K).
+// The list is seeded with the query's 64 Morton neighbours (group-sorted),
+// and subtrees and candidates beyond the bound are skipped from the start.
+// Subtrees are expanded 16 at a time from a per-group LDS stack, pruned by min(bound, current k-th distance) (strict, ties survive),
+// and their leaf ranges streamed one point per lane.  The list is spread
+// over the group - entry e*16 + l is register e of lane l - and a candidate
+// that beats the k-th key is inserted by a ballot for its rank and a shift;
+// the four groups of a wave insert their own candidates in the same steps.
+// Most deferred queries have frontiers narrower than 16 subtrees, so a
+// group does the work a whole wave did, at a quarter of the instructions.
+// A query whose stack would overflow is marked in the queue (index
+// complemented) and searched by knn_overflow_kernel, one lane per query: the
+// per-lane search inlined here would raise the group kernel's register count
+// for every query.
 // ---------------------------------------------------------------------------
-constexpr int kStackCap = 512;
-constexpr int kWaveRanges = 2 * kWave;
+constexpr int kGroup = 16;
+constexpr int kGroupsPerWave = kWave / kGroup;
+constexpr int kGroupStack = 160;
+constexpr int kGroupRanges = 2 * kGroup;  // one expansion adds at most two per lane
+
+// Group-local collectives; gs = 16 * (group within the wave).
+__device__ __forceinline__ uint32_t group_ballot(bool p, int gs) {
+    return static_cast<uint32_t>(__ballot(p) >> gs) & 0xFFFFu;
+}
+__device__ __forceinline__ uint64_t group_shfl64(uint64_t v, int src) {
+    const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(v), src, kGroup));
+    const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(v >> 32), src, kGroup));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t group_shfl_up64(uint64_t v) {
+    const uint32_t lo = static_cast<uint32_t>(__shfl_up(static_cast<int>(v), 1, kGroup));
+    const uint32_t hi = static_cast<uint32_t>(__shfl_up(static_cast<int>(v >> 32), 1, kGroup));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ int group_incl_sum(int v, int gl) {
+#pragma unroll
+    for (int o = 1; o < kGroup; o <<= 1) {
+        const int t = __shfl_up(v, o, kGroup);
+        if (gl >= o) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t group_shfl_xor64(uint64_t v, int mask) {
+    const uint32_t lo = static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), mask, kGroup));
+    const uint32_t hi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(v >> 32), mask, kGroup));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+// Ascending bitonic sort of 64 keys over a group: element r*16 + l is
+// register r of lane l.  Partners 16 or 32 apart are in the same lane.
+__device__ __forceinline__ void group_sort64(uint64_t (&x)[4], int gl) {
+#pragma unroll
+    for (int k = 2; k <= 4 * kGroup; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= kGroup) {
+                const int rj = j / kGroup;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (r & rj) continue;
+                    const bool asc = ((r * kGroup + gl) & k) == 0;
+                    uint64_t& a = x[r];
+                    uint64_t& b = x[r | rj];
+                    const bool s = asc ? b < a : a < b;
+                    const uint64_t lo = s ? b : a;
+                    b = s ? a : b;
+                    a = lo;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const uint64_t o = group_shfl_xor64(x[r], j);
+                    const bool asc = ((r * kGroup + gl) & k) == 0;
+                    const bool keep_min = ((gl & j) == 0) == asc;
+                    x[r] = keep_min ? (x[r] < o ? x[r] : o) : (x[r] < o ? o : x[r]);
+                }
+            }
+        }
+    }
+}
+
+// Sorted list of K keys spread over a group.
+template <int K>
+struct GroupList {
+    static constexpr int E = (K + kGroup - 1) / kGroup;
+    uint64_t key[E];
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int e = 0; e < E; ++e) key[e] = ~0ull;
+    }
+    __device__ __forceinline__ uint64_t kth() const { return group_shfl64(key[(K - 1) / kGroup], (K - 1) % kGroup); }
+    // x (uniform over the group) goes to its rank; entries behind it move up
+    __device__ __forceinline__ void insert(uint64_t x, int gl, int gs) {
+        int pos = 0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) pos += __popc(group_ballot(key[e] < x, gs));
+#pragma unroll
+        for (int e = E - 1; e >= 0; --e) {
+            const uint64_t up = group_shfl_up64(key[e]);
+            const uint64_t carry = e > 0 ? group_shfl64(key[e > 0 ? e - 1 : 0], kGroup - 1) : ~0ull;
+            const uint64_t prev = gl == 0 ? carry : up;
+            const int idx = e * kGroup + gl;
+            key[e] = idx >= K ? ~0ull : (idx == pos ? x : (idx > pos ? prev : key[e]));
+        }
+    }
+};
 
 template <int K>
-__global__ void __launch_bounds__(kKnnBlock) knn_wave_kernel(const float4* __restrict__ pts,
-                                                             const TravNode* __restrict__ nodes,
-                                                             const int32_t* __restrict__ d_counts, int64_t q_start,
-                                                             const int32_t* __restrict__ queue,
-                                                             const int32_t* __restrict__ queue_count, int bucket,
-                                                             int32_t* __restrict__ out, int32_t* deferred_total,
-                                                             CloudMap cm) {
-    __shared__ int2 s_stack[kTileWaves][kStackCap];  // (node, box distance bits)
-    __shared__ int2 s_rng[kTileWaves][kWaveRanges];
-    __shared__ int s_pre[kTileWaves][kWaveRanges];
-    const int w = static_cast<int>(threadIdx.x) / kWave;
+__global__ void __launch_bounds__(kKnnBlock) knn_group_kernel(const float4* __restrict__ pts,
+                                                              const TravNode* __restrict__ nodes,
+                                                              const int32_t* __restrict__ d_counts, int64_t q_start,
+                                                              int2* __restrict__ queue,
+                                                              const int32_t* __restrict__ queue_count, int bucket,
+                                                              int32_t* __restrict__ out, int32_t* deferred_total,
+                                                              CloudMap cm) {
+    constexpr int kGroups = kTileWaves * kGroupsPerWave;
+    __shared__ int2 s_stack[kGroups][kGroupStack];  // (node, box distance bits)
+    __shared__ int2 s_rng[kGroups][kGroupRanges];
+    __shared__ int s_pre[kGroups][kGroupRanges];
     const int lane = static_cast<int>(lane_id());
+    const int gl = lane & (kGroup - 1);
+    const int gs = lane & ~(kGroup - 1);
+    const int gid = static_cast<int>(threadIdx.x) / kGroup;  // group within the block
+    int2* stk = s_stack[gid];
+    int2* rng = s_rng[gid];
+    int* pre = s_pre[gid];
     const int total = d_counts[0];
     const int n = *queue_count;
     if (blockIdx.x == 0 && threadIdx.x == 0 && deferred_total) atomicAdd(deferred_total, n);
-    const int n_waves = static_cast<int>(gridDim.x) * kTileWaves;
-    for (int t = static_cast<int>(blockIdx.x) * kTileWaves + w; t < n; t += n_waves) {
-        const int qi = uniform(queue[t]);
+    const int n_groups = static_cast<int>(gridDim.x) * kGroups;
+    for (int t = static_cast<int>(blockIdx.x) * kGroups + gid; t < n; t += n_groups) {
+        const int2 entry = queue[t];
+        const int qi = entry.x;
+        const uint32_t bound = static_cast<uint32_t>(entry.y);
         const float4 q = pts[qi];
         // the query's cloud: points and radix nodes [cbase, cend)
         const int2 cloud = cm.at(qi, total);
         const int cbase = cloud.x, U = cloud.y, cend = cloud.x + cloud.y;
-        // seed: the 64 Morton neighbours of the query
-        int wbase = qi - kWave / 2;
-        if (wbase > cend - kWave) wbase = cend - kWave;
+        // seed: the 64 Morton neighbours of the query, group-sorted; the K
+        // nearest of them start the list
+        int wbase = qi - 2 * kGroup;
+        if (wbase > cend - 4 * kGroup) wbase = cend - 4 * kGroup;
         if (wbase < cbase) wbase = cbase;
-        const int wend = wbase + kWave - 1;
-        uint64_t key = ~0ull;
+        const int wend = wbase + 4 * kGroup - 1 < cend - 1 ? wbase + 4 * kGroup - 1 : cend - 1;
+        GroupList<K> list;
         {
-            const int j = wbase + lane;
-            if (j < cend) {
-                const float4 p = pts[j];
-                key = make_key(dist2(q.x, q.y, q.z, p.x, p.y, p.z), j);
+            uint64_t sk[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int j = wbase + r * kGroup + gl;
+                sk[r] = ~0ull;
+                if (j <= wend) {
+                    const float4 p = pts[j];
+                    sk[r] = make_key(dist2(q.x, q.y, q.z, p.x, p.y, p.z), j);
+                }
             }
+            group_sort64(sk, gl);
+#pragma unroll
+            for (int e = 0; e < GroupList<K>::E; ++e) list.key[e] = e * kGroup + gl < K ? sk[e] : ~0ull;
         }
-        key = wave_sort64(key);
-        uint64_t kth = read_lane64(key, K - 1);
-
+        uint64_t kth = list.kth();
         int sp = 0, nr = 0;
         bool overflow = false;
-        if (U <= kWave) {
-            sp = 0;  // the seed window holds every point
+        if (U <= 4 * kGroup) {
+            sp = 0;  // the seed holds every point
         } else if (U <= bucket) {
-            if (lane == 0) s_rng[w][0] = make_int2(cbase, cend - 1);
+            if (gl == 0) rng[0] = make_int2(cbase, cend - 1);
             nr = 1;
         } else {
-            if (lane == 0) s_stack[w][0] = make_int2(cbase, 0);  // the cloud's root
+            if (gl == 0) stk[0] = make_int2(cbase, 0);  // the cloud's root
             sp = 1;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         while (sp > 0 || nr > 0) {
             if (sp > 0) {
-                const int c = sp < kWave ? sp : kWave;
+                const int c = sp < kGroup ? sp : kGroup;
                 sp -= c;
-                const int2 e = lane < c ? s_stack[w][sp + lane] : make_int2(-1, -1);
+                const int2 e = gl < c ? stk[sp + gl] : make_int2(-1, -1);
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                const uint32_t kd = static_cast<uint32_t>(kth >> 32);
+                const uint32_t kb = static_cast<uint32_t>(kth >> 32);
+                const uint32_t kd = kb < bound ? kb : bound;
                 bool nodeL = false, nodeR = false, rangeL = false, rangeR = false;
                 int a = 0, g = 0, b = 0;
                 uint32_t dlb = 0, drb = 0;
-                if (lane < c && static_cast<uint32_t>(e.y) <= kd) {
+                if (gl < c && static_cast<uint32_t>(e.y) <= kd) {
                     const TravNode nd = nodes[e.x];
                     a = w_int(nd.lo_l.w);
                     g = w_int(nd.hi_l.w);
@@ -831,46 +948,55 @@ __global__ void __launch_bounds__(kKnnBlock) knn_wave_kernel(const float4* __res
                         if (b - g - 1 < bucket) rangeR = true; else nodeR = true;
                     }
                 }
-                const uint64_t mnl = __ballot(nodeL), mnr = __ballot(nodeR);
-                const uint64_t mrl = __ballot(rangeL), mrr = __ballot(rangeR);
-                const int pushes = __popcll(mnl) + __popcll(mnr);
-                if (sp + pushes > kStackCap) {
+                const uint32_t mnl = group_ballot(nodeL, gs), mnr = group_ballot(nodeR, gs);
+                const uint32_t mrl = group_ballot(rangeL, gs), mrr = group_ballot(rangeR, gs);
+                const int pushes = __popc(mnl) + __popc(mnr);
+                if (sp + pushes > kGroupStack) {
                     overflow = true;
                     break;
                 }
-                const int pnl = sp + rank_below(mnl);
-                const int pnr = sp + __popcll(mnl) + rank_below(mnr);
-                const int prl = nr + rank_below(mrl);
-                const int prr = nr + __popcll(mrl) + rank_below(mrr);
-                if (nodeL) s_stack[w][pnl] = make_int2(g, static_cast<int>(dlb));
-                if (nodeR) s_stack[w][pnr] = make_int2(g + 1, static_cast<int>(drb));
-                if (rangeL) s_rng[w][prl] = make_int2(a, g);
-                if (rangeR) s_rng[w][prr] = make_int2(g + 1, b);
+                const uint32_t below = (1u << gl) - 1u;
+                const int pnl = sp + __popc(mnl & below);
+                const int pnr = sp + __popc(mnl) + __popc(mnr & below);
+                const int prl = nr + __popc(mrl & below);
+                const int prr = nr + __popc(mrl) + __popc(mrr & below);
+                if (nodeL) stk[pnl] = make_int2(g, static_cast<int>(dlb));
+                if (nodeR) stk[pnr] = make_int2(g + 1, static_cast<int>(drb));
+                if (rangeL) rng[prl] = make_int2(a, g);
+                if (rangeR) rng[prr] = make_int2(g + 1, b);
                 sp += pushes;
-                nr += __popcll(mrl) + __popcll(mrr);
+                nr += __popc(mrl) + __popc(mrr);
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
             if (nr > 0) {
-                const int total = range_prefix(s_rng[w], s_pre[w], nr);
-                for (int f0 = 0; f0 < total; f0 += kWave) {
-                    const int f = f0 + lane;
-                    int j = f < total ? range_point(s_rng[w], s_pre[w], nr, f) : -1;
-                    if (j >= wbase && j <= wend) j = -1;  // in the seed
+                // inclusive size prefix of the ranges (nr <= kGroupRanges)
+                int run = 0;
+                for (int r0 = 0; r0 < nr; r0 += kGroup) {
+                    const int idx = r0 + gl;
+                    const int2 rg = idx < nr ? rng[idx] : make_int2(0, -1);
+                    const int incl = run + group_incl_sum(rg.y - rg.x + 1, gl);
+                    if (idx < nr) pre[idx] = incl;
+                    run = __shfl(incl, kGroup - 1, kGroup);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                for (int f0 = 0; f0 < run; f0 += kGroup) {
+                    const int f = f0 + gl;
                     uint64_t ck = ~0ull;
-                    if (j >= 0) {
-                        const float4 p = pts[j];
-                        ck = make_key(dist2(q.x, q.y, q.z, p.x, p.y, p.z), j);
+                    if (f < run) {
+                        const int j = range_point(rng, pre, nr, f);
+                        if (j < wbase || j > wend) {  // seed points are in the list already
+                            const float4 p = pts[j];
+                            ck = make_key(dist2(q.x, q.y, q.z, p.x, p.y, p.z), j);
+                        }
                     }
-                    uint64_t m = __ballot(ck < kth);
+                    uint32_t m = group_ballot(ck < kth && static_cast<uint32_t>(ck >> 32) <= bound, gs);
                     while (m) {
-                        const int src = static_cast<int>(__builtin_ctzll(m));
-                        m &= m - 1ull;
-                        const uint64_t x = read_lane64(ck, src);
+                        const int src = static_cast<int>(__builtin_ctz(m));
+                        m &= m - 1u;
+                        const uint64_t x = group_shfl64(ck, src);
                         if (x < kth) {
-                            const int pos = __popcll(__ballot(lane < K && key < x));
-                            const uint64_t up = shfl64(key, lane > 0 ? lane - 1 : 0);
-                            key = lane == pos ? x : (lane > pos ? up : key);
-                            kth = read_lane64(key, K - 1);
+                            list.insert(x, gl, gs);
+                            kth = list.kth();
                         }
                     }
                 }
@@ -878,24 +1004,50 @@ __global__ void __launch_bounds__(kKnnBlock) knn_wave_kernel(const float4* __res
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
         }
-        int32_t* row = out + static_cast<int64_t>(qi - q_start) * K;
         if (overflow) {
-            TopK<K> top;
-            top.init();
-            if (lane == 0) {
-                knn_search<K>(pts, nodes, cbase, U, q.x, q.y, q.z, 0xFFFFFFFFu, bucket, top);
-                write_row<K>(row, top, cbase);
-            }
+            // marked for knn_overflow_kernel, which searches it with one lane
+            if (gl == 0) queue[t].x = ~qi;
             continue;
         }
-        const int first = __builtin_amdgcn_readfirstlane(static_cast<int>(key)) - cbase;
-        if (lane < K) row[lane] = key == ~0ull ? first : static_cast<int32_t>(key & 0xFFFFFFFFull) - cbase;
+        int32_t* row = out + static_cast<int64_t>(qi - q_start) * K;
+        const int first = __shfl(static_cast<int>(list.key[0]), 0, kGroup) - cbase;
+#pragma unroll
+        for (int e = 0; e < GroupList<K>::E; ++e) {
+            const int idx = e * kGroup + gl;
+            if (idx < K)
+                row[idx] = list.key[e] == ~0ull ? first : static_cast<int32_t>(list.key[e] & 0xFFFFFFFFull) - cbase;
+        }
+    }
+}
+
+// Queue entries the group kernel marked (index complemented): one lane each,
+// knn_search bounded by the entry's k-th distance bound.
+template <int K>
+__global__ void __launch_bounds__(kKnnBlock) knn_overflow_kernel(const float4* __restrict__ pts,
+                                                                 const TravNode* __restrict__ nodes,
+                                                                 const int32_t* __restrict__ d_counts,
+                                                                 int64_t q_start, const int2* __restrict__ queue,
+                                                                 const int32_t* __restrict__ queue_count,
+                                                                 int bucket, int32_t* __restrict__ out, CloudMap cm) {
+    const int total = d_counts[0];
+    const int n = *queue_count;
+    const int stride = static_cast<int>(gridDim.x) * kKnnBlock;
+    for (int t = static_cast<int>(blockIdx.x) * kKnnBlock + static_cast<int>(threadIdx.x); t < n; t += stride) {
+        const int2 e = queue[t];
+        if (e.x >= 0) continue;
+        const int qi = ~e.x;
+        const float4 q = pts[qi];
+        const int2 cloud = cm.at(qi, total);
+        TopK<K> top;
+        top.init();
+        knn_search<K>(pts, nodes, cloud.x, cloud.y, q.x, q.y, q.z, static_cast<uint32_t>(e.y), bucket, top);
+        write_row<K>(out + static_cast<int64_t>(qi - q_start) * K, top, cloud.x);
     }
 }
 
 template <int K>
 int knn_tiles_k(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
-                const float* radius2, int bucket, int32_t* out, int32_t* queue, int32_t* queue_count,
+                const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
                 int32_t* deferred_total, hipStream_t st, CloudMap cm, int queue_blocks) {
     const int64_t tiles = (n_q + kWave - 1) / kWave;
     const unsigned g = static_cast<unsigned>((tiles + kTileWaves - 1) / kTileWaves);
@@ -906,19 +1058,23 @@ int knn_tiles_k(const float4* pts, const TravNode* nodes, const int32_t* d_count
         knn_tile_kernel<K, false><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, nullptr, bucket, out,
                                                           queue, queue_count, cm);
     SX_LAUNCH_CHECK("knn_tile_kernel");
-    int64_t gq = (n_q + kTileWaves - 1) / kTileWaves;  // one wave per query at most
+    constexpr int kGroups = kTileWaves * kGroupsPerWave;
+    int64_t gq = (n_q + kGroups - 1) / kGroups;  // one group per query at most
     if (gq > 8192) gq = 8192;
     if (queue_blocks > 0 && queue_blocks < gq) gq = queue_blocks;
-    knn_wave_kernel<K><<<static_cast<unsigned>(gq), kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, queue,
-                                                                      queue_count, bucket, out, deferred_total, cm);
-    SX_LAUNCH_CHECK("knn_wave_kernel");
+    knn_group_kernel<K><<<static_cast<unsigned>(gq), kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, queue,
+                                                                       queue_count, bucket, out, deferred_total, cm);
+    SX_LAUNCH_CHECK("knn_group_kernel");
+    knn_overflow_kernel<K><<<64, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, queue, queue_count, bucket, out,
+                                                     cm);
+    SX_LAUNCH_CHECK("knn_overflow_kernel");
     return SX_OK;
 }
 
-// Index queries through the tiles.  queue: n_q int32; queue_count: one int32,
+// Index queries through the tiles.  queue: n_q int2; queue_count: one int32,
 // cleared here; deferred_total accumulates the number of queued queries.
 int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
-                     int k, const float* radius2, int bucket, int32_t* out, int32_t* queue, int32_t* queue_count,
+                     int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
                      int32_t* deferred_total, hipStream_t st, CloudMap cm = CloudMap(), int queue_blocks = 0) {
     if (n_q <= 0) return SX_OK;
     if (reinterpret_cast<uintptr_t>(out) % 16 != 0) return fail(SX_EINVAL, "knn: output must be 16-byte aligned");
