@@ -277,6 +277,29 @@ def test_knn_radius_seed_is_exact(gpu):
             assert reruns < 0.05 * len(kth)
 
 
+def test_knn_equidistant_shell_exact(gpu, O):
+    """A query at the centre of a shell has every subtree within its k-th
+    distance, so its deferred search overflows the 16-lane group's stack and
+    is finished by the one-lane fallback; every row must still be exact."""
+    g = rng(11)
+    v = g.normal(size=(30_000, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    p = np.vstack([v, np.zeros((1, 3))]).astype(np.float32)
+    x, y, z = (np.ascontiguousarray(p[:, i]) for i in range(3))
+    ix = gpu.Index(len(x), 63).build(x, y, z)
+    ix.set_bucket(1)
+    ref = O.knn_brute(ix.fetch()["points"], 16)
+    assert np.array_equal(ix.knn(16).numpy(), ref)
+    # a tiny radius defers every query to the group kernel, with the bound of
+    # its 17-point seed list
+    r2 = gpu.DeviceArray.from_numpy(np.full(ix.n_unique, 1e-12, np.float32))
+    ix.knn_reruns(reset=True)
+    got = ix.knn(16, mode=gpu.SX_KNN_RADIUS, radius2=r2).numpy()
+    assert ix.knn_reruns() > 0.9 * ix.n_unique
+    assert np.array_equal(got, ref)
+    ix.set_bucket(16)
+
+
 @pytest.mark.parametrize("kind,seed,n,k", [("scene", 2, 1_000_000, 16), ("gmm", 3, 600_000, 32),
                                             ("degenerate", 5, 400_000, 24), ("sphere", 1, 100_000, 8)])
 def test_knn_algorithms_agree(gpu, kind, seed, n, k):
