@@ -263,7 +263,12 @@ constexpr int kTileWaves = kKnnBlock / kWave;
 constexpr int kTileStack = 256;                          // subtrees waiting to be tested against E
 constexpr int kRangeFlush = 64;                          // stream the ranges once this many are pending
 constexpr int kTileRanges = kRangeFlush - 1 + 2 * kWave;  // one expansion adds at most 2 per lane
-constexpr int kWindow = 32;
+// Plain window: the tile's points +-kWindow<K> in Morton order.  Measured:
+// K = 16 is flat from 16 to 64 (8M scene); K = 32 gains from 96 (16M
+// mixture: 24.7 -> 22.2 ms for the kNN) since 128 points hold too few
+// 32-neighbourhoods.
+template <int K>
+constexpr int kWindow = K >= 32 ? 96 : 32;
 constexpr int kSeed = 8;                                  // learned: each lane's own +-kSeed Morton neighbours
 constexpr int kSegBudget = 8192;                         // candidates one segment may stream before deferring
 constexpr float kSegGap = 8.0f;                          // segment break: gap > kSegGap x mean radius
@@ -562,8 +567,9 @@ knn_tile_kernel(const float4* __restrict__ pts,
     int wlo = 0, whi = -1, seed_lo = 0, seed_hi = -1;
     uint32_t r2bits;
     if (!kLearned) {
-        wlo = static_cast<int>(tile0 - kWindow > cbase ? tile0 - kWindow : cbase);
-        whi = static_cast<int>(tile0 + kWave - 1 + kWindow < cend - 1 ? tile0 + kWave - 1 + kWindow : cend - 1);
+        constexpr int hw = kWindow<K>;
+        wlo = static_cast<int>(tile0 - hw > cbase ? tile0 - hw : cbase);
+        whi = static_cast<int>(tile0 + kWave - 1 + hw < cend - 1 ? tile0 + kWave - 1 + hw : cend - 1);
         for (int base = wlo; base <= whi; base += kWave) {
             const int j = base + lane;
             if constexpr (kNetWindow<K>) {
