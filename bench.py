@@ -54,7 +54,7 @@ def cpu_baseline(w, workload_n: int, k: int) -> dict:
 
     import shapeextraction_amd as sx
 
-    n = min(int(os.environ.get("SX_CPU_SAMPLE", 1_048_576)), workload_n)
+    n = min(int(os.environ.get("SX_CPU_SAMPLE", 4_194_304)), workload_n)  # ~10 s on 16 host threads
     x, y, z = sx.Workload(w.name, n, w.k, w.code_bits, w.seed, w.kind).generate()
     t0 = time.perf_counter()
     U, _, stage = O.pipeline(x, y, z, w.code_bits, k, knn_mode=0)
