@@ -22,7 +22,7 @@ namespace sx {
 // launchers from the other translation units
 size_t bbox_ws_bytes();
 int bbox_launch(const float* x, const float* y, const float* z, int64_t n, float* bounds, int32_t* d_bad, void* ws,
-                bool clear_counter, hipStream_t st);
+                hipStream_t st);
 int encode_launch(const float* x, const float* y, const float* z, int64_t n, const float* bounds, int code_bits,
                   uint64_t* codes, hipStream_t st);
 int decode_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, const float* bounds, int code_bits,
@@ -205,7 +205,6 @@ int sx_index_create(sx_index** out, int64_t capacity, int code_bits, int root_mo
     }
     index_layout(ix, ix->mem);
     int rc = sx::fill_u8(ix->bounds, 0, 256, nullptr);                    // bounds + counts
-    if (rc == SX_OK) rc = sx::fill_u8(ix->ws_bbox, 0, sx::bbox_ws_bytes(), nullptr);  // bbox counter
     if (rc == SX_OK && hipEventCreateWithFlags(&ix->built_event, hipEventDisableTiming) != hipSuccess)
         rc = fail(SX_EHIP, "sx_index_create: event creation failed");
     if (rc == SX_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(SX_EHIP, "sx_index_create: sync failed");
@@ -259,7 +258,7 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
     // unique / octree / non-finite counts; the deferred-query total accumulates
     // across builds until sx_index_knn_reruns resets it
     SX_TRY(sx::fill_i32(ix->counts, 0, kCntReruns, st));
-    SX_TRY(sx::bbox_launch(d_x, d_y, d_z, n, ix->bounds, ix->counts + kCntBad, ix->ws_bbox, false, st));
+    SX_TRY(sx::bbox_launch(d_x, d_y, d_z, n, ix->bounds, ix->counts + kCntBad, ix->ws_bbox, st));
     SX_TRY(sx::encode_launch(d_x, d_y, d_z, n, ix->bounds, cb, ix->codes_in, st));
     SX_TRY(sx::sort_pairs_launch(ix->codes_in, nullptr, n, cb, ix->codes_sorted, ix->perm, ix->ws, ix->sort_algo, st));
     SX_TRY(sx::unique_launch(ix->codes_sorted, ix->perm, n, ix->ucodes, ix->rep, ix->uos, nullptr,

@@ -1,8 +1,8 @@
 // Cube bounds, Morton encode and decode.
 //
-// bbox:   one streaming pass over x, y, z (12 B/point) with a last-block-done
-//         reduction; replaces the six cub reductions and the host min/max of
-//         RadixTree::encodePoints (src/RadixTree.cu:208-241).
+// bbox:   one streaming pass over x, y, z (12 B/point) into per-block partials
+//         and a one-block reduction; replaces the six cub reductions and the
+//         host min/max of RadixTree::encodePoints (src/RadixTree.cu:208-241).
 // encode: 4 points per thread, float4 loads, 20 B/point of HBM traffic;
 //         replaces makeCodes / pointToCode (src/RadixTree.cu:21-39,
 //         src/MortonUtils.cu:7-19).
@@ -21,12 +21,10 @@ constexpr int kBBoxBlock = 256;
 constexpr int kBBoxMaxBlocks = 1024;
 
 __device__ __forceinline__ void bbox_acc(float v, float& mn, float& mx, int& bad) {
-    if (isfinite(v)) {
-        mn = fminf(mn, v);
-        mx = fmaxf(mx, v);
-    } else {
-        bad += 1;
-    }
+    const bool ok = isfinite(v);
+    mn = fminf(mn, ok ? v : INFINITY);
+    mx = fmaxf(mx, ok ? v : -INFINITY);
+    bad += ok ? 0 : 1;
 }
 
 __device__ __forceinline__ void block_minmax(float& mn, float& mx, int& bad) {
@@ -51,38 +49,56 @@ __device__ __forceinline__ void block_minmax(float& mn, float& mx, int& bad) {
     __syncthreads();
 }
 
-// partial: [blocks] float2 (min, max); bad_partial: [blocks]; counter: one
-// uint32 that is zero on entry and left zero on exit.
+// Per-block (min, max, non-finite count) into partial / bad_partial, then
+// one block reduces them.  A last-block-done reduction in one kernel would
+// need an agent-scope release per block, i.e. an L2 write-back on a GPU whose
+// XCDs have separate L2s; the kernel boundary orders the partials instead.
+// kVec: x, y, z are 16-byte aligned and read as float4.
+template <bool kVec>
 __global__ void __launch_bounds__(kBBoxBlock) bbox_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                                           const float* __restrict__ z, int64_t n, float2* partial,
-                                                          int* bad_partial, unsigned* counter, float* bounds,
-                                                          int32_t* d_bad) {
+                                                          int* bad_partial) {
     float mn = INFINITY, mx = -INFINITY;
     int bad = 0;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t rest = 0;
+    if (kVec) {
+        const int64_t n4 = n / 4;
+        const float4* x4 = reinterpret_cast<const float4*>(x);
+        const float4* y4 = reinterpret_cast<const float4*>(y);
+        const float4* z4 = reinterpret_cast<const float4*>(z);
+#pragma unroll 2
+        for (int64_t i = tid; i < n4; i += stride) {
+            const float4 a = x4[i], b = y4[i], c = z4[i];
+            bbox_acc(a.x, mn, mx, bad); bbox_acc(a.y, mn, mx, bad); bbox_acc(a.z, mn, mx, bad); bbox_acc(a.w, mn, mx, bad);
+            bbox_acc(b.x, mn, mx, bad); bbox_acc(b.y, mn, mx, bad); bbox_acc(b.z, mn, mx, bad); bbox_acc(b.w, mn, mx, bad);
+            bbox_acc(c.x, mn, mx, bad); bbox_acc(c.y, mn, mx, bad); bbox_acc(c.z, mn, mx, bad); bbox_acc(c.w, mn, mx, bad);
+        }
+        rest = n4 * 4;
+    }
+    for (int64_t i = rest + tid; i < n; i += stride) {
         bbox_acc(x[i], mn, mx, bad);
         bbox_acc(y[i], mn, mx, bad);
         bbox_acc(z[i], mn, mx, bad);
     }
     block_minmax(mn, mx, bad);
-    __shared__ bool s_last;
     if (threadIdx.x == 0) {
         partial[blockIdx.x] = make_float2(mn, mx);
         bad_partial[blockIdx.x] = bad;
-        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (prev == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!s_last) return;
-    mn = INFINITY; mx = -INFINITY; bad = 0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-        float pmn = __hip_atomic_load(&partial[b].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        float pmx = __hip_atomic_load(&partial[b].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int pb = __hip_atomic_load(&bad_partial[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        mn = fminf(mn, pmn);
-        mx = fmaxf(mx, pmx);
-        bad += pb;
+}
+
+__global__ void __launch_bounds__(kBBoxBlock) bbox_final_kernel(const float2* __restrict__ partial,
+                                                                const int* __restrict__ bad_partial, int n_blocks,
+                                                                float* bounds, int32_t* d_bad) {
+    float mn = INFINITY, mx = -INFINITY;
+    int bad = 0;
+    for (int b = threadIdx.x; b < n_blocks; b += blockDim.x) {
+        const float2 p = partial[b];
+        mn = fminf(mn, p.x);
+        mx = fmaxf(mx, p.y);
+        bad += bad_partial[b];
     }
     block_minmax(mn, mx, bad);
     if (threadIdx.x == 0) {
@@ -90,7 +106,6 @@ __global__ void __launch_bounds__(kBBoxBlock) bbox_kernel(const float* __restric
         bounds[1] = mx;
         bounds[2] = __fsub_rn(mx, mn);
         if (d_bad) *d_bad = bad;
-        *counter = 0u;
     }
 }
 
@@ -105,21 +120,24 @@ size_t bbox_ws_bytes() {
     Carver c(nullptr);
     c.take<float2>(kBBoxMaxBlocks);
     c.take<int>(kBBoxMaxBlocks);
-    c.take<unsigned>(1);
     return c.off;
 }
 
-// The workspace counter must be zero on the first use; the kernel leaves it
-// zero, so a workspace that was filled once can be reused.
 int bbox_launch(const float* x, const float* y, const float* z, int64_t n, float* bounds, int32_t* d_bad, void* ws,
-                bool clear_counter, hipStream_t st) {
+                hipStream_t st) {
     Carver c(ws);
     float2* partial = c.take<float2>(kBBoxMaxBlocks);
     int* badp = c.take<int>(kBBoxMaxBlocks);
-    unsigned* counter = c.take<unsigned>(1);
-    if (clear_counter) SX_TRY(fill_u8(counter, 0, sizeof(unsigned) * 4, st));
-    bbox_kernel<<<bbox_blocks(n), kBBoxBlock, 0, st>>>(x, y, z, n, partial, badp, counter, bounds, d_bad);
+    const int nb = bbox_blocks(n);
+    const bool vec = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y) |
+                       reinterpret_cast<uintptr_t>(z)) & 15u) == 0;
+    if (vec)
+        bbox_kernel<true><<<nb, kBBoxBlock, 0, st>>>(x, y, z, n, partial, badp);
+    else
+        bbox_kernel<false><<<nb, kBBoxBlock, 0, st>>>(x, y, z, n, partial, badp);
     SX_LAUNCH_CHECK("bbox_kernel");
+    bbox_final_kernel<<<1, kBBoxBlock, 0, st>>>(partial, badp, nb, bounds, d_bad);
+    SX_LAUNCH_CHECK("bbox_final_kernel");
     return SX_OK;
 }
 
@@ -336,7 +354,7 @@ int sx_bbox(const float* d_x, const float* d_y, const float* d_z, int64_t n, flo
     }
     if (*ws_bytes < sx::bbox_ws_bytes()) return fail(SX_ENOMEM, "sx_bbox: workspace too small");
     if (n <= 0 || !d_x || !d_y || !d_z || !d_bounds) return fail(SX_EINVAL, "sx_bbox: bad arguments");
-    return sx::bbox_launch(d_x, d_y, d_z, n, d_bounds, d_bad, ws, true, sx::as_stream(stream));
+    return sx::bbox_launch(d_x, d_y, d_z, n, d_bounds, d_bad, ws, sx::as_stream(stream));
 }
 
 int sx_morton_encode(const float* d_x, const float* d_y, const float* d_z, int64_t n, const float* d_bounds,

@@ -186,8 +186,9 @@ __global__ void __launch_bounds__(kSortBlock) onesweep_pass_kernel(const uint64_
     __shared__ uint32_t cnt[kSortWaves][kRadix];
     __shared__ uint32_t tile_start[kRadix];
     __shared__ uint32_t dest_base[kRadix];
-    __shared__ uint64_t s_keys[kSortTile];
-    __shared__ uint32_t s_vals[kSortTile];
+    // the tile in digit order: keys first, then (same buffer) values
+    __shared__ uint64_t s_buf[kSortTile];
+    __shared__ uint8_t s_digit[kSortTile];
     __shared__ uint32_t s_tile;
     const int tid = static_cast<int>(threadIdx.x);
     const int w = tid / kWave;
@@ -258,8 +259,8 @@ __global__ void __launch_bounds__(kSortBlock) onesweep_pass_kernel(const uint64_
     for (int r = 0; r < kSortItems; ++r) {
         const uint32_t d = static_cast<uint32_t>(k[r] >> shift) & (kRadix - 1);
         const uint32_t pos = tile_start[d] + cnt[w][d] + rank[r];
-        s_keys[pos] = k[r];
-        s_vals[pos] = v[r];
+        s_buf[pos] = k[r];
+        rank[r] = pos;
     }
     __syncthreads();
     const int64_t valid = n - tile_base < kSortTile ? n - tile_base : kSortTile;
@@ -267,12 +268,21 @@ __global__ void __launch_bounds__(kSortBlock) onesweep_pass_kernel(const uint64_
     for (int r = 0; r < kSortItems; ++r) {
         const int j = r * kSortBlock + tid;
         if (j < valid) {
-            const uint64_t key = s_keys[j];
+            const uint64_t key = s_buf[j];
             const uint32_t d = static_cast<uint32_t>(key >> shift) & (kRadix - 1);
-            const uint32_t o = dest_base[d] + static_cast<uint32_t>(j);
-            keys_out[o] = key;
-            vals_out[o] = s_vals[j];
+            keys_out[dest_base[d] + static_cast<uint32_t>(j)] = key;
+            s_digit[j] = static_cast<uint8_t>(d);
         }
+    }
+    __syncthreads();
+    uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_buf);
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) s_vals[rank[r]] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int j = r * kSortBlock + tid;
+        if (j < valid) vals_out[dest_base[s_digit[j]] + static_cast<uint32_t>(j)] = s_vals[j];
     }
 }
 
