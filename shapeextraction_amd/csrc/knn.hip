@@ -242,11 +242,14 @@ int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts
 
 // ---------------------------------------------------------------------------
 // Wave-cooperative tiles.  A wave owns 64 consecutive queries (Morton
-// neighbours, so spatially compact).  Each lane gets a squared radius r2:
-//   plain:   the k-th best squared distance among the index points within
-//            +-32 positions of the tile (a true upper bound of its k-th
-//            neighbour distance); those points seed the lane's list;
-//   learned: the predicted r2 (any value; checked at the end).
+// neighbours, so spatially compact).  Each lane's list is seeded with the
+// index points within +-kWindow<K> positions of the tile, and its squared
+// radius r2 is:
+//   plain:   the k-th best squared distance in that window (a true upper
+//            bound of its k-th neighbour distance);
+//   learned: the same, capped by kLearnedCap x the predicted r2 where that is
+//            smaller - the window bound is loose exactly where the Morton
+//            curve jumps, and a capped lane must prove its list at the end.
 // Lanes whose radius is at most twice the tile's mean join the tile search:
 // the wave walks the tree once, breadth first, for the union box E of their
 // balls, collecting the leaf ranges of every subtree whose box meets E.  E is
@@ -269,11 +272,10 @@ constexpr int kTileRanges = kRangeFlush - 1 + 2 * kWave;  // one expansion adds 
 // 32-neighbourhoods.
 template <int K>
 constexpr int kWindow = K >= 32 ? 96 : 32;
-constexpr int kSeed = 8;                                  // learned: each lane's own +-kSeed Morton neighbours
 constexpr int kSegBudget = 8192;                         // candidates one segment may stream before deferring
 constexpr float kSegGap = 8.0f;                          // segment break: gap > kSegGap x mean radius
 constexpr float kRadiusStretch = 2.25f;                  // learned: accept up to 1.5x the predicted radius
-constexpr float kLearnedPad = 1.1f;                      // learned: box E bounds balls of 1.1x the predicted radius
+constexpr float kLearnedCap = 1.21f;                     // learned: radius capped at 1.1x the predicted one
 
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
@@ -450,9 +452,9 @@ __device__ __forceinline__ int range_prefix(const int2* rng, int* pre, int nr) {
 // per step, uniform LDS reads) to every lane; a lane inserts a candidate when
 // on, its squared distance is <= tau and it beats the lane's k-th key.
 // cand[n] must hold a point at +inf when n is odd.
-template <int K, bool kLaneSkip>
+template <int K>
 __device__ __forceinline__ void broadcast_insert(const float4* cand, int n, float4 q, bool on, uint32_t tau,
-                                                 int skip_lo, int skip_hi, TopK<K>& top) {
+                                                 TopK<K>& top) {
     for (int i = 0; i < n; i += 2) {
         const float4 c0 = cand[i];
         const float4 c1 = cand[i + 1];
@@ -461,10 +463,8 @@ __device__ __forceinline__ void broadcast_insert(const float4* cand, int n, floa
         const int j0 = __float_as_int(c0.w), j1 = __float_as_int(c1.w);
         const uint64_t k0 = (static_cast<uint64_t>(d0) << 32) | static_cast<uint32_t>(j0);
         const uint64_t k1 = (static_cast<uint64_t>(d1) << 32) | static_cast<uint32_t>(j1);
-        const bool s0 = kLaneSkip && j0 >= skip_lo && j0 <= skip_hi;  // already in this lane's seed
-        const bool s1 = kLaneSkip && j1 >= skip_lo && j1 <= skip_hi;
-        if (on && !s0 && d0 <= tau && k0 < top.e[K - 1]) top.insert(k0);
-        if (on && !s1 && d1 <= tau && k1 < top.e[K - 1]) top.insert(k1);
+        if (on && d0 <= tau && k0 < top.e[K - 1]) top.insert(k0);
+        if (on && d1 <= tau && k1 < top.e[K - 1]) top.insert(k1);
     }
 }
 
@@ -504,13 +504,13 @@ __device__ __forceinline__ int stage_candidates(float4* cand, float4 p, int j, b
 
 // Stream the leaf ranges rng[0, nr): 64 candidates per load (one per lane,
 // coalesced); those inside E are staged and broadcast.  Indices in
-// [skip_lo, skip_hi] were seen by every lane already and are not staged;
-// with kLaneSkip, indices in the lane's [lane_lo, lane_hi] are ignored by it.
-template <int K, bool kLaneSkip>
+// [skip_lo, skip_hi] (the window) were seen by every lane already and are
+// not staged.
+template <int K>
 __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, const int2* rng, int* pre, int nr,
-                                              int skip_lo, int skip_hi, int lane_lo, int lane_hi, float ex0,
-                                              float ey0, float ez0, float ex1, float ey1, float ez1, float4 q,
-                                              bool on, uint32_t tau, TopK<K>& top, int& seen, float4* cand) {
+                                              int skip_lo, int skip_hi, float ex0, float ey0, float ez0, float ex1,
+                                              float ey1, float ez1, float4 q, bool on, uint32_t tau, TopK<K>& top,
+                                              int& seen, float4* cand) {
     const int lane = static_cast<int>(lane_id());
     const int total = range_prefix(rng, pre, nr);
     seen += total;
@@ -522,7 +522,7 @@ __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, co
         if (j >= 0) p = pts[j];
         const bool in_e = j >= 0 && p.x >= ex0 && p.x <= ex1 && p.y >= ey0 && p.y <= ey1 && p.z >= ez0 && p.z <= ez1;
         const int n = stage_candidates(cand, p, j, in_e);
-        broadcast_insert<K, kLaneSkip>(cand, n, q, on, tau, lane_lo, lane_hi, top);
+        broadcast_insert<K>(cand, n, q, on, tau, top);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
 }
@@ -543,7 +543,6 @@ knn_tile_kernel(const float4* __restrict__ pts,
     __shared__ int2 s_range[kTileWaves][kTileRanges];
     __shared__ int s_pre[kTileWaves][kTileRanges];
     __shared__ float4 s_cand[kTileWaves][kWave + 1];
-    __shared__ float4 s_seed[kTileWaves][kLearned ? kWave + 2 * kSeed : 1];
     const int w = static_cast<int>(threadIdx.x) / kWave;
     const int lane = static_cast<int>(lane_id());
     const int total = d_counts[0];
@@ -563,10 +562,9 @@ knn_tile_kernel(const float4* __restrict__ pts,
     TopK<K> top;
     top.init();
 
-    // ---- per-lane radius
-    int wlo = 0, whi = -1, seed_lo = 0, seed_hi = -1;
-    uint32_t r2bits;
-    if (!kLearned) {
+    // ---- per-lane radius: the window
+    int wlo, whi;
+    {
         constexpr int hw = kWindow<K>;
         wlo = static_cast<int>(tile0 - hw > cbase ? tile0 - hw : cbase);
         whi = static_cast<int>(tile0 + kWave - 1 + hw < cend - 1 ? tile0 + kWave - 1 + hw : cend - 1);
@@ -584,36 +582,20 @@ knn_tile_kernel(const float4* __restrict__ pts,
             } else {
                 const float4 p = j <= whi ? pts[j] : make_float4(0.f, 0.f, 0.f, 0.f);
                 const int n = stage_candidates(s_cand[w], p, j, j <= whi);
-                broadcast_insert<K, false>(s_cand[w], n, q, true, 0xFFFFFFFFu, 0, -1, top);
+                broadcast_insert<K>(s_cand[w], n, q, true, 0xFFFFFFFFu, top);
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
-        r2bits = top.kth_bits();  // >= 0x7F800000 when the window holds fewer than K points
-    } else {
-        const float r2 = valid ? radius2[qi - q_start] : -1.0f;
-        r2bits = (r2 >= 0.0f && r2 < INFINITY) ? __float_as_uint(r2) : 0xFFFFFFFFu;
-        // seed every lane with its own Morton neighbours: all lanes insert at
-        // every step (no wave-wide union of insert events), filling the lists
-        // before the stream, which then only inserts improvements
-        seed_lo = static_cast<int>(qi) - kSeed;
-        seed_hi = static_cast<int>(qi) + kSeed;
-        // the tile's points +-kSeed, staged in LDS by two coalesced loads
-        float4* win = s_seed[w];
-        const int64_t w0 = tile0 - kSeed;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int idx = lane + h * kWave;
-            const int64_t jj = w0 + idx;
-            if (idx < kWave + 2 * kSeed && jj >= cbase && jj < cend) win[idx] = pts[jj];
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-#pragma unroll 1
-        for (int o = -kSeed; o <= kSeed; ++o) {
-            const int jj = static_cast<int>(qi) + o;
-            if (valid && jj >= cbase && jj < cend) {
-                const float4 p = win[lane + kSeed + o];
-                const uint64_t key = make_key(dist2(q.x, q.y, q.z, p.x, p.y, p.z), jj);
-                if (key < top.e[K - 1]) top.insert(key);
+    }
+    uint32_t r2bits = top.kth_bits();  // >= 0x7F800000 when the window holds fewer than K points
+    bool capped = false;
+    if (kLearned && valid) {
+        const float pr2 = radius2[qi - q_start];
+        if (pr2 >= 0.0f && pr2 < INFINITY) {
+            const uint32_t c = __float_as_uint(pr2 * kLearnedCap);
+            if (c < r2bits) {
+                r2bits = c;
+                capped = true;
             }
         }
     }
@@ -624,9 +606,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
     const bool join = finite && r <= 2.0f * mean_r;
     // padded radius: fl(rp * rp) > r2; each box bound is stepped one ulp
     // outward so it bounds the exact q -+ rp
-    // (learned: the box is grown by kLearnedPad so that lanes whose radius
-    // was predicted slightly short can still be proven complete inside it)
-    const float rp = r * (kLearned ? 1.001f * kLearnedPad : 1.001f) + 1e-19f;
+    const float rp = r * 1.001f + 1e-19f;
 
     // ---- segments: runs of joining lanes without a spatial gap larger than
     // kSegGap mean radii between consecutive ones, so that a tile straddling
@@ -661,7 +641,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
         uint32_t tau = 0u;
         if (in_seg) {
             tau = r2bits;
-            if (kLearned) {
+            if (kLearned && capped) {
                 // Every point with dist2 <= rho2 lies inside E: the gap to the
                 // nearest face is shrunk by 2^-20 twice, far more than the
                 // rounding of the gap, the square and dist2 itself.  A lane may
@@ -671,7 +651,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
                 g = fmaxf(g, 0.0f) * (1.0f - 0x1p-20f);
                 const float rho2 = (g * g) * (1.0f - 0x1p-20f);
                 const uint32_t t = __float_as_uint(fminf(rho2, __uint_as_float(r2bits) * kRadiusStretch));
-                if (t > tau) tau = t;  // the predicted ball itself lies inside E by construction
+                if (t > tau) tau = t;  // the capped ball itself lies inside E by construction
             }
             lim = tau;
         }
@@ -728,7 +708,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
             if (overflow || nr == 0) break;
-            stream_ranges<K, kLearned>(pts, rng, pre, nr, wlo, whi, seed_lo, seed_hi, ex0, ey0, ez0, ex1, ey1, ez1,
+            stream_ranges<K>(pts, rng, pre, nr, wlo, whi, ex0, ey0, ez0, ex1, ey1, ez1,
                                        q, in_seg, tau, top, seen, s_cand[w]);
             nr = 0;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -737,9 +717,9 @@ knn_tile_kernel(const float4* __restrict__ pts,
         }
         if (overflow) lost = lost || in_seg;  // the segment's lanes are deferred
     }
-    // plain: every point within the window radius was a candidate; learned:
-    // the k-th distance must lie within the radius the segment box proves
-    const bool exact = join && !lost && (kLearned ? top.kth_bits() <= lim : true);
+    // uncapped: every point within the window radius was a candidate;
+    // capped: the k-th distance must lie within the radius the box proves
+    const bool exact = join && !lost && (capped ? top.kth_bits() <= lim : true);
     if (valid && exact) write_row<K>(out + (qi - q_start) * K, top, cbase);
     const bool defer = valid && !exact;
     const uint64_t md = __ballot(defer);
