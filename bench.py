@@ -139,7 +139,6 @@ def main() -> int:
     log(f"U={U} octree nodes={O_nodes} r={O_nodes / U:.3f}")
     nn = sx.DeviceArray((U, k), np.int32)
     normals = sx.DeviceArray((U, 3), np.float32)
-    feats = sx.DeviceArray((U, 8), np.float32)
     radius2 = sx.DeviceArray((U,), np.float32)
 
     model = None
@@ -160,8 +159,7 @@ def main() -> int:
         sx._lib.call("sx_index_build", ix.handle, dx.ptr, dy.ptr, dz.ptr, n, st.handle)
         rec(1)
         if mode == "learned":
-            sx._lib.call("sx_radius_features", ix.handle, 0, U, feats.ptr, st.handle)
-            model.predict_radius2(feats, U, radius2, stream=st.handle)
+            model.predict_index_radius2(ix, 0, U, radius2, stream=st.handle)
         rec(2)
         sx._lib.call("sx_knn", ix.handle, 0, U, k, sx.SX_KNN_RADIUS if mode == "learned" else sx.SX_KNN_PLAIN,
                      radius2.ptr if mode == "learned" else None, nn.ptr, st.handle)

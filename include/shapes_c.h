@@ -296,6 +296,10 @@ int sx_mlp_adam_step(sx_mlp* mlp, const float* d_feats, const float* d_targets, 
 /* The 8 octree occupancy/depth features of index points [q_start, q_start+n_q)
  * into d_feats (n_q x 8 floats, 16-byte aligned). */
 int sx_radius_features(sx_index* index, int64_t q_start, int64_t n_q, float* d_feats, void* stream);
+/* Squared search radius exp(y + margin) of index points [q_start, q_start+n_q)
+ * into d_radius2: the features and the MLP in one kernel (the feature rows
+ * never reach HBM); equal to sx_radius_features + sx_mlp_predict_radius2. */
+int sx_radius_predict(sx_index* index, sx_mlp* mlp, int64_t q_start, int64_t n_q, float* d_radius2, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Batched clouds (no counterpart in the reference, which builds one cloud at a

@@ -159,6 +159,7 @@ SIGNATURES = {
     "sx_mlp_backward": (I, [P, P, P, I64, P, P, P]),
     "sx_mlp_adam_step": (I, [P, P, P, I64, F, F, P, P]),
     "sx_radius_features": (I, [P, I64, I64, P, P]),
+    "sx_radius_predict": (I, [P, P, I64, I64, P, P]),
     # batched clouds
     "sx_batch_create": (I, [C.POINTER(P), I64, I, I]),
     "sx_batch_destroy": (I, [P]),

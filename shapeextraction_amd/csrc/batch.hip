@@ -40,8 +40,8 @@ int sort_pairs_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t 
                       uint32_t* vals_out, void* ws, int algo, hipStream_t st);
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
-                  int32_t* node_end, int32_t* leaf_parent, TravNode* trav, const float4* pts, void* box_ws,
-                  hipStream_t st, CloudMap cm);
+                  int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav, const float4* pts,
+                  void* box_ws, hipStream_t st, CloudMap cm);
 size_t box_ws_bytes(int64_t n_max);
 int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                      int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
@@ -256,7 +256,7 @@ int batch_build(sx_batch* b, const float* x, const float* y, const float* z, hip
     const int32_t* slots = b->counts + sx::kBSlots;
     SX_TRY(sx::seg_decode_launch(b->ucodes, slots, b->slot_cap, b->map(), b->bounds, cb, b->pts16, st));
     SX_TRY(sx::karras_launch(b->ucodes, slots, b->slot_cap, cb, SX_ROOT_TEXTBOOK, b->pn, b->lc, nullptr, nullptr,
-                             b->parent, nullptr, nullptr, b->trav, b->pts16, b->ws, st, b->map()));
+                             b->parent, nullptr, nullptr, nullptr, b->trav, b->pts16, b->ws, st, b->map()));
     return SX_OK;
 }
 

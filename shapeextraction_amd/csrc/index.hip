@@ -34,14 +34,14 @@ int unique_launch(const uint64_t* sorted, const uint32_t* vals, int64_t n, uint6
                   int32_t* uos, int32_t* uoi, int32_t* d_count, void* ws, hipStream_t st);
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
-                  int32_t* node_end, int32_t* leaf_parent, TravNode* trav, const float4* pts, void* box_ws,
-                  hipStream_t st, CloudMap cm = CloudMap());
+                  int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav, const float4* pts,
+                  void* box_ws, hipStream_t st, CloudMap cm = CloudMap());
 size_t box_ws_bytes(int64_t n_max);
 size_t octree_ws_bytes(int64_t n_max);
 int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, const float* bounds,
                   const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
                   const int32_t* parent, sx_otnode* nodes, int64_t cap, int32_t* d_oct_count, int32_t* leaf_oct,
-                  void* ws, hipStream_t st);
+                  void* ws, hipStream_t st, const TravNode* trav = nullptr, int2* aux = nullptr);
 int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
                hipStream_t st, CloudMap cm = CloudMap());
@@ -86,7 +86,9 @@ struct sx_index {
     int32_t* lc = nullptr;
     int32_t* parent = nullptr;
     int32_t* leaf_parent = nullptr;
+    int2* up = nullptr;  // (parent, prefix length) per node of the traversal tree
     int32_t* leaf_oct = nullptr;
+    int2* oct_aux = nullptr;  // (parent octree node, points in the cell) per octree node
     int2* knn_queue = nullptr;  // deferred queries: (index, k-th distance bound bits)
     sx::TravNode* trav = nullptr;
     sx_otnode* oct = nullptr;
@@ -126,10 +128,12 @@ size_t index_layout(sx_index* ix, void* base) {
     ix->lc = c.take<int32_t>(n);
     ix->parent = c.take<int32_t>(n);
     ix->leaf_parent = c.take<int32_t>(n);
+    ix->up = c.take<int2>(n);
     ix->leaf_oct = c.take<int32_t>(n);
     ix->knn_queue = c.take<int2>(n);
     ix->trav = c.take<sx::TravNode>(n);
     ix->oct = c.take<sx_otnode>(static_cast<size_t>(ix->oct_cap));
+    ix->oct_aux = c.take<int2>(static_cast<size_t>(ix->oct_cap));
     ix->ws_bbox = c.take<char>(sx::bbox_ws_bytes());
     const size_t ws = std::max({sx::sort_ws_bytes(ix->cap), sx::scan_ws_bytes(ix->cap), sx::octree_ws_bytes(ix->cap),
                                 sx::box_ws_bytes(ix->cap)});
@@ -157,23 +161,18 @@ bool valid_k(int k) { return k == 8 || k == 16 || k == 24 || k == 32; }
 
 namespace sx {
 // Arrays the learned-radius feature kernel reads (mlp.hip).
-int index_feature_arrays(sx_index* ix, const TravNode** trav, const int32_t** leaf_parent, const uint64_t** codes,
-                         const int32_t** leaf_oct, const sx_otnode** oct, const float** bounds,
-                         const int32_t** counts, const uint8_t** pn, const int32_t** parent, int* code_bits,
-                         int64_t* limit) {
+int index_feature_arrays(sx_index* ix, const TravNode** trav, const int2** up, const int2** oct_aux,
+                         const int32_t** leaf_parent, const int32_t** leaf_oct, const sx_otnode** oct,
+                         const float** bounds, const int32_t** counts, int64_t* limit) {
     SX_TRY(check_index(ix));
-    // the compact arrays describe the traversal tree only with the textbook root
-    const bool textbook = ix->root_mode == SX_ROOT_TEXTBOOK;
-    *pn = textbook ? ix->pn : nullptr;
-    *parent = textbook ? ix->parent : nullptr;
     *trav = ix->trav;
+    *up = ix->up;
+    *oct_aux = ix->root_mode == SX_ROOT_TEXTBOOK ? ix->oct_aux : nullptr;
     *leaf_parent = ix->leaf_parent;
-    *codes = ix->ucodes;
     *leaf_oct = ix->leaf_oct;
     *oct = ix->oct;
     *bounds = ix->bounds;
     *counts = ix->counts;
-    *code_bits = ix->code_bits;
     *limit = ix->counts_valid ? ix->n_unique : ix->n_input;
     return SX_OK;
 }
@@ -266,16 +265,22 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
     SX_TRY(sx::decode_launch(ix->ucodes, ix->counts + kCntUnique, n, ix->bounds, cb, ix->pts12, ix->pts16, st));
     if (ix->root_mode == SX_ROOT_TEXTBOOK) {
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, ix->pn, ix->lc, ix->hll,
-                                 ix->hlr, ix->parent, nullptr, ix->leaf_parent, ix->trav, ix->pts16, ix->ws, st));
+                                 ix->hlr, ix->parent, nullptr, ix->leaf_parent, ix->up, ix->trav, ix->pts16, ix->ws,
+                                 st));
     } else {
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_REFERENCE, ix->pn, ix->lc,
-                                 ix->hll, ix->hlr, ix->parent, nullptr, nullptr, nullptr, nullptr, nullptr, st));
-        SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, nullptr, nullptr,
-                                 nullptr, nullptr, nullptr, nullptr, ix->leaf_parent, ix->trav, ix->pts16, ix->ws,
+                                 ix->hll, ix->hlr, ix->parent, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                                  st));
+        SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, nullptr, nullptr,
+                                 nullptr, nullptr, nullptr, nullptr, ix->leaf_parent, ix->up, ix->trav, ix->pts16,
+                                 ix->ws, st));
     }
+    // with the textbook root the octree and the traversal tree are one tree:
+    // record each octree node's parent and point count for the features
+    const bool textbook = ix->root_mode == SX_ROOT_TEXTBOOK;
     SX_TRY(sx::octree_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, ix->bounds, ix->pn, ix->lc, ix->hll, ix->hlr,
-                             ix->parent, ix->oct, ix->oct_cap, ix->counts + kCntOct, ix->leaf_oct, ix->ws, st));
+                             ix->parent, ix->oct, ix->oct_cap, ix->counts + kCntOct, ix->leaf_oct, ix->ws, st,
+                             textbook ? ix->trav : nullptr, textbook ? ix->oct_aux : nullptr));
     ix->n_input = n;
     SX_HIP(hipEventRecord(ix->built_event, sx::as_stream(stream)));
     ix->built = true;

@@ -47,11 +47,9 @@ constexpr int kNParams = kOffB3 + 1;          // 4801
 constexpr int kMlpBlock = 256;                // 4 waves, 64 queries per wave
 constexpr int kWavesPerBlock = kMlpBlock / 64;
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (values here are finite)
-    return static_cast<uint16_t>(u >> 16);
-}
+// Round to nearest even: v_cvt_pk_bf16_f32 on gfx950, the same bits as
+// adding 0x7FFF + the kept LSB and truncating, for the finite values here.
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f)); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
 __device__ __forceinline__ bf16x4_t pack4(float a, float b, float c, float d) {
@@ -401,22 +399,61 @@ __global__ void mlp_adam_kernel(float* __restrict__ p, float* __restrict__ mo, f
 }
 
 // ---------------------------------------------------------------- features
-// The walk up the radix tree reads the compact prefix-length (1 B) and parent
-// (4 B) arrays when they describe the traversal tree (textbook root), else
-// the traversal records and the codes.
+// Cell counts come from the octree when it is the traversal tree's (textbook
+// root): each octree node's (parent, point count), one dependent load per
+// level.  Otherwise the walk climbs the traversal tree, one int2 (parent,
+// prefix length) per radix node, about three steps per octree level.
 struct FeatTree {
     const TravNode* trav;
-    const uint64_t* codes;
-    const uint8_t* pn;      // nullable
-    const int32_t* parent;  // nullable
-    int code_bits;
-    __device__ __forceinline__ int prefix(int node) const {
-        if (pn) return pn[node];
-        const float4 lo_l = trav[node].lo_l, lo_r = trav[node].lo_r;
-        return prefix_len(codes[w_int(lo_l.w)], codes[w_int(lo_r.w)], code_bits);
-    }
-    __device__ __forceinline__ int up(int node) const { return parent ? parent[node] : w_int(trav[node].hi_r.w); }
+    const int2* up;
+    const int2* aux;  // nullable
 };
+
+// Features of index point u (U >= 2 points in the index).
+__device__ __forceinline__ void query_features(const FeatTree& tr, const int32_t* __restrict__ leaf_parent,
+                                               const int32_t* __restrict__ leaf_oct,
+                                               const sx_otnode* __restrict__ oct, float log2_range, int root_level,
+                                               int64_t u, float (&f)[kIn]) {
+    int c = leaf_parent[u] >> 1;
+    const int2 uc = tr.up[c];
+    const int d0 = uc.y / 3;
+    const int on = leaf_oct[u];
+    if (tr.aux && on >= 0) {
+        // the octree node holding u is the cell at depth d0; its ancestors are
+        // the cells at d0-1, d0-2, ... up to the root, which repeats
+        int o = on;
+        int2 a = tr.aux[o];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            f[s] = __log2f(static_cast<float>(a.y));
+            if (s < 3 && a.x >= 0) {
+                o = a.x;
+                a = tr.aux[o];
+            }
+        }
+    } else {
+        int pc = uc.x;
+        int2 upc = pc >= 0 ? tr.up[pc] : make_int2(-1, -1);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int L = max(d0 - s, root_level);
+            // highest ancestor whose prefix still holds 3L bits
+            while (pc >= 0 && upc.y >= 3 * L) {
+                c = pc;
+                pc = upc.x;
+                upc = pc >= 0 ? tr.up[pc] : make_int2(-1, -1);
+            }
+            const int cnt = w_int(tr.trav[c].lo_r.w) - w_int(tr.trav[c].lo_l.w) + 1;
+            f[s] = __log2f(static_cast<float>(cnt));
+        }
+    }
+    f[4] = log2_range - static_cast<float>(d0);
+    f[5] = static_cast<float>(d0 - root_level);
+    if (on >= 0) {
+        f[6] = static_cast<float>(__popc(oct[on].child_leaf_mask));
+        f[7] = static_cast<float>(__popc(oct[on].child_node_mask));
+    }
+}
 
 __global__ void __launch_bounds__(256) features_kernel(FeatTree tr, const int32_t* __restrict__ leaf_parent,
                                                        const int32_t* __restrict__ leaf_oct,
@@ -429,35 +466,126 @@ __global__ void __launch_bounds__(256) features_kernel(FeatTree tr, const int32_
     const int U = d_counts[0];
     const int64_t u = q_start + t;
     float f[kIn] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (u < U && U >= 2) {
-        const int root_level = tr.prefix(0) / 3;
-        int c = leaf_parent[u] >> 1;
-        const int d0 = tr.prefix(c) / 3;
-        int pc = tr.up(c);
-        int ppn = pc >= 0 ? tr.prefix(pc) : -1;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int L = max(d0 - s, root_level);
-            // highest ancestor whose prefix still holds 3L bits
-            while (pc >= 0 && ppn >= 3 * L) {
-                c = pc;
-                pc = tr.up(c);
-                ppn = pc >= 0 ? tr.prefix(pc) : -1;
-            }
-            const int cnt = w_int(tr.trav[c].lo_r.w) - w_int(tr.trav[c].lo_l.w) + 1;
-            f[s] = __log2f(static_cast<float>(cnt));
-        }
-        f[4] = __log2f(bounds[2]) - static_cast<float>(d0);
-        f[5] = static_cast<float>(d0 - root_level);
-        const int on = leaf_oct[u];
-        if (on >= 0) {
-            f[6] = static_cast<float>(__popc(oct[on].child_leaf_mask));
-            f[7] = static_cast<float>(__popc(oct[on].child_node_mask));
-        }
-    }
+    if (u < U && U >= 2) query_features(tr, leaf_parent, leaf_oct, oct, __log2f(bounds[2]), tr.up[0].y / 3, u, f);
     float4* out = reinterpret_cast<float4*>(feats + t * kIn);
     out[0] = make_float4(f[0], f[1], f[2], f[3]);
     out[1] = make_float4(f[4], f[5], f[6], f[7]);
+}
+
+// Features and the MLP in one pass: each wave computes the features of its
+// 64 queries (one per lane), stages them in LDS, and runs the four 16-query
+// tiles on MFMA with the weights read from LDS (copied once per block), so
+// the walk up the tree keeps the occupancy of the plain feature kernel;
+// r2 = exp(y + margin).  The same arithmetic as features_kernel followed by
+// mlp_forward_kernel (mode 1), without the 32-byte feature rows in HBM.
+// Rows padded by 4 BF16 (8 bytes): the 16 lanes of an A-operand read hit
+// distinct banks (unpadded, the 128-byte W2 rows would put all 16 on one).
+constexpr int kW1Row = kInPad + 4, kW2Row = kH + 4;
+struct MlpLds {
+    uint16_t w1p[kH * kW1Row];
+    uint16_t w2[kH * kW2Row];
+    uint16_t w3[kH];
+    float b1[kH], b2[kH], b3;
+    float mu[kIn], is[kIn], margin;
+};
+
+__device__ __forceinline__ bf16x4_t lds4(const uint16_t* p) {
+    const uint2 w = *reinterpret_cast<const uint2*>(p);
+    bf16x4_t v;
+    v[0] = static_cast<short>(w.x & 0xFFFF);
+    v[1] = static_cast<short>(w.x >> 16);
+    v[2] = static_cast<short>(w.y & 0xFFFF);
+    v[3] = static_cast<short>(w.y >> 16);
+    return v;
+}
+
+__global__ void __launch_bounds__(kMlpBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) radius_kernel(FeatTree tr, const int32_t* __restrict__ leaf_parent,
+                                                           const int32_t* __restrict__ leaf_oct,
+                                                           const sx_otnode* __restrict__ oct,
+                                                           const float* __restrict__ bounds,
+                                                           const int32_t* __restrict__ d_counts, int64_t q_start,
+                                                           int64_t n_q, MlpDev m, float* __restrict__ r2) {
+    __shared__ MlpLds s_w;
+    __shared__ float s_f[kWavesPerBlock][kWave][kIn + 1];  // +1: no bank conflicts on the column reads
+    const int tid = static_cast<int>(threadIdx.x);
+    for (int i = tid; i < kH * kInPad / 4; i += kMlpBlock) {
+        const int row = i / (kInPad / 4), c4 = i % (kInPad / 4);
+        *reinterpret_cast<uint2*>(s_w.w1p + row * kW1Row + 4 * c4) = reinterpret_cast<const uint2*>(m.w1p)[i];
+    }
+    for (int i = tid; i < kH * kH / 4; i += kMlpBlock) {
+        const int row = i / (kH / 4), c4 = i % (kH / 4);
+        *reinterpret_cast<uint2*>(s_w.w2 + row * kW2Row + 4 * c4) = reinterpret_cast<const uint2*>(m.w2)[i];
+    }
+    if (tid < kH) {
+        s_w.w3[tid] = m.w3[tid];
+        s_w.b1[tid] = m.p[kOffB1 + tid];
+        s_w.b2[tid] = m.p[kOffB2 + tid];
+    }
+    if (tid < kIn) {
+        s_w.mu[tid] = m.norm[tid];
+        s_w.is[tid] = m.norm[kIn + tid];
+    }
+    if (tid == 0) {
+        s_w.b3 = m.p[kOffB3];
+        s_w.margin = m.norm[2 * kIn];
+    }
+    __syncthreads();
+    const int w = tid / kWave;
+    const int l = static_cast<int>(lane_id());
+    const int U = d_counts[0];
+    const float log2_range = __log2f(bounds[2]);
+    const int root_level = U >= 2 ? tr.up[0].y / 3 : 0;
+    const int r = l % 16, g = l / 16;
+    const bf16x4_t zero4 = {0, 0, 0, 0};
+    // persistent: each wave walks groups of 64 queries (no block barriers below)
+    const int64_t groups = (n_q + kWave - 1) / kWave;
+    for (int64_t grp = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w; grp < groups;
+         grp += static_cast<int64_t>(gridDim.x) * kWavesPerBlock) {
+        const int64_t q0 = grp * kWave;
+        {
+            const int64_t u = q_start + q0 + l;
+            float f[kIn] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (q0 + l < n_q && u < U && U >= 2)
+                query_features(tr, leaf_parent, leaf_oct, oct, log2_range, root_level, u, f);
+#pragma unroll
+            for (int i = 0; i < kIn; ++i) s_f[w][l][i] = f[i];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll 1
+        for (int t = 0; t < kWave / 16; ++t) {
+            // the weights are re-read from LDS per tile rather than hoisted into
+            // registers (which would halve the occupancy of the walk above)
+            int zo = 0;
+            asm volatile("" : "+v"(zo));
+            const MlpLds& sw = *reinterpret_cast<const MlpLds*>(reinterpret_cast<const char*>(&s_w) + zo);
+            const int qi = 16 * t + r;
+            bf16x4_t xb = zero4;
+            if (g < 2 && q0 + qi < n_q) {
+                const float* v = s_f[w][qi] + 4 * g;
+                xb = pack4((v[0] - sw.mu[4 * g]) * sw.is[4 * g], (v[1] - sw.mu[4 * g + 1]) * sw.is[4 * g + 1],
+                           (v[2] - sw.mu[4 * g + 2]) * sw.is[4 * g + 2], (v[3] - sw.mu[4 * g + 3]) * sw.is[4 * g + 3]);
+            }
+            bf16x4_t h1[4], h2[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                const f32x4_t b = *reinterpret_cast<const f32x4_t*>(sw.b1 + 16 * mt + 4 * g);
+                h1[mt] = relu_pack(mfma(lds4(sw.w1p + (16 * mt + r) * kW1Row + 4 * g), xb, b));
+            }
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                f32x4_t c = *reinterpret_cast<const f32x4_t*>(sw.b2 + 16 * mt + 4 * g);
+#pragma unroll
+                for (int kc = 0; kc < 4; ++kc)
+                    c = mfma(lds4(sw.w2 + (16 * mt + r) * kW2Row + 16 * kc + 4 * g), h1[kc], c);
+                h2[mt] = relu_pack(c);
+            }
+            f32x4_t c = {g == 0 ? sw.b3 : 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) c = mfma(r == 0 ? lds4(sw.w3 + 16 * kc + 4 * g) : zero4, h2[kc], c);
+            if (l < 16 && q0 + 16 * t + l < n_q) r2[q0 + 16 * t + l] = __expf(c[0] + sw.margin);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // s_f is refilled by the next group
+    }
 }
 
 }  // namespace sx
@@ -511,10 +639,9 @@ int check_mlp(const sx_mlp* m) { return m ? SX_OK : sx::fail(SX_EINVAL, "null ml
 // sx_index internals needed by the feature kernel
 struct sx_index;
 namespace sx {
-int index_feature_arrays(sx_index* ix, const TravNode** trav, const int32_t** leaf_parent, const uint64_t** codes,
-                         const int32_t** leaf_oct, const sx_otnode** oct, const float** bounds,
-                         const int32_t** counts, const uint8_t** pn, const int32_t** parent, int* code_bits,
-                         int64_t* limit);
+int index_feature_arrays(sx_index* ix, const TravNode** trav, const int2** up, const int2** oct_aux,
+                         const int32_t** leaf_parent, const int32_t** leaf_oct, const sx_otnode** oct,
+                         const float** bounds, const int32_t** counts, int64_t* limit);
 }
 
 using sx::fail;
@@ -672,22 +799,42 @@ int sx_mlp_adam_step(sx_mlp* m, const float* d_feats, const float* d_targets, in
 // Features of index points [q_start, q_start + n_q) into d_feats (n_q x 8 floats).
 int sx_radius_features(sx_index* index, int64_t q_start, int64_t n_q, float* d_feats, void* stream) {
     const sx::TravNode* trav;
-    const int32_t *lp, *lo, *counts, *parent;
-    const uint64_t* codes;
-    const uint8_t* pn;
+    const int32_t *lp, *lo, *counts;
+    const int2 *up, *aux;
     const sx_otnode* oct;
     const float* bounds;
-    int cb;
     int64_t limit;
-    SX_TRY(sx::index_feature_arrays(index, &trav, &lp, &codes, &lo, &oct, &bounds, &counts, &pn, &parent, &cb,
-                                    &limit));
+    SX_TRY(sx::index_feature_arrays(index, &trav, &up, &aux, &lp, &lo, &oct, &bounds, &counts, &limit));
     if (q_start < 0 || n_q < 0 || q_start + n_q > limit) return fail(SX_EINVAL, "sx_radius_features: bad query range");
     if (n_q == 0) return SX_OK;
     if (!d_feats || reinterpret_cast<uintptr_t>(d_feats) % 16 != 0)
         return fail(SX_EINVAL, "sx_radius_features: output must be 16-byte aligned");
     sx::features_kernel<<<sx::grid_for(n_q, 256), 256, 0, sx::as_stream(stream)>>>(
-        sx::FeatTree{trav, codes, pn, parent, cb}, lp, lo, oct, bounds, counts, q_start, n_q, d_feats);
+        sx::FeatTree{trav, up, aux}, lp, lo, oct, bounds, counts, q_start, n_q, d_feats);
     SX_LAUNCH_CHECK("features_kernel");
+    return SX_OK;
+}
+
+// Squared search radius of index points [q_start, q_start + n_q): features and
+// MLP in one kernel, equal to sx_radius_features + sx_mlp_predict_radius2.
+int sx_radius_predict(sx_index* index, sx_mlp* mlp, int64_t q_start, int64_t n_q, float* d_radius2, void* stream) {
+    SX_TRY(check_mlp(mlp));
+    const sx::TravNode* trav;
+    const int32_t *lp, *lo, *counts;
+    const int2 *up, *aux;
+    const sx_otnode* oct;
+    const float* bounds;
+    int64_t limit;
+    SX_TRY(sx::index_feature_arrays(index, &trav, &up, &aux, &lp, &lo, &oct, &bounds, &counts, &limit));
+    if (q_start < 0 || n_q < 0 || q_start + n_q > limit) return fail(SX_EINVAL, "sx_radius_predict: bad query range");
+    if (n_q == 0) return SX_OK;
+    if (!d_radius2) return fail(SX_EINVAL, "sx_radius_predict: null output");
+    const int64_t waves = (n_q + sx::kWave - 1) / sx::kWave;
+    int64_t blocks = (waves + sx::kWavesPerBlock - 1) / sx::kWavesPerBlock;
+    if (blocks > 2048) blocks = 2048;  // persistent: the weights are copied to LDS once per block
+    sx::radius_kernel<<<static_cast<unsigned>(blocks), sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
+        sx::FeatTree{trav, up, aux}, lp, lo, oct, bounds, counts, q_start, n_q, dev_view(mlp), d_radius2);
+    SX_LAUNCH_CHECK("radius_kernel");
     return SX_OK;
 }
 

@@ -86,7 +86,8 @@ __global__ void __launch_bounds__(256) oct_gather_kernel(const uint64_t* __restr
                                                          const uint8_t* __restrict__ hlr,
                                                          const int32_t* __restrict__ offsets,
                                                          const int32_t* __restrict__ counts, sx_otnode* nodes,
-                                                         int64_t cap, int32_t* leaf_oct) {
+                                                         int64_t cap, int32_t* leaf_oct,
+                                                         const TravNode* __restrict__ trav, int2* aux) {
     const int64_t U = scan_count(d_count, n_max);
     const int64_t M = U - 1;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -99,6 +100,18 @@ __global__ void __launch_bounds__(256) oct_gather_kernel(const uint64_t* __restr
     const int D = pn[i] / 3;
     const int base = offsets[i];
     const uint64_t ci = codes[i];
+    if (aux) {
+        // (parent octree node, points in the cell) of radix node i's chain:
+        // each chain node's parent is the next one up; the top's parent is
+        // written by the ancestor that gathers it (below), the root's is -1
+        const int sz = w_int(trav[i].lo_r.w) - w_int(trav[i].lo_l.w) + 1;
+        for (int j = 0; j < c; ++j) {
+            if (base + j >= cap) break;
+            if (j + 1 < c) aux[base + j] = make_int2(base + j + 1, sz);
+            else if (i == 0) aux[base + j] = make_int2(-1, sz);
+            else aux[base + j].y = sz;
+        }
+    }
     int32_t ch[8];
     // chain nodes above the bottom
     for (int j = 1; j < c; ++j) {
@@ -130,8 +143,10 @@ __global__ void __launch_bounds__(256) oct_gather_kernel(const uint64_t* __restr
                 if (leaf_oct) leaf_oct[child] = base;
             } else if (counts[child] > 0) {
                 const int slot = static_cast<int>(code_prefix(codes[child], D + 1, code_bits) & 7);
-                ch[slot] = offsets[child] + counts[child] - 1;
+                const int top = offsets[child] + counts[child] - 1;
+                ch[slot] = top;
                 node_mask |= 1 << slot;
+                if (aux && top < cap) aux[top].x = base;
             } else if (sp < 8) {
                 stack[sp++] = child;
             }
@@ -163,7 +178,7 @@ size_t octree_ws_bytes(int64_t n_max) {
 int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, const float* bounds,
                   const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
                   const int32_t* parent, sx_otnode* nodes, int64_t cap, int32_t* d_oct_count, int32_t* leaf_oct,
-                  void* ws, hipStream_t st) {
+                  void* ws, hipStream_t st, const TravNode* trav = nullptr, int2* aux = nullptr) {
     if (n_max <= 0) return SX_OK;
     Carver c(ws);
     int32_t* offsets = c.take<int32_t>(static_cast<size_t>(n_max));
@@ -172,7 +187,8 @@ int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
     SX_TRY(device_scan(OctCountIn{pn, parent, d_count}, OctCountOut{offsets, counts}, d_count, n_max, d_oct_count,
                        scan_ws, st));
     oct_gather_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(codes, d_count, n_max, code_bits, bounds, pn, left_child,
-                                                            hll, hlr, offsets, counts, nodes, cap, leaf_oct);
+                                                            hll, hlr, offsets, counts, nodes, cap, leaf_oct, trav,
+                                                            trav ? aux : nullptr);
     SX_LAUNCH_CHECK("oct_gather_kernel");
     return SX_OK;
 }

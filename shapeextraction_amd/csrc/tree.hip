@@ -212,8 +212,8 @@ constexpr int kKarrasHalo = 256;
 __global__ void __launch_bounds__(kKarrasBlock)
 karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
               uint8_t* prefix_n, int32_t* left_child, uint8_t* has_leaf_left, uint8_t* has_leaf_right,
-              int32_t* parent, int32_t* node_end, int32_t* leaf_parent, TravNode* trav, const float4* pts,
-              BoxLevels lv, CloudMap cm) {
+              int32_t* parent, int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav,
+              const float4* pts, BoxLevels lv, CloudMap cm) {
     __shared__ uint64_t s_codes[kKarrasBlock + 2 * kKarrasHalo];
     // g: slot; the cloud holding it spans slots [base, base + U) and its
     // internal node i (local) is stored at slot base + i
@@ -231,6 +231,7 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
     const int64_t N = U - 1;  // internal nodes; leaves are 0..N
     const int64_t i = g - base;
     if (i == 0 && parent) parent[g] = -1;
+    if (i == 0 && up) up[g].x = -1;
     if (i >= N) return;
     // code of the cloud's leaf x (local index)
     auto code = [&](int64_t x) -> uint64_t {
@@ -291,6 +292,11 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
         if (hl) leaf_parent[G] = static_cast<int32_t>(2 * g);
         if (hr) leaf_parent[G + 1] = static_cast<int32_t>(2 * g + 1);
     }
+    if (up) {  // .x (parent) of a node is written by its parent's thread, .y here
+        up[g].y = dnode;
+        if (!hl) up[G].x = static_cast<int32_t>(g);
+        if (!hr) up[G + 1].x = static_cast<int32_t>(g);
+    }
     if (trav) {
         // the record of node g: child boxes with (first leaf, split, last
         // leaf) in .w, written whole; .w of hi_r (the parent) is written by
@@ -329,8 +335,8 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
 // (box_ws_bytes(n_max)) are required: the block-box levels are built first.
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
-                  int32_t* node_end, int32_t* leaf_parent, TravNode* trav, const float4* pts, void* box_ws,
-                  hipStream_t st, CloudMap cm = CloudMap()) {
+                  int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav, const float4* pts,
+                  void* box_ws, hipStream_t st, CloudMap cm = CloudMap()) {
     if (n_max <= 0) return SX_OK;
     BoxLevels lv{};
     if (trav) {
@@ -341,8 +347,8 @@ int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
     // rule) keep parent -1.
     if (parent) SX_TRY(fill_i32(parent, -1, n_max, st));
     karras_kernel<<<grid_for(n_max, kKarrasBlock), kKarrasBlock, 0, st>>>(
-        codes, d_count, n_max, code_bits, root_mode, prefix_n, left_child, hll, hlr, parent, node_end, leaf_parent, trav,
-        pts, lv, cm);
+        codes, d_count, n_max, code_bits, root_mode, prefix_n, left_child, hll, hlr, parent, node_end, leaf_parent, up,
+        trav, pts, lv, cm);
     SX_LAUNCH_CHECK("karras_kernel");
     return SX_OK;
 }
@@ -378,7 +384,7 @@ int sx_build_radix_tree(const uint64_t* d_codes, const int32_t* d_count, int64_t
     if (n_max <= 0 || !d_codes) return fail(SX_EINVAL, "sx_build_radix_tree: bad arguments");
     return sx::karras_launch(d_codes, d_count, n_max, code_bits, root_mode, d_prefix_n, d_left_child, d_has_leaf_left,
                              d_has_leaf_right, d_parent, d_node_end, d_leaf_parent, nullptr, nullptr, nullptr,
-                             sx::as_stream(stream));
+                             nullptr, sx::as_stream(stream));
 }
 
 }  // extern "C"

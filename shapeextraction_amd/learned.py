@@ -92,6 +92,14 @@ class RadiusModel:
         call("sx_mlp_predict_radius2", self.handle, feats.ptr, n, out.ptr, stream)
         return out
 
+    def predict_index_radius2(self, index, q_start: int = 0, n_q: int | None = None,
+                              out: DeviceArray | None = None, stream=None) -> DeviceArray:
+        """Features and MLP in one kernel for index points [q_start, q_start + n_q)."""
+        n_q = index.n_unique - q_start if n_q is None else n_q
+        out = out if out is not None else DeviceArray((n_q,), np.float32)
+        call("sx_radius_predict", index.handle, self.handle, q_start, n_q, out.ptr, stream)
+        return out
+
     def backward(self, feats: DeviceArray, targets: DeviceArray, n: int | None = None):
         n = len(feats) if n is None else n
         g = DeviceArray((self.n_params,), np.float32)

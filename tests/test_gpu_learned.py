@@ -99,6 +99,81 @@ def test_features_are_finite_and_consistent(gpu, L):
     assert len(f) == U
 
 
+def _features_by_definition(f, bits):
+    """The eight features from their definition, on the fetched (oracle-pinned)
+    tree and octree: point counts of the Morton cells at depths D0..D0-3
+    holding each point (D0 = prefix length of its radix parent / 3, clamped at
+    the root's depth), log2 edge, depth below the root, and the leaf / child
+    slots of the octree node holding the point."""
+    codes = f["codes"]
+    U = len(codes)
+    M = U - 1
+    pn = f["prefix_n"].astype(np.int64)
+    lc = f["left_child"].astype(np.int64)[:M]
+    hll = f["has_leaf_left"][:M] == 1
+    hlr = f["has_leaf_right"][:M] == 1
+    lp = np.empty(U, np.int64)
+    i = np.arange(M)
+    lp[lc[hll]] = i[hll]
+    lp[lc[hlr] + 1] = i[hlr]
+    d0 = pn[lp] // 3
+    root = pn[0] // 3
+    out = np.zeros((U, 8), np.float64)
+    for s in range(4):
+        L = np.maximum(d0 - s, root)
+        cnt = np.zeros(U)
+        for lv in np.unique(L):
+            sel = L == lv
+            pv = codes >> np.uint64(bits - 3 * int(lv))
+            cnt[sel] = np.searchsorted(pv, pv[sel], "right") - np.searchsorted(pv, pv[sel], "left")
+        out[:, s] = np.log2(cnt)
+    out[:, 4] = np.log2(np.float64(f["bounds"][2])) - d0
+    out[:, 5] = d0 - root
+    oct = f["oct_nodes"]
+    holder = np.full(U, -1, np.int64)
+    for t in range(8):
+        has = (oct["child_leaf_mask"] >> t) & 1 == 1
+        holder[oct["children"][has, t]] = np.nonzero(has)[0]
+    pop = np.vectorize(lambda v: bin(int(v)).count("1"))
+    out[:, 6] = pop(oct["child_leaf_mask"][holder])
+    out[:, 7] = pop(oct["child_node_mask"][holder])
+    return out
+
+
+@pytest.mark.parametrize("root_mode", [0, 1])
+def test_features_match_definition(gpu, L, root_mode):
+    """Features against their definition.  With the textbook root the kernel
+    reads cell counts from the octree's parent links; with the reference root
+    it climbs the traversal tree (f6, f7 then come from the reference octree,
+    whose root differs, so only f0..f5 are compared there)."""
+    x, y, z = gpu.gen_scene(2, 200_000)
+    ix = gpu.Index(len(x), 63, root_mode=root_mode).build(x, y, z)
+    got = L.features(ix).numpy().astype(np.float64)
+    want = _features_by_definition(ix.fetch() if root_mode == 0 else _textbook_fetch(gpu, x, y, z), 63)
+    assert np.allclose(got[:, :5], want[:, :5], rtol=0, atol=1e-5)
+    assert np.array_equal(got[:, 5], want[:, 5])
+    if root_mode == 0:
+        assert np.array_equal(got[:, 6:], want[:, 6:])
+
+
+def _textbook_fetch(gpu, x, y, z):
+    return gpu.Index(len(x), 63).build(x, y, z).fetch()
+
+
+def test_radius_predict_fused_equals_two_steps(gpu, L):
+    """sx_radius_predict (features + MLP in one kernel) returns the same bits
+    as sx_radius_features followed by sx_mlp_predict_radius2, also on a slice
+    whose length is not a multiple of the 64-query wave."""
+    model = L.RadiusModel.load(L.default_model_path(16))
+    x, y, z = gpu.gen_scene(7, 500_000)
+    ix = gpu.Index(len(x), 63).build(x, y, z)
+    two = model.predict_radius2(L.features(ix)).numpy()
+    fused = model.predict_index_radius2(ix).numpy()
+    assert np.array_equal(fused, two)
+    part = model.predict_index_radius2(ix, 777, 12_345).numpy()
+    assert np.array_equal(part, two[777:777 + 12_345])
+
+
 def test_learned_radius_knn_is_exact(gpu, L):
     """Train a small model quickly, then check that the seeded search returns
     exactly the plain result and report what it saves."""
