@@ -41,7 +41,7 @@ size_t octree_ws_bytes(int64_t n_max);
 int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, const float* bounds,
                   const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
                   const int32_t* parent, sx_otnode* nodes, int64_t cap, int32_t* d_oct_count, int32_t* leaf_oct,
-                  void* ws, hipStream_t st, const TravNode* trav = nullptr, int2* aux = nullptr);
+                  void* ws, hipStream_t st, const int32_t* node_end = nullptr, int2* aux = nullptr);
 int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
                hipStream_t st, CloudMap cm = CloudMap());
@@ -86,9 +86,10 @@ struct sx_index {
     int32_t* lc = nullptr;
     int32_t* parent = nullptr;
     int32_t* leaf_parent = nullptr;
-    int2* up = nullptr;  // (parent, prefix length) per node of the traversal tree
+    int2* up = nullptr;  // (parent, prefix length) per traversal-tree node, reference root only
     int32_t* leaf_oct = nullptr;
     int2* oct_aux = nullptr;  // (parent octree node, points in the cell) per octree node
+    int32_t* node_end = nullptr;  // the far end of each radix node's leaf range
     int2* knn_queue = nullptr;  // deferred queries: (index, k-th distance bound bits)
     sx::TravNode* trav = nullptr;
     sx_otnode* oct = nullptr;
@@ -134,6 +135,7 @@ size_t index_layout(sx_index* ix, void* base) {
     ix->trav = c.take<sx::TravNode>(n);
     ix->oct = c.take<sx_otnode>(static_cast<size_t>(ix->oct_cap));
     ix->oct_aux = c.take<int2>(static_cast<size_t>(ix->oct_cap));
+    ix->node_end = c.take<int32_t>(n);
     ix->ws_bbox = c.take<char>(sx::bbox_ws_bytes());
     const size_t ws = std::max({sx::sort_ws_bytes(ix->cap), sx::scan_ws_bytes(ix->cap), sx::octree_ws_bytes(ix->cap),
                                 sx::box_ws_bytes(ix->cap)});
@@ -162,15 +164,21 @@ bool valid_k(int k) { return k == 8 || k == 16 || k == 24 || k == 32; }
 namespace sx {
 // Arrays the learned-radius feature kernel reads (mlp.hip).
 int index_feature_arrays(sx_index* ix, const TravNode** trav, const int2** up, const int2** oct_aux,
-                         const int32_t** leaf_parent, const int32_t** leaf_oct, const sx_otnode** oct,
-                         const float** bounds, const int32_t** counts, int64_t* limit) {
+                         const uint8_t** pn, const int32_t** leaf_parent, const int32_t** leaf_oct,
+                         const sx_otnode** oct, int64_t* oct_cap, const float** bounds, const int32_t** counts,
+                         int64_t* limit) {
     SX_TRY(check_index(ix));
+    // textbook root: the octree links give the counts and the API's prefix
+    // array the depth; otherwise the traversal tree's (parent, prefix) pairs
+    const bool textbook = ix->root_mode == SX_ROOT_TEXTBOOK;
     *trav = ix->trav;
-    *up = ix->up;
-    *oct_aux = ix->root_mode == SX_ROOT_TEXTBOOK ? ix->oct_aux : nullptr;
+    *up = textbook ? nullptr : ix->up;
+    *oct_aux = textbook ? ix->oct_aux : nullptr;
+    *pn = textbook ? ix->pn : nullptr;
     *leaf_parent = ix->leaf_parent;
     *leaf_oct = ix->leaf_oct;
     *oct = ix->oct;
+    *oct_cap = ix->oct_cap;
     *bounds = ix->bounds;
     *counts = ix->counts;
     *limit = ix->counts_valid ? ix->n_unique : ix->n_input;
@@ -265,8 +273,8 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
     SX_TRY(sx::decode_launch(ix->ucodes, ix->counts + kCntUnique, n, ix->bounds, cb, ix->pts12, ix->pts16, st));
     if (ix->root_mode == SX_ROOT_TEXTBOOK) {
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, ix->pn, ix->lc, ix->hll,
-                                 ix->hlr, ix->parent, nullptr, ix->leaf_parent, ix->up, ix->trav, ix->pts16, ix->ws,
-                                 st));
+                                 ix->hlr, ix->parent, ix->node_end, ix->leaf_parent, nullptr, ix->trav, ix->pts16,
+                                 ix->ws, st));
     } else {
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_REFERENCE, ix->pn, ix->lc,
                                  ix->hll, ix->hlr, ix->parent, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
@@ -280,7 +288,7 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
     const bool textbook = ix->root_mode == SX_ROOT_TEXTBOOK;
     SX_TRY(sx::octree_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, ix->bounds, ix->pn, ix->lc, ix->hll, ix->hlr,
                              ix->parent, ix->oct, ix->oct_cap, ix->counts + kCntOct, ix->leaf_oct, ix->ws, st,
-                             textbook ? ix->trav : nullptr, textbook ? ix->oct_aux : nullptr));
+                             textbook ? ix->node_end : nullptr, textbook ? ix->oct_aux : nullptr));
     ix->n_input = n;
     SX_HIP(hipEventRecord(ix->built_event, sx::as_stream(stream)));
     ix->built = true;

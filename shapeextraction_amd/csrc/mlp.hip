@@ -399,14 +399,18 @@ __global__ void mlp_adam_kernel(float* __restrict__ p, float* __restrict__ mo, f
 }
 
 // ---------------------------------------------------------------- features
-// Cell counts come from the octree when it is the traversal tree's (textbook
-// root): each octree node's (parent, point count), one dependent load per
-// level.  Otherwise the walk climbs the traversal tree, one int2 (parent,
-// prefix length) per radix node, about three steps per octree level.
+// Textbook root (the octree is built on the traversal tree): cell counts come
+// from each octree node's (parent, point count), one dependent load per level,
+// and depths from the prefix-length array.  Reference root: the walk climbs
+// the traversal tree, one int2 (parent, prefix length) per radix node, about
+// three steps per octree level.
 struct FeatTree {
     const TravNode* trav;
-    const int2* up;
-    const int2* aux;  // nullable
+    const int2* up;      // reference root
+    const int2* aux;     // textbook root
+    const uint8_t* pn;   // textbook root
+    int64_t oct_cap;     // octree links are followed only inside the octree's capacity
+    __device__ __forceinline__ int prefix(int node) const { return aux ? pn[node] : up[node].y; }
 };
 
 // Features of index point u (U >= 2 points in the index).
@@ -415,24 +419,19 @@ __device__ __forceinline__ void query_features(const FeatTree& tr, const int32_t
                                                const sx_otnode* __restrict__ oct, float log2_range, int root_level,
                                                int64_t u, float (&f)[kIn]) {
     int c = leaf_parent[u] >> 1;
-    const int2 uc = tr.up[c];
-    const int d0 = uc.y / 3;
+    const int d0 = tr.prefix(c) / 3;
     const int on = leaf_oct[u];
-    if (tr.aux && on >= 0) {
+    if (tr.aux) {
         // the octree node holding u is the cell at depth d0; its ancestors are
         // the cells at d0-1, d0-2, ... up to the root, which repeats
-        int o = on;
-        int2 a = tr.aux[o];
+        int2 a = (on >= 0 && on < tr.oct_cap) ? tr.aux[on] : make_int2(-1, 1);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             f[s] = __log2f(static_cast<float>(a.y));
-            if (s < 3 && a.x >= 0) {
-                o = a.x;
-                a = tr.aux[o];
-            }
+            if (s < 3 && a.x >= 0 && a.x < tr.oct_cap) a = tr.aux[a.x];
         }
     } else {
-        int pc = uc.x;
+        int pc = tr.up[c].x;
         int2 upc = pc >= 0 ? tr.up[pc] : make_int2(-1, -1);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
@@ -449,7 +448,7 @@ __device__ __forceinline__ void query_features(const FeatTree& tr, const int32_t
     }
     f[4] = log2_range - static_cast<float>(d0);
     f[5] = static_cast<float>(d0 - root_level);
-    if (on >= 0) {
+    if (on >= 0 && on < tr.oct_cap) {
         f[6] = static_cast<float>(__popc(oct[on].child_leaf_mask));
         f[7] = static_cast<float>(__popc(oct[on].child_node_mask));
     }
@@ -466,7 +465,7 @@ __global__ void __launch_bounds__(256) features_kernel(FeatTree tr, const int32_
     const int U = d_counts[0];
     const int64_t u = q_start + t;
     float f[kIn] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (u < U && U >= 2) query_features(tr, leaf_parent, leaf_oct, oct, __log2f(bounds[2]), tr.up[0].y / 3, u, f);
+    if (u < U && U >= 2) query_features(tr, leaf_parent, leaf_oct, oct, __log2f(bounds[2]), tr.prefix(0) / 3, u, f);
     float4* out = reinterpret_cast<float4*>(feats + t * kIn);
     out[0] = make_float4(f[0], f[1], f[2], f[3]);
     out[1] = make_float4(f[4], f[5], f[6], f[7]);
@@ -534,7 +533,7 @@ __global__ void __launch_bounds__(kMlpBlock) __attribute__((amdgpu_waves_per_eu(
     const int l = static_cast<int>(lane_id());
     const int U = d_counts[0];
     const float log2_range = __log2f(bounds[2]);
-    const int root_level = U >= 2 ? tr.up[0].y / 3 : 0;
+    const int root_level = U >= 2 ? tr.prefix(0) / 3 : 0;
     const int r = l % 16, g = l / 16;
     const bf16x4_t zero4 = {0, 0, 0, 0};
     // persistent: each wave walks groups of 64 queries (no block barriers below)
@@ -640,8 +639,9 @@ int check_mlp(const sx_mlp* m) { return m ? SX_OK : sx::fail(SX_EINVAL, "null ml
 struct sx_index;
 namespace sx {
 int index_feature_arrays(sx_index* ix, const TravNode** trav, const int2** up, const int2** oct_aux,
-                         const int32_t** leaf_parent, const int32_t** leaf_oct, const sx_otnode** oct,
-                         const float** bounds, const int32_t** counts, int64_t* limit);
+                         const uint8_t** pn, const int32_t** leaf_parent, const int32_t** leaf_oct,
+                         const sx_otnode** oct, int64_t* oct_cap, const float** bounds, const int32_t** counts,
+                         int64_t* limit);
 }
 
 using sx::fail;
@@ -801,16 +801,19 @@ int sx_radius_features(sx_index* index, int64_t q_start, int64_t n_q, float* d_f
     const sx::TravNode* trav;
     const int32_t *lp, *lo, *counts;
     const int2 *up, *aux;
+    const uint8_t* pn;
     const sx_otnode* oct;
     const float* bounds;
     int64_t limit;
-    SX_TRY(sx::index_feature_arrays(index, &trav, &up, &aux, &lp, &lo, &oct, &bounds, &counts, &limit));
+    int64_t oct_cap;
+    SX_TRY(sx::index_feature_arrays(index, &trav, &up, &aux, &pn, &lp, &lo, &oct, &oct_cap, &bounds, &counts,
+                                    &limit));
     if (q_start < 0 || n_q < 0 || q_start + n_q > limit) return fail(SX_EINVAL, "sx_radius_features: bad query range");
     if (n_q == 0) return SX_OK;
     if (!d_feats || reinterpret_cast<uintptr_t>(d_feats) % 16 != 0)
         return fail(SX_EINVAL, "sx_radius_features: output must be 16-byte aligned");
     sx::features_kernel<<<sx::grid_for(n_q, 256), 256, 0, sx::as_stream(stream)>>>(
-        sx::FeatTree{trav, up, aux}, lp, lo, oct, bounds, counts, q_start, n_q, d_feats);
+        sx::FeatTree{trav, up, aux, pn, oct_cap}, lp, lo, oct, bounds, counts, q_start, n_q, d_feats);
     SX_LAUNCH_CHECK("features_kernel");
     return SX_OK;
 }
@@ -822,10 +825,13 @@ int sx_radius_predict(sx_index* index, sx_mlp* mlp, int64_t q_start, int64_t n_q
     const sx::TravNode* trav;
     const int32_t *lp, *lo, *counts;
     const int2 *up, *aux;
+    const uint8_t* pn;
     const sx_otnode* oct;
     const float* bounds;
     int64_t limit;
-    SX_TRY(sx::index_feature_arrays(index, &trav, &up, &aux, &lp, &lo, &oct, &bounds, &counts, &limit));
+    int64_t oct_cap;
+    SX_TRY(sx::index_feature_arrays(index, &trav, &up, &aux, &pn, &lp, &lo, &oct, &oct_cap, &bounds, &counts,
+                                    &limit));
     if (q_start < 0 || n_q < 0 || q_start + n_q > limit) return fail(SX_EINVAL, "sx_radius_predict: bad query range");
     if (n_q == 0) return SX_OK;
     if (!d_radius2) return fail(SX_EINVAL, "sx_radius_predict: null output");
@@ -833,7 +839,7 @@ int sx_radius_predict(sx_index* index, sx_mlp* mlp, int64_t q_start, int64_t n_q
     int64_t blocks = (waves + sx::kWavesPerBlock - 1) / sx::kWavesPerBlock;
     if (blocks > 2048) blocks = 2048;  // persistent: the weights are copied to LDS once per block
     sx::radius_kernel<<<static_cast<unsigned>(blocks), sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
-        sx::FeatTree{trav, up, aux}, lp, lo, oct, bounds, counts, q_start, n_q, dev_view(mlp), d_radius2);
+        sx::FeatTree{trav, up, aux, pn, oct_cap}, lp, lo, oct, bounds, counts, q_start, n_q, dev_view(mlp), d_radius2);
     SX_LAUNCH_CHECK("radius_kernel");
     return SX_OK;
 }

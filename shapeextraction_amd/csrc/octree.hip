@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(256) oct_gather_kernel(const uint64_t* __restr
                                                          const int32_t* __restrict__ offsets,
                                                          const int32_t* __restrict__ counts, sx_otnode* nodes,
                                                          int64_t cap, int32_t* leaf_oct,
-                                                         const TravNode* __restrict__ trav, int2* aux) {
+                                                         const int32_t* __restrict__ node_end, int2* aux) {
     const int64_t U = scan_count(d_count, n_max);
     const int64_t M = U - 1;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -104,7 +104,8 @@ __global__ void __launch_bounds__(256) oct_gather_kernel(const uint64_t* __restr
         // (parent octree node, points in the cell) of radix node i's chain:
         // each chain node's parent is the next one up; the top's parent is
         // written by the ancestor that gathers it (below), the root's is -1
-        const int sz = w_int(trav[i].lo_r.w) - w_int(trav[i].lo_l.w) + 1;
+        const int e = node_end[i];
+        const int sz = (e > i ? e - static_cast<int>(i) : static_cast<int>(i) - e) + 1;
         for (int j = 0; j < c; ++j) {
             if (base + j >= cap) break;
             if (j + 1 < c) aux[base + j] = make_int2(base + j + 1, sz);
@@ -178,7 +179,7 @@ size_t octree_ws_bytes(int64_t n_max) {
 int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, const float* bounds,
                   const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
                   const int32_t* parent, sx_otnode* nodes, int64_t cap, int32_t* d_oct_count, int32_t* leaf_oct,
-                  void* ws, hipStream_t st, const TravNode* trav = nullptr, int2* aux = nullptr) {
+                  void* ws, hipStream_t st, const int32_t* node_end = nullptr, int2* aux = nullptr) {
     if (n_max <= 0) return SX_OK;
     Carver c(ws);
     int32_t* offsets = c.take<int32_t>(static_cast<size_t>(n_max));
@@ -187,8 +188,8 @@ int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
     SX_TRY(device_scan(OctCountIn{pn, parent, d_count}, OctCountOut{offsets, counts}, d_count, n_max, d_oct_count,
                        scan_ws, st));
     oct_gather_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(codes, d_count, n_max, code_bits, bounds, pn, left_child,
-                                                            hll, hlr, offsets, counts, nodes, cap, leaf_oct, trav,
-                                                            trav ? aux : nullptr);
+                                                            hll, hlr, offsets, counts, nodes, cap, leaf_oct, node_end,
+                                                            node_end ? aux : nullptr);
     SX_LAUNCH_CHECK("oct_gather_kernel");
     return SX_OK;
 }
