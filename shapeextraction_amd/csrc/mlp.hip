@@ -23,6 +23,7 @@
 // LDS.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -76,6 +77,8 @@ __device__ __forceinline__ f32x4_t mfma(bf16x4_t a, bf16x4_t b, f32x4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+struct MlpLds;
+
 // Device-side weights: BF16 copies laid out for the A operand.
 struct MlpDev {
     const uint16_t* w1p;  // [64][16]  W1 padded with zero inputs 8..15
@@ -84,6 +87,7 @@ struct MlpDev {
     const uint16_t* w3;   // [64]
     const float* p;       // FP32 master parameters (biases are read from here)
     const float* norm;    // mu[8], inv_sigma[8], margin
+    const MlpLds* img;    // inference operands in their LDS layout (pack_image_kernel)
 };
 
 struct Fwd {
@@ -126,9 +130,9 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpDev& m, bf16x4_t xb, F
     f.y = c[0];
 }
 
-// Layer-1 B operand for the tile starting at query q0: lane l holds the
-// normalised features 4*(l/16)+e of query q0 + l%16 (zero padding for l >= 32
-// and for queries >= n).
+// Layer-1 B operand for the training tile starting at query q0: lane l holds
+// the normalised features 4*(l/16)+e of query q0 + l%16 (zero padding for
+// l >= 32 and for queries >= n).
 __device__ __forceinline__ bf16x4_t load_x(const MlpDev& m, const float* __restrict__ feats, int64_t q0, int64_t n) {
     const int l = static_cast<int>(lane_id());
     const int r = l % 16, g = l / 16;
@@ -141,71 +145,175 @@ __device__ __forceinline__ bf16x4_t load_x(const MlpDev& m, const float* __restr
                  (v.z - mu[4 * g + 2]) * is[4 * g + 2], (v.w - mu[4 * g + 3]) * is[4 * g + 3]);
 }
 
-// A operands and biases of one lane, loaded once per wave for inference.
-struct MlpRegs {
-    bf16x4_t a1[4], a2[4][4], a3[4];
-    f32x4_t b1[4], b2[4];
-    float b3;
-    __device__ __forceinline__ void load(const MlpDev& m) {
-        const int l = static_cast<int>(lane_id());
-        const int r = l % 16, g = l / 16;
-        const bf16x4_t zero4 = {0, 0, 0, 0};
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-            a1[mt] = load4(m.w1p + (16 * mt + r) * kInPad + 4 * g);
-            a3[mt] = (r == 0) ? load4(m.w3 + 16 * mt + 4 * g) : zero4;
-#pragma unroll
-            for (int kc = 0; kc < 4; ++kc) a2[mt][kc] = load4(m.w2 + (16 * mt + r) * kH + 16 * kc + 4 * g);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                b1[mt][e] = m.p[kOffB1 + 16 * mt + 4 * g + e];
-                b2[mt][e] = m.p[kOffB2 + 16 * mt + 4 * g + e];
-            }
-        }
-        b3 = (g == 0) ? m.p[kOffB3] : 0.0f;
-    }
-    // y of the tile's 16 queries (valid in lanes 0..15)
-    __device__ __forceinline__ float forward(bf16x4_t xb) const {
-        bf16x4_t h1[4], h2[4];
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt) h1[mt] = relu_pack(mfma(a1[mt], xb, b1[mt]));
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-            f32x4_t c = b2[mt];
-#pragma unroll
-            for (int kc = 0; kc < 4; ++kc) c = mfma(a2[mt][kc], h1[kc], c);
-            h2[mt] = relu_pack(c);
-        }
-        f32x4_t c = {b3, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kc = 0; kc < 4; ++kc) c = mfma(a3[kc], h2[kc], c);
-        return c[0];
-    }
+// ---------------------------------------------------------------- inference
+// 64 queries (four 16-query tiles) per pass, so each weight operand is read
+// from LDS once per 64 queries.  Layer 1 (8 -> 64) on v_mfma_f32_16x16x16_bf16
+// (inputs padded to 16); layers 2 (64 -> 64) and 3 (64 -> 1) on gfx950's
+// v_mfma_f32_16x16x32_bf16: lane l holds A[i = l%16][k = 8*(l/16)+e],
+// B[k = 8*(l/16)+e][j = l%16], e = 0..7, and D as for 16x16x16.  The K index
+// of a 32-unit chunk kc is permuted so that the D tiles 2kc and 2kc+1 of the
+// previous layer, concatenated per lane, are already its B operand:
+//   k = 8g + e  <->  hidden unit 32kc + 16*(e/4) + 4g + e%4,
+// and the A operands (weights) are stored in LDS in that order, lane-major,
+// so a wave's operand read is one contiguous block.  ReLU is taken after the
+// BF16 rounding, on the packed halves as int16 (v_pk_max_i16): a negative
+// BF16 value is a negative int16, so this equals rounding max(z, 0).
+typedef short bf16x8s_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f32x4_t mfma32(bf16x8s_t a, bf16x8s_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                   c, 0, 0, 0);
+}
+__device__ __forceinline__ bf16x8s_t cat8(bf16x4_t a, bf16x4_t b) {
+    return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+__device__ __forceinline__ bf16x4_t round_relu(f32x4_t z) {
+    const bf16x4_t zero4 = {0, 0, 0, 0};
+    return __builtin_elementwise_max(pack4(z[0], z[1], z[2], z[3]), zero4);
+}
+// Hidden unit of (chunk kc, lane group g, element e) in the permuted K order.
+__host__ __device__ constexpr int perm_unit(int kc, int g, int e) { return 32 * kc + 16 * (e / 4) + 4 * g + e % 4; }
+
+struct alignas(16) MlpLds {
+    uint16_t w1x[4 * kWave * 4];  // layer 1, output tile mt: lane l's four BF16 at (mt*64 + l)*4
+    uint16_t w2x[8 * kWave * 8];  // layer 2, (output tile mt, chunk kc): lane l's eight at ((mt*2+kc)*64 + l)*8
+    uint16_t w3x[2 * kWave * 8];  // layer 3, chunk kc: row 0 only (zero for l%16 != 0)
+    float b1[kH], b2[kH], b3;
+    float mu[kIn], is[kIn], margin;
 };
 
-// mode 0: y (raw prediction); mode 1: exp(y + margin) (squared radius).
-// Each wave keeps the weights in registers and walks tiles of 16 queries.
-__global__ void __launch_bounds__(kMlpBlock) mlp_forward_kernel(MlpDev m, const float* __restrict__ feats, int64_t n,
-                                                                int mode, float* __restrict__ out) {
-    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-    const int64_t n_waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
-    const int l = static_cast<int>(lane_id());
-    const float margin = m.norm[2 * kIn];
-    MlpRegs w;
-    w.load(m);
-    const int64_t tiles = (n + 15) / 16;
-    for (int64_t t = wave; t < tiles; t += n_waves) {
-        const int64_t q0 = t * 16;
-        const float y = w.forward(load_x(m, feats, q0, n));
-        if (l < 16 && q0 + l < n) out[q0 + l] = mode == 0 ? y : __expf(y + margin);
+// The weights in the operand layouts (block-wide; sw may be global memory).
+__device__ __forceinline__ void build_operands(const MlpDev& m, MlpLds& sw) {
+    const int tid = static_cast<int>(threadIdx.x);
+    for (int i = tid; i < 4 * kWave * 4; i += blockDim.x) {
+        const int e = i % 4, l = (i / 4) % kWave, mt = i / (4 * kWave);
+        sw.w1x[i] = m.w1p[(16 * mt + l % 16) * kInPad + 4 * (l / 16) + e];
+    }
+    for (int i = tid; i < 8 * kWave * 8; i += blockDim.x) {
+        const int e = i % 8, l = (i / 8) % kWave, c = i / (8 * kWave), mt = c / 2, kc = c % 2;
+        sw.w2x[i] = m.w2[(16 * mt + l % 16) * kH + perm_unit(kc, l / 16, e)];
+    }
+    for (int i = tid; i < 2 * kWave * 8; i += blockDim.x) {
+        const int e = i % 8, l = (i / 8) % kWave, kc = i / (8 * kWave);
+        sw.w3x[i] = (l % 16 == 0) ? m.w3[perm_unit(kc, l / 16, e)] : static_cast<uint16_t>(0);
+    }
+    for (int i = tid; i < kH; i += blockDim.x) {
+        sw.b1[i] = m.p[kOffB1 + i];
+        sw.b2[i] = m.p[kOffB2 + i];
+    }
+    for (int i = tid; i < kIn; i += blockDim.x) {
+        sw.mu[i] = m.norm[i];
+        sw.is[i] = m.norm[kIn + i];
+    }
+    if (tid == 0) {
+        sw.b3 = m.p[kOffB3];
+        sw.margin = m.norm[2 * kIn];
     }
 }
 
-inline unsigned mlp_forward_grid(int64_t n) {
-    const int64_t tiles = (n + 15) / 16;
-    int64_t blocks = (tiles + 4 * kWavesPerBlock - 1) / (4 * kWavesPerBlock);  // about 4 tiles per wave
-    if (blocks > 4096) blocks = 4096;
-    return static_cast<unsigned>(blocks < 1 ? 1 : blocks);
+// One block: the image every inference block copies into LDS, rebuilt
+// whenever the parameters or the normaliser change.
+__global__ void __launch_bounds__(256) pack_image_kernel(MlpDev m, MlpLds* img) { build_operands(m, *img); }
+
+// Block-wide copy of the packed image into LDS (ends with a barrier).
+__device__ __forceinline__ void stage_weights(const MlpDev& m, MlpLds& sw) {
+    static_assert(sizeof(MlpLds) % 16 == 0, "stage_weights: 16-byte copies");
+    const uint4* src = reinterpret_cast<const uint4*>(m.img);
+    uint4* dst = reinterpret_cast<uint4*>(&sw);
+    for (int i = static_cast<int>(threadIdx.x); i < static_cast<int>(sizeof(MlpLds) / 16); i += blockDim.x)
+        dst[i] = src[i];
+    __syncthreads();
+}
+
+__device__ __forceinline__ bf16x8s_t lds8(const uint16_t* p) {
+    return __builtin_bit_cast(bf16x8s_t, *reinterpret_cast<const uint4*>(p));
+}
+
+// Layer-1 B operand of one tile: lane l holds the normalised features
+// 4*(l/16)+e of its query (v = those four raw features; lanes l >= 32 and
+// absent queries pass present = false and get zeros).
+__device__ __forceinline__ bf16x4_t feature_operand(const MlpLds& sw, float4 v, bool present) {
+    const int g = static_cast<int>(lane_id()) / 16;
+    if (!present || g >= 2) return bf16x4_t{0, 0, 0, 0};
+    const int o = 4 * g;
+    return pack4((v.x - sw.mu[o]) * sw.is[o], (v.y - sw.mu[o + 1]) * sw.is[o + 1], (v.z - sw.mu[o + 2]) * sw.is[o + 2],
+                 (v.w - sw.mu[o + 3]) * sw.is[o + 3]);
+}
+
+// y of the four tiles: y[t][0] in lanes 0..15 is the output of query 16t + l.
+// The weights are re-read from LDS per call: the empty asm hides that the
+// address is loop-invariant, so the compiler does not hoist every operand
+// into registers (which would cost the kernel half its occupancy).
+__device__ __forceinline__ void mlp_forward64(const MlpLds& s_w, const bf16x4_t (&xb)[4], f32x4_t (&y)[4]) {
+    int zo = 0;
+    asm volatile("" : "+v"(zo));
+    const MlpLds& sw = *reinterpret_cast<const MlpLds*>(reinterpret_cast<const char*>(&s_w) + zo);
+    const int l = static_cast<int>(lane_id());
+    const int g = l / 16;
+    bf16x4_t h1[4][4];  // [tile][output tile]
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const bf16x4_t a = load4(sw.w1x + (mt * kWave + l) * 4);
+        const f32x4_t b = *reinterpret_cast<const f32x4_t*>(sw.b1 + 16 * mt + 4 * g);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) h1[t][mt] = round_relu(mfma(a, xb[t], b));
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) y[t] = f32x4_t{g == 0 ? sw.b3 : 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc) {  // layer-2 output tiles 2kc, 2kc+1 = layer-3 chunk kc
+        bf16x4_t h2[4][2];
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int mt = 2 * kc + half;
+            const f32x4_t b = *reinterpret_cast<const f32x4_t*>(sw.b2 + 16 * mt + 4 * g);
+            f32x4_t c[4] = {b, b, b, b};
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk) {
+                const bf16x8s_t a = lds8(sw.w2x + ((mt * 2 + kk) * kWave + l) * 8);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) c[t] = mfma32(a, cat8(h1[t][2 * kk], h1[t][2 * kk + 1]), c[t]);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) h2[t][half] = round_relu(c[t]);
+        }
+        const bf16x8s_t a3 = lds8(sw.w3x + (kc * kWave + l) * 8);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) y[t] = mfma32(a3, cat8(h2[t][0], h2[t][1]), y[t]);
+    }
+}
+
+// mode 0: y (raw prediction); mode 1: exp(y + margin) (squared radius).
+// Persistent: each wave walks groups of 64 queries.
+__global__ void __launch_bounds__(kMlpBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) mlp_forward_kernel(MlpDev m, const float* __restrict__ feats, int64_t n,
+                                                                int mode, float* __restrict__ out) {
+    __shared__ MlpLds s_w;
+    stage_weights(m, s_w);
+    const int w = static_cast<int>(threadIdx.x) / kWave;
+    const int l = static_cast<int>(lane_id());
+    const int r = l % 16, g = l / 16;
+    const int64_t groups = (n + kWave - 1) / kWave;
+    for (int64_t grp = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w; grp < groups;
+         grp += static_cast<int64_t>(gridDim.x) * kWavesPerBlock) {
+        const int64_t q0 = grp * kWave;
+        bf16x4_t xb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int64_t q = q0 + 16 * t + r;
+            const bool present = g < 2 && q < n;
+            const float4 v = present ? *reinterpret_cast<const float4*>(feats + q * kIn + 4 * g)
+                                     : make_float4(0.f, 0.f, 0.f, 0.f);
+            xb[t] = feature_operand(s_w, v, present);
+        }
+        f32x4_t y[4];
+        mlp_forward64(s_w, xb, y);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int64_t q = q0 + 16 * t + l;
+            if (l < 16 && q < n) out[q] = mode == 0 ? y[t][0] : __expf(y[t][0] + s_w.margin);
+        }
+    }
 }
 
 // Gradient of mean((y - t)^2) over n samples.  Each wave handles 64 samples
@@ -472,70 +580,24 @@ __global__ void __launch_bounds__(256) features_kernel(FeatTree tr, const int32_
 }
 
 // Features and the MLP in one pass: each wave computes the features of its
-// 64 queries (one per lane), stages them in LDS, and runs the four 16-query
-// tiles on MFMA with the weights read from LDS (copied once per block), so
-// the walk up the tree keeps the occupancy of the plain feature kernel;
-// r2 = exp(y + margin).  The same arithmetic as features_kernel followed by
-// mlp_forward_kernel (mode 1), without the 32-byte feature rows in HBM.
-// Rows padded by 4 BF16 (8 bytes): the 16 lanes of an A-operand read hit
-// distinct banks (unpadded, the 128-byte W2 rows would put all 16 on one).
-constexpr int kW1Row = kInPad + 4, kW2Row = kH + 4;
-struct MlpLds {
-    uint16_t w1p[kH * kW1Row];
-    uint16_t w2[kH * kW2Row];
-    uint16_t w3[kH];
-    float b1[kH], b2[kH], b3;
-    float mu[kIn], is[kIn], margin;
-};
-
-__device__ __forceinline__ bf16x4_t lds4(const uint16_t* p) {
-    const uint2 w = *reinterpret_cast<const uint2*>(p);
-    bf16x4_t v;
-    v[0] = static_cast<short>(w.x & 0xFFFF);
-    v[1] = static_cast<short>(w.x >> 16);
-    v[2] = static_cast<short>(w.y & 0xFFFF);
-    v[3] = static_cast<short>(w.y >> 16);
-    return v;
-}
-
-__global__ void __launch_bounds__(kMlpBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) radius_kernel(FeatTree tr, const int32_t* __restrict__ leaf_parent,
+// 64 queries (one per lane), stages them in LDS and runs mlp_forward64 on
+// them, r2 = exp(y + margin): the same arithmetic as features_kernel followed
+// by mlp_forward_kernel (mode 1), without the 32-byte feature rows in HBM.
+__global__ void __launch_bounds__(kMlpBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) radius_kernel(FeatTree tr, const int32_t* __restrict__ leaf_parent,
                                                            const int32_t* __restrict__ leaf_oct,
                                                            const sx_otnode* __restrict__ oct,
                                                            const float* __restrict__ bounds,
                                                            const int32_t* __restrict__ d_counts, int64_t q_start,
                                                            int64_t n_q, MlpDev m, float* __restrict__ r2) {
     __shared__ MlpLds s_w;
-    __shared__ float s_f[kWavesPerBlock][kWave][kIn + 1];  // +1: no bank conflicts on the column reads
-    const int tid = static_cast<int>(threadIdx.x);
-    for (int i = tid; i < kH * kInPad / 4; i += kMlpBlock) {
-        const int row = i / (kInPad / 4), c4 = i % (kInPad / 4);
-        *reinterpret_cast<uint2*>(s_w.w1p + row * kW1Row + 4 * c4) = reinterpret_cast<const uint2*>(m.w1p)[i];
-    }
-    for (int i = tid; i < kH * kH / 4; i += kMlpBlock) {
-        const int row = i / (kH / 4), c4 = i % (kH / 4);
-        *reinterpret_cast<uint2*>(s_w.w2 + row * kW2Row + 4 * c4) = reinterpret_cast<const uint2*>(m.w2)[i];
-    }
-    if (tid < kH) {
-        s_w.w3[tid] = m.w3[tid];
-        s_w.b1[tid] = m.p[kOffB1 + tid];
-        s_w.b2[tid] = m.p[kOffB2 + tid];
-    }
-    if (tid < kIn) {
-        s_w.mu[tid] = m.norm[tid];
-        s_w.is[tid] = m.norm[kIn + tid];
-    }
-    if (tid == 0) {
-        s_w.b3 = m.p[kOffB3];
-        s_w.margin = m.norm[2 * kIn];
-    }
-    __syncthreads();
-    const int w = tid / kWave;
+    __shared__ float4 s_f[kWavesPerBlock][2][kWave];  // features 0-3 and 4-7 of the wave's queries
+    stage_weights(m, s_w);
+    const int w = static_cast<int>(threadIdx.x) / kWave;
     const int l = static_cast<int>(lane_id());
+    const int r = l % 16, g = l / 16;
     const int U = d_counts[0];
     const float log2_range = __log2f(bounds[2]);
     const int root_level = U >= 2 ? tr.prefix(0) / 3 : 0;
-    const int r = l % 16, g = l / 16;
-    const bf16x4_t zero4 = {0, 0, 0, 0};
     // persistent: each wave walks groups of 64 queries (no block barriers below)
     const int64_t groups = (n_q + kWave - 1) / kWave;
     for (int64_t grp = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w; grp < groups;
@@ -546,42 +608,23 @@ __global__ void __launch_bounds__(kMlpBlock) __attribute__((amdgpu_waves_per_eu(
             float f[kIn] = {0, 0, 0, 0, 0, 0, 0, 0};
             if (q0 + l < n_q && u < U && U >= 2)
                 query_features(tr, leaf_parent, leaf_oct, oct, log2_range, root_level, u, f);
-#pragma unroll
-            for (int i = 0; i < kIn; ++i) s_f[w][l][i] = f[i];
+            s_f[w][0][l] = make_float4(f[0], f[1], f[2], f[3]);
+            s_f[w][1][l] = make_float4(f[4], f[5], f[6], f[7]);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-#pragma unroll 1
-        for (int t = 0; t < kWave / 16; ++t) {
-            // the weights are re-read from LDS per tile rather than hoisted into
-            // registers (which would halve the occupancy of the walk above)
-            int zo = 0;
-            asm volatile("" : "+v"(zo));
-            const MlpLds& sw = *reinterpret_cast<const MlpLds*>(reinterpret_cast<const char*>(&s_w) + zo);
-            const int qi = 16 * t + r;
-            bf16x4_t xb = zero4;
-            if (g < 2 && q0 + qi < n_q) {
-                const float* v = s_f[w][qi] + 4 * g;
-                xb = pack4((v[0] - sw.mu[4 * g]) * sw.is[4 * g], (v[1] - sw.mu[4 * g + 1]) * sw.is[4 * g + 1],
-                           (v[2] - sw.mu[4 * g + 2]) * sw.is[4 * g + 2], (v[3] - sw.mu[4 * g + 3]) * sw.is[4 * g + 3]);
-            }
-            bf16x4_t h1[4], h2[4];
+        bf16x4_t xb[4];
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt) {
-                const f32x4_t b = *reinterpret_cast<const f32x4_t*>(sw.b1 + 16 * mt + 4 * g);
-                h1[mt] = relu_pack(mfma(lds4(sw.w1p + (16 * mt + r) * kW1Row + 4 * g), xb, b));
-            }
+        for (int t = 0; t < 4; ++t) {
+            const bool present = g < 2 && q0 + 16 * t + r < n_q;
+            xb[t] = feature_operand(s_w, present ? s_f[w][g & 1][16 * t + r] : make_float4(0.f, 0.f, 0.f, 0.f),
+                                    present);
+        }
+        f32x4_t y[4];
+        mlp_forward64(s_w, xb, y);
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt) {
-                f32x4_t c = *reinterpret_cast<const f32x4_t*>(sw.b2 + 16 * mt + 4 * g);
-#pragma unroll
-                for (int kc = 0; kc < 4; ++kc)
-                    c = mfma(lds4(sw.w2 + (16 * mt + r) * kW2Row + 16 * kc + 4 * g), h1[kc], c);
-                h2[mt] = relu_pack(c);
-            }
-            f32x4_t c = {g == 0 ? sw.b3 : 0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int kc = 0; kc < 4; ++kc) c = mfma(r == 0 ? lds4(sw.w3 + 16 * kc + 4 * g) : zero4, h2[kc], c);
-            if (l < 16 && q0 + 16 * t + l < n_q) r2[q0 + 16 * t + l] = __expf(c[0] + sw.margin);
+        for (int t = 0; t < 4; ++t) {
+            const int64_t q = q0 + 16 * t + l;
+            if (l < 16 && q < n_q) r2[q] = __expf(y[t][0] + s_w.margin);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // s_f is refilled by the next group
     }
@@ -602,6 +645,7 @@ struct sx_mlp {
     uint16_t* w2 = nullptr;
     uint16_t* w2t = nullptr;
     uint16_t* w3 = nullptr;
+    sx::MlpLds* img = nullptr;  // inference operands, packed
     float* partial = nullptr;
     float* loss_partial = nullptr;
     int64_t max_batch = 0;
@@ -622,6 +666,7 @@ size_t mlp_layout(sx_mlp* m, void* base) {
     m->w2 = c.take<uint16_t>(sx::kH * sx::kH);
     m->w2t = c.take<uint16_t>(sx::kH * sx::kH);
     m->w3 = c.take<uint16_t>(sx::kH);
+    m->img = c.take<sx::MlpLds>(1);
     // whole blocks of waves: every wave of the last block writes its (zero) partial
     const int64_t waves = ((m->max_batch + 63) / 64 + sx::kWavesPerBlock - 1) / sx::kWavesPerBlock * sx::kWavesPerBlock;
     m->partial = c.take<float>(static_cast<size_t>(waves) * sx::kNParams);
@@ -629,7 +674,40 @@ size_t mlp_layout(sx_mlp* m, void* base) {
     return c.off;
 }
 
-sx::MlpDev dev_view(const sx_mlp* m) { return sx::MlpDev{m->w1p, m->w2, m->w2t, m->w3, m->p, m->norm}; }
+sx::MlpDev dev_view(const sx_mlp* m) { return sx::MlpDev{m->w1p, m->w2, m->w2t, m->w3, m->p, m->norm, m->img}; }
+
+// Rebuild the packed inference operands (after any change of the BF16
+// weights, biases or normaliser; stream-ordered after that change).
+int pack_image(sx_mlp* m, hipStream_t st) {
+    sx::pack_image_kernel<<<1, 256, 0, st>>>(dev_view(m), m->img);
+    SX_LAUNCH_CHECK("pack_image_kernel");
+    return SX_OK;
+}
+
+// Grid of a persistent inference kernel (64 queries per wave and step): the
+// blocks the queries need, at most the blocks the device holds at once (a
+// block that starts after the first ones finish would double the tail).  The
+// occupancy is queried once per device and kernel.
+std::atomic<int64_t> g_resident_forward[64], g_resident_radius[64];
+
+template <typename Kern>
+int persistent_grid(Kern kernel, std::atomic<int64_t>* cache, int64_t n, unsigned* grid) {
+    const int64_t groups = (n + sx::kWave - 1) / sx::kWave;
+    int64_t blocks = (groups + sx::kWavesPerBlock - 1) / sx::kWavesPerBlock;
+    int dev = 0;
+    SX_HIP(hipGetDevice(&dev));
+    int64_t resident = dev >= 0 && dev < 64 ? cache[dev].load(std::memory_order_relaxed) : 0;
+    if (resident <= 0) {
+        int cus = 0, per_cu = 0;
+        SX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        SX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, sx::kMlpBlock, 0));
+        resident = static_cast<int64_t>(cus > 0 ? cus : 1) * (per_cu > 0 ? per_cu : 1);
+        if (dev >= 0 && dev < 64) cache[dev].store(resident, std::memory_order_relaxed);
+    }
+    if (blocks > resident) blocks = resident;
+    *grid = static_cast<unsigned>(blocks < 1 ? 1 : blocks);
+    return SX_OK;
+}
 
 int check_mlp(const sx_mlp* m) { return m ? SX_OK : sx::fail(SX_EINVAL, "null mlp"); }
 
@@ -691,6 +769,7 @@ int sx_mlp_set_params(sx_mlp* m, const float* h_params, void* stream) {
     SX_TRY(sx::fill_u8(m->ve, 0, sizeof(float) * sx::kNParams, st));
     sx::mlp_sync_bf16_kernel<<<(sx::kNParams + 1023) / 1024 * 1, 1024, 0, st>>>(m->p, m->w1p, m->w2, m->w2t, m->w3);
     SX_LAUNCH_CHECK("mlp_sync_bf16_kernel");
+    SX_TRY(pack_image(m, st));
     SX_HIP(hipStreamSynchronize(st));
     m->step = 0;
     return SX_OK;
@@ -735,6 +814,7 @@ int sx_mlp_set_normalizer(sx_mlp* m, const float* h_mu, const float* h_sigma, fl
     h[2 * sx::kIn] = margin;
     hipStream_t st = sx::as_stream(stream);
     SX_HIP(hipMemcpyAsync(m->norm, h, sizeof(h), hipMemcpyHostToDevice, st));
+    SX_TRY(pack_image(m, st));
     SX_HIP(hipStreamSynchronize(st));
     return SX_OK;
 }
@@ -743,8 +823,9 @@ int sx_mlp_forward(sx_mlp* m, const float* d_feats, int64_t n, float* d_out, voi
     SX_TRY(check_mlp(m));
     if (n < 0 || (n > 0 && (!d_feats || !d_out))) return fail(SX_EINVAL, "sx_mlp_forward: bad arguments");
     if (n == 0) return SX_OK;
-    sx::mlp_forward_kernel<<<sx::mlp_forward_grid(n), sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
-        dev_view(m), d_feats, n, 0, d_out);
+    unsigned grid = 1;
+    SX_TRY(persistent_grid(sx::mlp_forward_kernel, g_resident_forward, n, &grid));
+    sx::mlp_forward_kernel<<<grid, sx::kMlpBlock, 0, sx::as_stream(stream)>>>(dev_view(m), d_feats, n, 0, d_out);
     SX_LAUNCH_CHECK("mlp_forward_kernel");
     return SX_OK;
 }
@@ -753,8 +834,9 @@ int sx_mlp_predict_radius2(sx_mlp* m, const float* d_feats, int64_t n, float* d_
     SX_TRY(check_mlp(m));
     if (n < 0 || (n > 0 && (!d_feats || !d_radius2))) return fail(SX_EINVAL, "sx_mlp_predict_radius2: bad arguments");
     if (n == 0) return SX_OK;
-    sx::mlp_forward_kernel<<<sx::mlp_forward_grid(n), sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
-        dev_view(m), d_feats, n, 1, d_radius2);
+    unsigned grid = 1;
+    SX_TRY(persistent_grid(sx::mlp_forward_kernel, g_resident_forward, n, &grid));
+    sx::mlp_forward_kernel<<<grid, sx::kMlpBlock, 0, sx::as_stream(stream)>>>(dev_view(m), d_feats, n, 1, d_radius2);
     SX_LAUNCH_CHECK("mlp_forward_kernel");
     return SX_OK;
 }
@@ -793,6 +875,7 @@ int sx_mlp_adam_step(sx_mlp* m, const float* d_feats, const float* d_targets, in
     sx::mlp_adam_kernel<<<(sx::kNParams + 255) / 256, 256, 0, sx::as_stream(stream)>>>(
         m->p, m->mo, m->ve, m->grad, lr, b1, b2, 1e-8f, c1, c2, wd, m->w1p, m->w2, m->w2t, m->w3);
     SX_LAUNCH_CHECK("mlp_adam_kernel");
+    SX_TRY(pack_image(m, sx::as_stream(stream)));
     return SX_OK;
 }
 
@@ -835,10 +918,9 @@ int sx_radius_predict(sx_index* index, sx_mlp* mlp, int64_t q_start, int64_t n_q
     if (q_start < 0 || n_q < 0 || q_start + n_q > limit) return fail(SX_EINVAL, "sx_radius_predict: bad query range");
     if (n_q == 0) return SX_OK;
     if (!d_radius2) return fail(SX_EINVAL, "sx_radius_predict: null output");
-    const int64_t waves = (n_q + sx::kWave - 1) / sx::kWave;
-    int64_t blocks = (waves + sx::kWavesPerBlock - 1) / sx::kWavesPerBlock;
-    if (blocks > 2048) blocks = 2048;  // persistent: the weights are copied to LDS once per block
-    sx::radius_kernel<<<static_cast<unsigned>(blocks), sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
+    unsigned grid = 1;
+    SX_TRY(persistent_grid(sx::radius_kernel, g_resident_radius, n_q, &grid));
+    sx::radius_kernel<<<grid, sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
         sx::FeatTree{trav, up, aux, pn, oct_cap}, lp, lo, oct, bounds, counts, q_start, n_q, dev_view(mlp), d_radius2);
     SX_LAUNCH_CHECK("radius_kernel");
     return SX_OK;
