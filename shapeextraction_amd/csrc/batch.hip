@@ -41,8 +41,7 @@ int sort_pairs_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t 
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
                   int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav, const float4* pts,
-                  void* box_ws, hipStream_t st, CloudMap cm);
-size_t box_ws_bytes(int64_t n_max);
+                  const float* bounds, hipStream_t st, CloudMap cm);
 int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                      int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
                      int32_t* deferred_total, hipStream_t st, CloudMap cm, int queue_blocks);
@@ -181,7 +180,7 @@ struct sx_batch {
 namespace {
 size_t batch_ws_bytes(int64_t cap, int64_t slot_cap) {
     size_t ws = std::max(sx::sort_ws_bytes(cap), sx::scan_ws_bytes(cap));
-    return std::max(ws, sx::box_ws_bytes(slot_cap));
+    return ws;
 }
 
 size_t batch_layout(sx_batch* b, void* base) {
@@ -256,7 +255,7 @@ int batch_build(sx_batch* b, const float* x, const float* y, const float* z, hip
     const int32_t* slots = b->counts + sx::kBSlots;
     SX_TRY(sx::seg_decode_launch(b->ucodes, slots, b->slot_cap, b->map(), b->bounds, cb, b->pts16, st));
     SX_TRY(sx::karras_launch(b->ucodes, slots, b->slot_cap, cb, SX_ROOT_TEXTBOOK, b->pn, b->lc, nullptr, nullptr,
-                             b->parent, nullptr, nullptr, nullptr, b->trav, b->pts16, b->ws, st, b->map()));
+                             b->parent, nullptr, nullptr, nullptr, b->trav, b->pts16, b->bounds, st, b->map()));
     return SX_OK;
 }
 

@@ -35,8 +35,7 @@ int unique_launch(const uint64_t* sorted, const uint32_t* vals, int64_t n, uint6
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
                   int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav, const float4* pts,
-                  void* box_ws, hipStream_t st, CloudMap cm = CloudMap());
-size_t box_ws_bytes(int64_t n_max);
+                  const float* bounds, hipStream_t st, CloudMap cm = CloudMap());
 size_t octree_ws_bytes(int64_t n_max);
 int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, const float* bounds,
                   const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
@@ -137,8 +136,7 @@ size_t index_layout(sx_index* ix, void* base) {
     ix->oct_aux = c.take<int2>(static_cast<size_t>(ix->oct_cap));
     ix->node_end = c.take<int32_t>(n);
     ix->ws_bbox = c.take<char>(sx::bbox_ws_bytes());
-    const size_t ws = std::max({sx::sort_ws_bytes(ix->cap), sx::scan_ws_bytes(ix->cap), sx::octree_ws_bytes(ix->cap),
-                                sx::box_ws_bytes(ix->cap)});
+    const size_t ws = std::max({sx::sort_ws_bytes(ix->cap), sx::scan_ws_bytes(ix->cap), sx::octree_ws_bytes(ix->cap)});
     ix->ws = c.take<char>(ws);
     return c.off;
 }
@@ -274,14 +272,14 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
     if (ix->root_mode == SX_ROOT_TEXTBOOK) {
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, ix->pn, ix->lc, ix->hll,
                                  ix->hlr, ix->parent, ix->node_end, ix->leaf_parent, nullptr, ix->trav, ix->pts16,
-                                 ix->ws, st));
+                                 ix->bounds, st));
     } else {
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_REFERENCE, ix->pn, ix->lc,
                                  ix->hll, ix->hlr, ix->parent, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                                  st));
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, nullptr, nullptr,
                                  nullptr, nullptr, nullptr, nullptr, ix->leaf_parent, ix->up, ix->trav, ix->pts16,
-                                 ix->ws, st));
+                                 ix->bounds, st));
     }
     // with the textbook root the octree and the traversal tree are one tree:
     // record each octree node's parent and point count for the features
