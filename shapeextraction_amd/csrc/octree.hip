@@ -65,6 +65,15 @@ __device__ __forceinline__ uint64_t code_prefix(uint64_t code, int level, int co
 // node set and numbering makeNodes + linkLeafNodes produce
 // (src/Octree.cu:65-161), with unused child slots left 0 as after the
 // reference's cudaMemset (src/Octree.cu:253).
+//
+// The gather walks two or three binary levels below a node, each level one
+// dependent read of the radix arrays; the descendants of a short node lie
+// within a few hundred indices, so the arrays of the block's nodes +-
+// kGatherHalo are staged in LDS first and only far descendants are read
+// from HBM.
+constexpr int kGatherBlock = 256;
+constexpr int kGatherHalo = 256;
+constexpr int kGatherWin = kGatherBlock + 2 * kGatherHalo;
 __device__ __forceinline__ void store_node(sx_otnode* nodes, int64_t cap, int o, const int32_t ch[8], float3 corner,
                                            float cell, int node_mask, int leaf_mask) {
     if (o < 0 || o >= cap) return;
@@ -78,7 +87,7 @@ __device__ __forceinline__ void store_node(sx_otnode* nodes, int64_t cap, int o,
     w[6] = make_int2(node_mask, leaf_mask);
 }
 
-__global__ void __launch_bounds__(256) oct_gather_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count,
+__global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count,
                                                          int64_t n_max, int code_bits, const float* __restrict__ bounds,
                                                          const uint8_t* __restrict__ pn,
                                                          const int32_t* __restrict__ left_child,
@@ -88,18 +97,41 @@ __global__ void __launch_bounds__(256) oct_gather_kernel(const uint64_t* __restr
                                                          const int32_t* __restrict__ counts, sx_otnode* nodes,
                                                          int64_t cap, int32_t* leaf_oct,
                                                          const int32_t* __restrict__ node_end, int2* aux) {
+    __shared__ uint64_t s_code[kGatherWin];
+    __shared__ int32_t s_lc[kGatherWin], s_cnt[kGatherWin], s_off[kGatherWin];
+    __shared__ uint8_t s_leaf[kGatherWin];  // bit 0: left child a leaf, bit 1: right child a leaf
     const int64_t U = scan_count(d_count, n_max);
     const int64_t M = U - 1;
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t blk0 = static_cast<int64_t>(blockIdx.x) * kGatherBlock;
+    if (blk0 >= M) return;  // whole block idle (before the barrier)
+    const int64_t w0 = blk0 - kGatherHalo > 0 ? blk0 - kGatherHalo : 0;
+    const int64_t w1 = blk0 + kGatherBlock + kGatherHalo < U ? blk0 + kGatherBlock + kGatherHalo : U;
+    for (int64_t x = w0 + threadIdx.x; x < w1; x += kGatherBlock) {
+        const int o = static_cast<int>(x - w0);
+        s_code[o] = codes[x];
+        if (x < M) {
+            s_lc[o] = left_child[x];
+            s_cnt[o] = counts[x];
+            s_off[o] = offsets[x];
+            s_leaf[o] = static_cast<uint8_t>((hll[x] != 0 ? 1 : 0) | (hlr[x] != 0 ? 2 : 0));
+        }
+    }
+    __syncthreads();
+    // radix arrays at x: from LDS inside [w0, w1) (node arrays below M), else HBM
+    auto in_win = [&](int64_t x) { return x >= w0 && x < w1; };
+    auto code_at = [&](int64_t x) { return in_win(x) ? s_code[x - w0] : codes[x]; };
+    auto cnt_at = [&](int64_t x) { return in_win(x) ? s_cnt[x - w0] : counts[x]; };
+    auto off_at = [&](int64_t x) { return in_win(x) ? s_off[x - w0] : offsets[x]; };
+    const int64_t i = blk0 + threadIdx.x;
     if (i >= M) return;
-    const int c = counts[i];
+    const int c = cnt_at(i);
     if (c <= 0) return;
     const float mn = bounds[0], range = bounds[2];
     const float fscale = static_cast<float>(axis_scale(code_bits));
     const int root_level = pn[0] / 3;
     const int D = pn[i] / 3;
-    const int base = offsets[i];
-    const uint64_t ci = codes[i];
+    const int base = off_at(i);
+    const uint64_t ci = code_at(i);
     if (aux) {
         // (parent octree node, points in the cell) of radix node i's chain:
         // each chain node's parent is the next one up; the top's parent is
@@ -132,19 +164,29 @@ __global__ void __launch_bounds__(256) oct_gather_kernel(const uint64_t* __restr
     int sp = 0;
     int cur = static_cast<int>(i);
     while (true) {
-        const int g = left_child[cur];
-        const bool leaf_side[2] = {hll[cur] != 0, hlr[cur] != 0};
+        int g;
+        uint32_t leafs;
+        if (in_win(cur) && cur < M) {
+            g = s_lc[cur - w0];
+            leafs = s_leaf[cur - w0];
+        } else {
+            g = left_child[cur];
+            leafs = (hll[cur] != 0 ? 1u : 0u) | (hlr[cur] != 0 ? 2u : 0u);
+        }
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             const int child = g + side;
-            if (leaf_side[side]) {
-                const int slot = static_cast<int>(code_prefix(codes[child], D + 1, code_bits) & 7);
+            if ((leafs >> side) & 1u) {
+                const int slot = static_cast<int>(code_prefix(code_at(child), D + 1, code_bits) & 7);
                 ch[slot] = child;
                 leaf_mask |= 1 << slot;
                 if (leaf_oct) leaf_oct[child] = base;
-            } else if (counts[child] > 0) {
-                const int slot = static_cast<int>(code_prefix(codes[child], D + 1, code_bits) & 7);
-                const int top = offsets[child] + counts[child] - 1;
+                continue;
+            }
+            const int cc = cnt_at(child);
+            if (cc > 0) {
+                const int slot = static_cast<int>(code_prefix(code_at(child), D + 1, code_bits) & 7);
+                const int top = off_at(child) + cc - 1;
                 ch[slot] = top;
                 node_mask |= 1 << slot;
                 if (aux && top < cap) aux[top].x = base;
@@ -187,7 +229,7 @@ int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
     void* scan_ws = c.take<char>(scan_ws_bytes(n_max));
     SX_TRY(device_scan(OctCountIn{pn, parent, d_count}, OctCountOut{offsets, counts}, d_count, n_max, d_oct_count,
                        scan_ws, st));
-    oct_gather_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(codes, d_count, n_max, code_bits, bounds, pn, left_child,
+    oct_gather_kernel<<<grid_for(n_max, kGatherBlock), kGatherBlock, 0, st>>>(codes, d_count, n_max, code_bits, bounds, pn, left_child,
                                                             hll, hlr, offsets, counts, nodes, cap, leaf_oct, node_end,
                                                             node_end ? aux : nullptr);
     SX_LAUNCH_CHECK("oct_gather_kernel");
