@@ -177,15 +177,25 @@ __global__ void __launch_bounds__(kRadix) onesweep_base_kernel(uint32_t* ghist) 
     h[threadIdx.x] = static_cast<uint32_t>(ex);
 }
 
-__global__ void __launch_bounds__(kSortBlock) onesweep_pass_kernel(const uint64_t* __restrict__ keys_in,
+// Pass tiles: 4096 keys over 512 threads (8 items each) - the same tile and
+// LDS as 256 x 16, with twice the waves per CU to hide the ranking's LDS
+// round trips and the look-back.
+constexpr int kPassBlock = 512;
+constexpr int kPassWaves = kPassBlock / kWave;
+constexpr int kPassItems = kSortTile / kPassBlock;
+constexpr int kPassSpan = kPassItems * kWave;  // contiguous keys per wave
+static_assert(kPassBlock >= kRadix, "one thread per digit");
+
+__global__ void __launch_bounds__(kPassBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) onesweep_pass_kernel(const uint64_t* __restrict__ keys_in,
                                                                    const uint32_t* __restrict__ vals_in, int64_t n,
                                                                    int shift, const uint32_t* __restrict__ gbase,
                                                                    uint32_t* status, uint32_t* tile_counter,
                                                                    uint64_t* __restrict__ keys_out,
                                                                    uint32_t* __restrict__ vals_out) {
-    __shared__ uint32_t cnt[kSortWaves][kRadix];
+    __shared__ uint32_t cnt[kPassWaves][kRadix];
     __shared__ uint32_t tile_start[kRadix];
     __shared__ uint32_t dest_base[kRadix];
+    __shared__ int digit_wsum[kRadix / kWave];
     // the tile in digit order: keys first, then (same buffer) values
     __shared__ uint64_t s_buf[kSortTile];
     __shared__ uint8_t s_digit[kSortTile];
@@ -199,16 +209,19 @@ __global__ void __launch_bounds__(kSortBlock) onesweep_pass_kernel(const uint64_
     const uint32_t tile = s_tile;
     const int64_t tile_base = static_cast<int64_t>(tile) * kSortTile;
     const uint64_t lt = (1ull << lane) - 1ull;
-    const int64_t span = tile_base + static_cast<int64_t>(w) * kWaveSpan;
-    uint64_t k[kSortItems];
-    uint32_t v[kSortItems];
-    uint32_t rank[kSortItems];
+    const int64_t span = tile_base + static_cast<int64_t>(w) * kPassSpan;
+    uint64_t k[kPassItems];
+    uint32_t v[kPassItems];
+    uint32_t rank[kPassItems];
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
+    for (int r = 0; r < kPassItems; ++r) {
         const int64_t i = span + r * kWave + lane;
         const bool valid = i < n;
         k[r] = valid ? keys_in[i] : ~0ull;
         v[r] = valid ? (vals_in ? vals_in[i] : static_cast<uint32_t>(i)) : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kPassItems; ++r) {
         const uint32_t d = static_cast<uint32_t>(k[r] >> shift) & (kRadix - 1);
         const uint64_t m = match_digit(d);
         const uint32_t c = cnt[w][d];
@@ -216,47 +229,58 @@ __global__ void __launch_bounds__(kSortBlock) onesweep_pass_kernel(const uint64_
         if (((m >> lane) >> 1) == 0) cnt[w][d] = c + static_cast<uint32_t>(__popcll(m));
     }
     __syncthreads();
-    // per digit (thread tid = digit): wave offsets, tile count, look-back
-    uint32_t run = 0;
+    // per digit (thread tid < 256 = digit): wave offsets, tile count, look-back
+    const bool digit_thread = tid < kRadix;
+    uint32_t run = 0, excl = 0;
+    if (digit_thread) {
 #pragma unroll
-    for (int wv = 0; wv < kSortWaves; ++wv) {
-        const uint32_t c = cnt[wv][tid];
-        cnt[wv][tid] = run;
-        run += c;
-    }
-    uint32_t* my = status + static_cast<int64_t>(tile) * kRadix + tid;
-    uint32_t excl = 0;
-    if (tile == 0) {
-        __hip_atomic_store(my, kFlagPrefix | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-        __hip_atomic_store(my, kFlagAgg | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int64_t t = static_cast<int64_t>(tile) - 1;
-        uint32_t spins = 0;
-        while (true) {
-            const uint32_t s = __hip_atomic_load(status + t * kRadix + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((s & ~kCountMask) == 0) {
-                // Tile t holds an earlier ticket, so it is running and will
-                // publish; the bound only guards against a broken invariant.
-                if (++spins > (1u << 22)) {
-                    atomicOr(tile_counter + kMaxPasses, 1u);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            excl += s & kCountMask;
-            if ((s & ~kCountMask) == kFlagPrefix) break;
-            --t;
+        for (int wv = 0; wv < kPassWaves; ++wv) {
+            const uint32_t c = cnt[wv][tid];
+            cnt[wv][tid] = run;
+            run += c;
         }
-        __hip_atomic_store(my, kFlagPrefix | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t* my = status + static_cast<int64_t>(tile) * kRadix + tid;
+        if (tile == 0) {
+            __hip_atomic_store(my, kFlagPrefix | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(my, kFlagAgg | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t t = static_cast<int64_t>(tile) - 1;
+            uint32_t spins = 0;
+            while (true) {
+                const uint32_t st = __hip_atomic_load(status + t * kRadix + tid, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+                if ((st & ~kCountMask) == 0) {
+                    // Tile t holds an earlier ticket, so it is running and will
+                    // publish; the bound only guards against a broken invariant.
+                    if (++spins > (1u << 22)) {
+                        atomicOr(tile_counter + kMaxPasses, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += st & kCountMask;
+                if ((st & ~kCountMask) == kFlagPrefix) break;
+                --t;
+            }
+            __hip_atomic_store(my, kFlagPrefix | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
-    int total;
-    const uint32_t start = static_cast<uint32_t>(block_excl_sum(static_cast<int>(run), &total));
-    tile_start[tid] = start;
-    dest_base[tid] = gbase[tid] + excl - start;  // output index = dest_base[d] + LDS slot
+    // exclusive scan of the tile's digit counts (waves 0-3 hold the digits)
+    const int incl = digit_thread ? wave_incl_sum(static_cast<int>(run)) : 0;
+    if (digit_thread && lane == kWave - 1) digit_wsum[w] = incl;
+    __syncthreads();
+    if (digit_thread) {
+        int before = 0;
+#pragma unroll
+        for (int i = 0; i < kRadix / kWave; ++i) before += (i < w) ? digit_wsum[i] : 0;
+        const uint32_t start = static_cast<uint32_t>(before + incl) - run;
+        tile_start[tid] = start;
+        dest_base[tid] = gbase[tid] + excl - start;  // output index = dest_base[d] + LDS slot
+    }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
+    for (int r = 0; r < kPassItems; ++r) {
         const uint32_t d = static_cast<uint32_t>(k[r] >> shift) & (kRadix - 1);
         const uint32_t pos = tile_start[d] + cnt[w][d] + rank[r];
         s_buf[pos] = k[r];
@@ -265,8 +289,8 @@ __global__ void __launch_bounds__(kSortBlock) onesweep_pass_kernel(const uint64_
     __syncthreads();
     const int64_t valid = n - tile_base < kSortTile ? n - tile_base : kSortTile;
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
-        const int j = r * kSortBlock + tid;
+    for (int r = 0; r < kPassItems; ++r) {
+        const int j = r * kPassBlock + tid;
         if (j < valid) {
             const uint64_t key = s_buf[j];
             const uint32_t d = static_cast<uint32_t>(key >> shift) & (kRadix - 1);
@@ -277,11 +301,11 @@ __global__ void __launch_bounds__(kSortBlock) onesweep_pass_kernel(const uint64_
     __syncthreads();
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_buf);
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) s_vals[rank[r]] = v[r];
+    for (int r = 0; r < kPassItems; ++r) s_vals[rank[r]] = v[r];
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
-        const int j = r * kSortBlock + tid;
+    for (int r = 0; r < kPassItems; ++r) {
+        const int j = r * kPassBlock + tid;
         if (j < valid) vals_out[dest_base[s_digit[j]] + static_cast<uint32_t>(j)] = s_vals[j];
     }
 }
@@ -323,7 +347,7 @@ int onesweep_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n,
         const bool to_out = ((passes - 1 - p) % 2) == 0;
         uint64_t* dst_k = to_out ? keys_out : keys_tmp;
         uint32_t* dst_v = to_out ? vals_out : vals_tmp;
-        onesweep_pass_kernel<<<static_cast<unsigned>(tiles), kSortBlock, 0, st>>>(
+        onesweep_pass_kernel<<<static_cast<unsigned>(tiles), kPassBlock, 0, st>>>(
             src_k, src_v, n, 8 * p, ghist + p * kRadix, status + static_cast<size_t>(p) * kRadix * tiles,
             counters + p, dst_k, dst_v);
         SX_LAUNCH_CHECK("onesweep_pass_kernel");
