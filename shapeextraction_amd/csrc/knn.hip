@@ -753,7 +753,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
 // ---------------------------------------------------------------------------
 constexpr int kGroup = 16;
 constexpr int kGroupsPerWave = kWave / kGroup;
-constexpr int kGroupStack = 160;
+constexpr int kGroupStack = 112;  // 20 KB of LDS per block: 8 blocks (8 waves/SIMD) per CU
 constexpr int kGroupRanges = 2 * kGroup;  // one expansion adds at most two per lane
 
 // Group-local collectives; gs = 16 * (group within the wave).
@@ -844,7 +844,8 @@ struct GroupList {
 };
 
 template <int K>
-__global__ void __launch_bounds__(kKnnBlock) knn_group_kernel(const float4* __restrict__ pts,
+// Latency-bound (one dependent record load per level): eight waves per SIMD.
+__global__ void __launch_bounds__(kKnnBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) knn_group_kernel(const float4* __restrict__ pts,
                                                               const TravNode* __restrict__ nodes,
                                                               const int32_t* __restrict__ d_counts, int64_t q_start,
                                                               int2* __restrict__ queue,
