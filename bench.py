@@ -274,6 +274,8 @@ def main() -> int:
     }
     if cls is not None:
         line["per_class_angular_error_deg"] = per_class_errors(sx, ix, nn, normals, cls, dirs, k, st)
+    else:
+        line["normals_vs_fp64"] = normal_errors(sx, ix, nn, normals, k, st)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(w, n, k)
@@ -281,6 +283,19 @@ def main() -> int:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
     return 0
+
+
+def normal_errors(sx, ix, nn, normals, k, st) -> dict:
+    """The metric's precision gate, checked after the timed steps: the timed
+    FP32 normals against the FP64 path on the same neighbours (FP64 covariance
+    + FP64 Jacobi on the GPU, itself pinned to the CPU FP64 PCA in the tests),
+    angle up to sign."""
+    n32 = normals.numpy().astype(np.float64)
+    n64 = ix.normals(nn, k, precision=sx.SX_PREC_FP64, stream=st.handle).numpy().astype(np.float64)
+    err = np.degrees(np.arccos(np.clip(np.abs(np.sum(n32 * n64, axis=1)), 0.0, 1.0)))
+    return {"reference": "FP64 covariance + FP64 Jacobi (GPU), same neighbours", "median_deg": float(np.median(err)),
+            "p99_deg": float(np.percentile(err, 99)), "max_deg": float(err.max()),
+            "gate_p99_le_1deg": bool(np.percentile(err, 99) <= 1.0)}
 
 
 def per_class_errors(sx, ix, nn, normals, cls, dirs, k, st) -> dict:
