@@ -266,6 +266,7 @@ def main() -> int:
             "traffic": traffic,
             "algorithmic_bytes_per_launch": kb,
         },
+        "valu_roofline": valu_roofline(head, knn_avg, n, k),
         "pipeline_roofline": {
             "bytes_per_point": pipeline_bytes_per_point(8, 8, k, r),
             "achieved_GBs": pipe_bytes / (ms_per_step / 1000.0) / 1e9,
@@ -283,6 +284,23 @@ def main() -> int:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
     return 0
+
+
+def valu_roofline(mode: str, knn_ms: float, n: int, k: int):
+    """The kNN's actual ceiling: VALU issue.  Wave instructions per call from
+    the committed SQ_INSTS_VALU profile of this workload, divided by the kNN
+    time measured in this run, against 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles
+    per wave64 instruction."""
+    f = ROOT / "profiles" / "knn_scene8m_valu.json"
+    if not (f.exists() and n == 8_388_608 and k == 16):
+        return None
+    prof = json.loads(f.read_text())
+    per_call = prof["valu_wave_instructions_per_call"][mode]
+    achieved = per_call / (knn_ms / 1000.0)
+    peak = prof["peak_wave_instructions_per_s"]
+    return {"kernel": "sx_knn (tile + deferred-query kernels)", "bound": "valu", "achieved": achieved, "peak": peak,
+            "unit": "wave-instructions/s", "frac": achieved / peak, "instructions_per_call": per_call,
+            "source": "profiles/knn_scene8m_valu.json (SQ_INSTS_VALU)"}
 
 
 def normal_errors(sx, ix, nn, normals, k, st) -> dict:

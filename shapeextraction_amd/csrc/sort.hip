@@ -184,6 +184,7 @@ constexpr int kPassBlock = 512;
 constexpr int kPassWaves = kPassBlock / kWave;
 constexpr int kPassItems = kSortTile / kPassBlock;
 constexpr int kPassSpan = kPassItems * kWave;  // contiguous keys per wave
+constexpr int kLookWin = 4;                    // predecessor tiles read per look-back round trip
 static_assert(kPassBlock >= kRadix, "one thread per digit");
 
 __global__ void __launch_bounds__(kPassBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) onesweep_pass_kernel(const uint64_t* __restrict__ keys_in,
@@ -229,9 +230,11 @@ __global__ void __launch_bounds__(kPassBlock) __attribute__((amdgpu_waves_per_eu
         if (((m >> lane) >> 1) == 0) cnt[w][d] = c + static_cast<uint32_t>(__popcll(m));
     }
     __syncthreads();
-    // per digit (thread tid < 256 = digit): wave offsets, tile count, look-back
+    // per digit (thread tid < 256 = digit): wave offsets and the tile's count,
+    // published at once so that later tiles can look back past this one
     const bool digit_thread = tid < kRadix;
     uint32_t run = 0, excl = 0;
+    uint32_t* my = status + static_cast<int64_t>(tile) * kRadix + tid;
     if (digit_thread) {
 #pragma unroll
         for (int wv = 0; wv < kPassWaves; ++wv) {
@@ -239,46 +242,23 @@ __global__ void __launch_bounds__(kPassBlock) __attribute__((amdgpu_waves_per_eu
             cnt[wv][tid] = run;
             run += c;
         }
-        uint32_t* my = status + static_cast<int64_t>(tile) * kRadix + tid;
-        if (tile == 0) {
-            __hip_atomic_store(my, kFlagPrefix | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(my, kFlagAgg | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int64_t t = static_cast<int64_t>(tile) - 1;
-            uint32_t spins = 0;
-            while (true) {
-                const uint32_t st = __hip_atomic_load(status + t * kRadix + tid, __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_AGENT);
-                if ((st & ~kCountMask) == 0) {
-                    // Tile t holds an earlier ticket, so it is running and will
-                    // publish; the bound only guards against a broken invariant.
-                    if (++spins > (1u << 22)) {
-                        atomicOr(tile_counter + kMaxPasses, 1u);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += st & kCountMask;
-                if ((st & ~kCountMask) == kFlagPrefix) break;
-                --t;
-            }
-            __hip_atomic_store(my, kFlagPrefix | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        __hip_atomic_store(my, (tile == 0 ? kFlagPrefix : kFlagAgg) | run, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     }
     // exclusive scan of the tile's digit counts (waves 0-3 hold the digits)
     const int incl = digit_thread ? wave_incl_sum(static_cast<int>(run)) : 0;
     if (digit_thread && lane == kWave - 1) digit_wsum[w] = incl;
     __syncthreads();
+    uint32_t start = 0;
     if (digit_thread) {
         int before = 0;
 #pragma unroll
         for (int i = 0; i < kRadix / kWave; ++i) before += (i < w) ? digit_wsum[i] : 0;
-        const uint32_t start = static_cast<uint32_t>(before + incl) - run;
+        start = static_cast<uint32_t>(before + incl) - run;
         tile_start[tid] = start;
-        dest_base[tid] = gbase[tid] + excl - start;  // output index = dest_base[d] + LDS slot
     }
     __syncthreads();
+    // the tile in digit order in LDS (needs only tile-local offsets)
 #pragma unroll
     for (int r = 0; r < kPassItems; ++r) {
         const uint32_t d = static_cast<uint32_t>(k[r] >> shift) & (kRadix - 1);
@@ -286,6 +266,46 @@ __global__ void __launch_bounds__(kPassBlock) __attribute__((amdgpu_waves_per_eu
         s_buf[pos] = k[r];
         rank[r] = pos;
     }
+    // look-back: kLookWin earlier tiles per round trip (independent loads),
+    // nearest first; t is the nearest tile not yet added
+    if (digit_thread && tile != 0) {
+        int64_t t = static_cast<int64_t>(tile) - 1;
+        uint32_t spins = 0;
+        bool found = false;
+        while (!found) {
+            uint32_t st[kLookWin];
+#pragma unroll
+            for (int lw = 0; lw < kLookWin; ++lw)
+                st[lw] = t - lw >= 0 ? __hip_atomic_load(status + (t - lw) * kRadix + tid, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)
+                                     : kFlagPrefix;  // before tile 0: an exclusive prefix of 0
+            int used = 0;
+            bool waiting = false;
+#pragma unroll
+            for (int lw = 0; lw < kLookWin; ++lw) {
+                if (found || waiting) continue;
+                if ((st[lw] & ~kCountMask) == 0) {
+                    waiting = true;  // not published yet: reload from here
+                    continue;
+                }
+                excl += st[lw] & kCountMask;
+                ++used;
+                if ((st[lw] & ~kCountMask) == kFlagPrefix) found = true;
+            }
+            t -= used;
+            if (waiting) {
+                // Tile t holds an earlier ticket, so it is running and will
+                // publish; the bound only guards against a broken invariant.
+                if (++spins > (1u << 22)) {
+                    atomicOr(tile_counter + kMaxPasses, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __hip_atomic_store(my, kFlagPrefix | (excl + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (digit_thread) dest_base[tid] = gbase[tid] + excl - start;  // output index = dest_base[d] + LDS slot
     __syncthreads();
     const int64_t valid = n - tile_base < kSortTile ? n - tile_base : kSortTile;
 #pragma unroll
