@@ -135,6 +135,8 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
         const int64_t gx = base + x;
         return (gx >= w0 && gx < w1) ? s_codes[gx - w0] : codes[gx];
     };
+    // i + t * d for the direction d = +-1, without a 64-bit multiply
+    auto step = [&](int64_t from, int64_t t, int d) -> int64_t { return d > 0 ? from + t : from - t; };
     int64_t A = 0, B = 0, G = 0;  // the node's range [A, B] and split G (slots)
     bool hl = false, hr = false;
     if (active) {
@@ -156,25 +158,24 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
             const int dmin = prefix_len(ci, code(i - d), code_bits);
             int64_t lmax = 2;
             while (true) {
-                const int64_t jj = i + lmax * d;
+                const int64_t jj = step(i, lmax, d);
                 if (jj < 0 || jj > N || prefix_len(ci, code(jj), code_bits) <= dmin) break;
                 lmax *= 2;
             }
             const int64_t cutoff = (d == -1) ? i : N - i;
             for (int64_t t = lmax / 2; t >= 1; t /= 2)
-                if (l + t <= cutoff && prefix_len(ci, code(i + (l + t) * d), code_bits) > dmin) l += t;
+                if (l + t <= cutoff && prefix_len(ci, code(step(i, l + t, d)), code_bits) > dmin) l += t;
         }
-        const int64_t j = i + l * d;
+        const int64_t j = step(i, l, d);
         const int dnode = prefix_len(ci, code(j), code_bits);
         int64_t s = 0;
         const int64_t scut = (d == -1) ? i - 1 : N - i - 1;
         const int lg = ceil_log2_i64(l);
         for (int e = 1; e <= lg; ++e) {
-            const int64_t div = 1ll << e;
-            const int64_t t = (l + div - 1) / div;
-            if (s + t <= scut && prefix_len(ci, code(i + (s + t) * d), code_bits) > dnode) s += t;
+            const int64_t t = (l + (1ll << e) - 1) >> e;  // ceil(l / 2^e)
+            if (s + t <= scut && prefix_len(ci, code(step(i, s + t, d)), code_bits) > dnode) s += t;
         }
-        const int64_t gamma = i + s * d + (d < 0 ? d : 0);
+        const int64_t gamma = d > 0 ? i + s : i - s - 1;
         const int64_t a = i < j ? i : j, b = i < j ? j : i;
         hl = (a == gamma);
         hr = (b == gamma + 1);
@@ -240,8 +241,17 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
     }
     const float fscale = static_cast<float>(axis_scale(code_bits));
     Box3 bl, br;
+    // a leaf child's point: from LDS in the block, else from HBM (the empty asm
+    // keeps the two loads apart; merged they would become one generic load)
+    auto leaf_point = [&](int64_t x) -> float4 {
+        const int o = static_cast<int>(x - blk0);
+        float4 p = s_pts[o < 0 ? 0 : (o >= kKarrasBlock ? kKarrasBlock - 1 : o)];
+        asm volatile("" : "+v"(p.x), "+v"(p.y), "+v"(p.z));
+        if (x < blk0 || x >= blk1) p = pts[x];
+        return p;
+    };
     if (hl) {
-        bl.set(G >= blk0 && G < blk1 ? s_pts[G - blk0] : pts[G]);
+        bl.set(leaf_point(G));
     } else if (!far_l) {
         bl.set(s_lo[G - blk0], s_hi[G - blk0]);
     } else {
@@ -249,7 +259,7 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
         bl = cell_box(ca, prefix_len(ca, code(G - base), code_bits), code_bits, mn, range, fscale);
     }
     if (hr) {
-        br.set(G + 1 >= blk0 && G + 1 < blk1 ? s_pts[G + 1 - blk0] : pts[G + 1]);
+        br.set(leaf_point(G + 1));
     } else if (!far_r) {
         br.set(s_lo[G + 1 - blk0], s_hi[G + 1 - blk0]);
     } else {
