@@ -117,11 +117,34 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
         }
     }
     __syncthreads();
-    // radix arrays at x: from LDS inside [w0, w1) (node arrays below M), else HBM
+    // radix arrays at x: from LDS inside [w0, w1) (node arrays below M), else
+    // HBM.  The LDS read is unconditional (index clamped) and the empty asm
+    // keeps it apart from the HBM read: merged they would become one generic
+    // load on every access.
     auto in_win = [&](int64_t x) { return x >= w0 && x < w1; };
-    auto code_at = [&](int64_t x) { return in_win(x) ? s_code[x - w0] : codes[x]; };
-    auto cnt_at = [&](int64_t x) { return in_win(x) ? s_cnt[x - w0] : counts[x]; };
-    auto off_at = [&](int64_t x) { return in_win(x) ? s_off[x - w0] : offsets[x]; };
+    const int win = static_cast<int>(w1 - w0);
+    auto clampw = [&](int64_t x) {
+        const int64_t o = x - w0;
+        return static_cast<int>(o < 0 ? 0 : (o >= win ? win - 1 : o));
+    };
+    auto code_at = [&](int64_t x) {
+        uint64_t v = s_code[clampw(x)];
+        asm volatile("" : "+v"(v));
+        if (!in_win(x)) v = codes[x];
+        return v;
+    };
+    auto cnt_at = [&](int64_t x) {
+        int v = s_cnt[clampw(x)];
+        asm volatile("" : "+v"(v));
+        if (!in_win(x)) v = counts[x];
+        return v;
+    };
+    auto off_at = [&](int64_t x) {
+        int v = s_off[clampw(x)];
+        asm volatile("" : "+v"(v));
+        if (!in_win(x)) v = offsets[x];
+        return v;
+    };
     const int64_t i = blk0 + threadIdx.x;
     if (i >= M) return;
     const int c = cnt_at(i);
@@ -166,10 +189,10 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
     while (true) {
         int g;
         uint32_t leafs;
-        if (in_win(cur) && cur < M) {
-            g = s_lc[cur - w0];
-            leafs = s_leaf[cur - w0];
-        } else {
+        g = s_lc[clampw(cur)];
+        leafs = s_leaf[clampw(cur)];
+        asm volatile("" : "+v"(g), "+v"(leafs));
+        if (!in_win(cur)) {  // internal nodes are below M, so inside the window they are staged
             g = left_child[cur];
             leafs = (hll[cur] != 0 ? 1u : 0u) | (hlr[cur] != 0 ? 2u : 0u);
         }
