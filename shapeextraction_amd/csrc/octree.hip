@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
         const int slot = static_cast<int>(code_prefix(ci, L + 1, code_bits) & 7);
         ch[slot] = base + j - 1;
         store_node(nodes, cap, base + j, ch, cell_corner(code_prefix(ci, L, code_bits), L, code_bits, mn, range, fscale),
-                   __fdiv_rn(range, static_cast<float>(1 << (L - root_level))), 1 << slot, 0);
+                   ldexpf(range, root_level - L), 1 << slot, 0);
     }
     // bottom node: gather children through count-0 radix descendants
 #pragma unroll
@@ -227,7 +227,7 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
         cell = range;
     } else {
         corner = cell_corner(code_prefix(ci, D, code_bits), D, code_bits, mn, range, fscale);
-        cell = __fdiv_rn(range, static_cast<float>(1 << (D - root_level)));
+        cell = ldexpf(range, root_level - D);
     }
     store_node(nodes, cap, base, ch, corner, cell, node_mask, leaf_mask);
 }

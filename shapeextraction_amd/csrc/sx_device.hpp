@@ -64,9 +64,24 @@ __device__ __forceinline__ uint32_t quantize(float v, float min_coord, float ran
     return static_cast<uint32_t>(s);
 }
 
+// (float)raw / bitscale, correctly rounded.  For the two Morton scales
+// (2^21 - 1 and 2^10 - 1) one reciprocal product plus one Newton correction,
+// q = fma(fma(-q0, d, x), 1/d, q0) with q0 = x * (1/d), equals the IEEE
+// quotient for every raw in [0, 2^21); that equality is checked exhaustively
+// on the host by tests/test_arith.py.  Any other scale takes the full division.
+__device__ __forceinline__ float raw_over_scale(uint32_t raw, float fscale) {
+    const float x = static_cast<float>(raw);
+    float r;
+    if (fscale == 2097151.0f) r = 0x1.000008p-21f;
+    else if (fscale == 1023.0f) r = 0x1.00401p-10f;
+    else return __fdiv_rn(x, fscale);
+    const float q0 = __fmul_rn(x, r);
+    return __fmaf_rn(__fmaf_rn(-q0, fscale, x), r, q0);
+}
+
 // ((float)raw / bitscale) * range + min, uncontracted (src/MortonUtils.cu:25-27).
 __device__ __forceinline__ float dequantize(uint32_t raw, float fscale, float range, float min_coord) {
-    return __fadd_rn(__fmul_rn(__fdiv_rn(static_cast<float>(raw), fscale), range), min_coord);
+    return __fadd_rn(__fmul_rn(raw_over_scale(raw, fscale), range), min_coord);
 }
 
 // Length of the common prefix of two codes of `code_bits` significant bits
