@@ -130,10 +130,19 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
     if (in_range && i == 0 && parent) parent[g] = -1;
     if (in_range && i == 0 && up) up[g].x = -1;
     const bool active = in_range && i < N;
-    // code of the cloud's leaf x (local index)
+    // code of the cloud's leaf x (local index): from LDS inside [w0, w1), else
+    // from HBM.  The LDS read is unconditional (index clamped) and the empty
+    // asm keeps it apart from the HBM read: merged they would become one
+    // generic load on every search step.
+    const uint64_t win = static_cast<uint64_t>(w1 - w0);
     auto code = [&](int64_t x) -> uint64_t {
         const int64_t gx = base + x;
-        return (gx >= w0 && gx < w1) ? s_codes[gx - w0] : codes[gx];
+        const uint64_t o = static_cast<uint64_t>(gx - w0);
+        const bool in_win = o < win;
+        uint64_t v = s_codes[in_win ? static_cast<int>(o) : 0];
+        asm volatile("" : "+v"(v));
+        if (!in_win) v = codes[gx];
+        return v;
     };
     // i + t * d for the direction d = +-1, without a 64-bit multiply
     auto step = [&](int64_t from, int64_t t, int d) -> int64_t { return d > 0 ? from + t : from - t; };
