@@ -77,6 +77,12 @@ __device__ __forceinline__ Box3 cell_box(uint64_t code, int p, int code_bits, fl
     return bx;
 }
 
+// For codes of code_bits significant bits: prefix_len(a, b) > d exactly when
+// (a ^ b) < longer_than(d): the first differing bit lies below the top d + 1.
+__device__ __forceinline__ uint64_t longer_than(int d, int code_bits) {
+    return d >= code_bits ? 0ull : (1ull << (code_bits - 1 - d));
+}
+
 __device__ __forceinline__ int ceil_log2_i64(int64_t l) {
     return l <= 1 ? 0 : 64 - __clzll(static_cast<long long>(l - 1));
 }
@@ -164,25 +170,26 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
             // so both rules use the full range there.
             l = (root_mode == SX_ROOT_REFERENCE && N >= 2) ? N - 1 : N;
         } else {
-            const int dmin = prefix_len(ci, code(i - d), code_bits);
+            const uint64_t tmin = longer_than(prefix_len(ci, code(i - d), code_bits), code_bits);
             int64_t lmax = 2;
             while (true) {
                 const int64_t jj = step(i, lmax, d);
-                if (jj < 0 || jj > N || prefix_len(ci, code(jj), code_bits) <= dmin) break;
+                if (jj < 0 || jj > N || (ci ^ code(jj)) >= tmin) break;
                 lmax *= 2;
             }
             const int64_t cutoff = (d == -1) ? i : N - i;
             for (int64_t t = lmax / 2; t >= 1; t /= 2)
-                if (l + t <= cutoff && prefix_len(ci, code(step(i, l + t, d)), code_bits) > dmin) l += t;
+                if (l + t <= cutoff && (ci ^ code(step(i, l + t, d))) < tmin) l += t;
         }
         const int64_t j = step(i, l, d);
         const int dnode = prefix_len(ci, code(j), code_bits);
         int64_t s = 0;
         const int64_t scut = (d == -1) ? i - 1 : N - i - 1;
         const int lg = ceil_log2_i64(l);
+        const uint64_t tnode = longer_than(dnode, code_bits);
         for (int e = 1; e <= lg; ++e) {
             const int64_t t = (l + (1ll << e) - 1) >> e;  // ceil(l / 2^e)
-            if (s + t <= scut && prefix_len(ci, code(step(i, s + t, d)), code_bits) > dnode) s += t;
+            if (s + t <= scut && (ci ^ code(step(i, s + t, d))) < tnode) s += t;
         }
         const int64_t gamma = d > 0 ? i + s : i - s - 1;
         const int64_t a = i < j ? i : j, b = i < j ? j : i;
