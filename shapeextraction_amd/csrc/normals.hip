@@ -142,7 +142,9 @@ __global__ void __launch_bounds__(kNormalsBlock) normals_kernel(const float4* __
                                                                 int64_t n_q, int precision, float vx, float vy,
                                                                 float vz, float* __restrict__ normals,
                                                                 uint8_t* __restrict__ flags, CloudMap cm) {
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    // XCD-contiguous block order: neighbours of a block's queries are mostly
+    // the neighbouring blocks' points, read from the same L2
+    const int64_t t = static_cast<int64_t>(xcd_block()) * blockDim.x + threadIdx.x;
     if (t >= n_q) return;
     const int total = d_counts[0];
     const int64_t qi = q_start + t;

@@ -102,7 +102,9 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
     __shared__ uint8_t s_leaf[kGatherWin];  // bit 0: left child a leaf, bit 1: right child a leaf
     const int64_t U = scan_count(d_count, n_max);
     const int64_t M = U - 1;
-    const int64_t blk0 = static_cast<int64_t>(blockIdx.x) * kGatherBlock;
+    // XCD-contiguous block order: a block's halo is its neighbours' nodes, read
+    // from the same L2
+    const int64_t blk0 = static_cast<int64_t>(xcd_block()) * kGatherBlock;
     if (blk0 >= M) return;  // whole block idle (before the barrier)
     const int64_t w0 = blk0 - kGatherHalo > 0 ? blk0 - kGatherHalo : 0;
     const int64_t w1 = blk0 + kGatherBlock + kGatherHalo < U ? blk0 + kGatherBlock + kGatherHalo : U;

@@ -115,7 +115,9 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
     // g: slot; the cloud holding it spans slots [base, base + U) and its
     // internal node i (local) is stored at slot base + i
     const int64_t total = scan_count(d_count, n_max);
-    const int64_t blk0 = static_cast<int64_t>(blockIdx.x) * kKarrasBlock;
+    // XCD-contiguous block order: a block's halo is its neighbours' keys, read
+    // from the same L2
+    const int64_t blk0 = static_cast<int64_t>(xcd_block()) * kKarrasBlock;
     if (blk0 >= total) return;  // whole block idle (before every barrier)
     const int tid = static_cast<int>(threadIdx.x);
     const int64_t blk1 = blk0 + kKarrasBlock < total ? blk0 + kKarrasBlock : total;  // the block's slots [blk0, blk1)
