@@ -48,6 +48,21 @@ __device__ __forceinline__ float3 cell_corner(uint64_t prefix, int level, int co
                        dequantize(compact3(c >> 2), fscale, range, mn));
 }
 
+// The same corner from the raw axis coordinates of any code inside the cell:
+// the level-L prefix keeps the top L bits of each axis (B = code_bits / 3 bits
+// per axis), so clearing the low B - L raw bits gives exactly the raw
+// coordinates cell_corner decodes.
+struct RawXYZ {
+    uint32_t x, y, z;
+};
+__device__ __forceinline__ RawXYZ raw_xyz(uint64_t code) { return {compact3(code), compact3(code >> 1), compact3(code >> 2)}; }
+__device__ __forceinline__ float3 cell_corner_raw(RawXYZ r, int level, int code_bits, float mn, float range,
+                                                  float fscale) {
+    const uint32_t m = ~0u << (code_bits / 3 - level);
+    return make_float3(dequantize(r.x & m, fscale, range, mn), dequantize(r.y & m, fscale, range, mn),
+                       dequantize(r.z & m, fscale, range, mn));
+}
+
 __device__ __forceinline__ uint64_t code_prefix(uint64_t code, int level, int code_bits) {
     const int sh = code_bits - 3 * level;
     return sh >= 64 ? 0ull : (code >> sh);
@@ -171,6 +186,7 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
         }
     }
     int32_t ch[8];
+    const RawXYZ ri = raw_xyz(ci);
     // chain nodes above the bottom
     for (int j = 1; j < c; ++j) {
         const int L = D - j;
@@ -178,7 +194,7 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
         for (int s = 0; s < 8; ++s) ch[s] = 0;
         const int slot = static_cast<int>(code_prefix(ci, L + 1, code_bits) & 7);
         ch[slot] = base + j - 1;
-        store_node(nodes, cap, base + j, ch, cell_corner(code_prefix(ci, L, code_bits), L, code_bits, mn, range, fscale),
+        store_node(nodes, cap, base + j, ch, cell_corner_raw(ri, L, code_bits, mn, range, fscale),
                    ldexpf(range, root_level - L), 1 << slot, 0);
     }
     // bottom node: gather children through count-0 radix descendants
@@ -228,7 +244,7 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
         corner = cell_corner(code_prefix(codes[0], root_level, code_bits), root_level, code_bits, mn, range, fscale);
         cell = range;
     } else {
-        corner = cell_corner(code_prefix(ci, D, code_bits), D, code_bits, mn, range, fscale);
+        corner = cell_corner_raw(ri, D, code_bits, mn, range, fscale);
         cell = ldexpf(range, root_level - D);
     }
     store_node(nodes, cap, base, ch, corner, cell, node_mask, leaf_mask);
