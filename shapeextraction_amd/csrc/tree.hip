@@ -10,8 +10,8 @@
 //          constructTree (src/RadixTree.cu:99-194); root rule selectable.
 // boxes:   the Karras thread of a node also writes its traversal record:
 //          tight child boxes of subtrees inside the thread block bottom-up in
-//          LDS, Morton-cell boxes for the others (no atomics or fences
-//          between threads).
+//          LDS, Morton-cell boxes for the others (no global atomics or
+//          fences between threads).
 #include <hip/hip_runtime.h>
 
 #include "sx_internal.hpp"
@@ -63,17 +63,24 @@ struct Box3 {
 // prefix, i.e. every point of a radix node whose prefix length is p.  The
 // corners are decoded like the points (dequantize is monotone in the raw
 // coordinate), so the box holds every decoded point of the cell.
+// The low code_bits - p code bits are free; axis a owns ceil((free - a) / 3) of
+// them, the low bits of its raw coordinate, so the corners are the code's raw
+// coordinates with those bits cleared and set.
 __device__ __forceinline__ Box3 cell_box(uint64_t code, int p, int code_bits, float mn, float range, float fscale) {
     const int free_bits = code_bits - p;
-    const uint64_t mask = free_bits >= 64 ? ~0ull : ((1ull << free_bits) - 1ull);
-    const uint64_t c0 = code & ~mask, c1 = code | mask;
+    auto axis_mask = [&](int a) -> uint32_t {
+        const int n = free_bits > a ? (free_bits - a + 2) / 3 : 0;
+        return (1u << n) - 1u;
+    };
+    const uint32_t mx = axis_mask(0), my = axis_mask(1), mz = axis_mask(2);
+    const uint32_t rx = compact3(code), ry = compact3(code >> 1), rz = compact3(code >> 2);
     Box3 bx;
-    bx.lx = dequantize(compact3(c0), fscale, range, mn);
-    bx.ly = dequantize(compact3(c0 >> 1), fscale, range, mn);
-    bx.lz = dequantize(compact3(c0 >> 2), fscale, range, mn);
-    bx.hx = dequantize(compact3(c1), fscale, range, mn);
-    bx.hy = dequantize(compact3(c1 >> 1), fscale, range, mn);
-    bx.hz = dequantize(compact3(c1 >> 2), fscale, range, mn);
+    bx.lx = dequantize(rx & ~mx, fscale, range, mn);
+    bx.ly = dequantize(ry & ~my, fscale, range, mn);
+    bx.lz = dequantize(rz & ~mz, fscale, range, mn);
+    bx.hx = dequantize(rx | mx, fscale, range, mn);
+    bx.hy = dequantize(ry | my, fscale, range, mn);
+    bx.hz = dequantize(rz | mz, fscale, range, mn);
     return bx;
 }
 
@@ -112,6 +119,7 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
     __shared__ float4 s_pts[kKarrasBlock];  // points of the block's own slots
     __shared__ float4 s_lo[kKarrasBlock], s_hi[kKarrasBlock];  // box of each in-block node
     __shared__ int s_done[kKarrasBlock];
+    __shared__ int s_pending[3];  // per bottom-up step (rotating): some in-block box still missing
     // g: slot; the cloud holding it spans slots [base, base + U) and its
     // internal node i (local) is stored at slot base + i
     const int64_t total = scan_count(d_count, n_max);
@@ -127,6 +135,7 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
     if (trav) {
         if (blk0 + tid < blk1) s_pts[tid] = pts[blk0 + tid];
         s_done[tid] = 0;
+        if (tid < 3) s_pending[tid] = 0;
     }
     __syncthreads();
     const int64_t g = blk0 + tid;
@@ -225,26 +234,37 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
     // ---- boxes of the in-block nodes, bottom-up: a node whose range lies in
     // [blk0, blk1) has both children there (leaves, or in-block nodes).
     const bool local = active && A >= blk0 && B < blk1;
+    // One barrier per step.  A box is written before its flag (release) and
+    // read after it (acquire), so a box finished in this step may already be
+    // used in it.  Pending flags rotate over three slots: slot it % 3 is set
+    // before step it's barrier and read after it by every thread alike; it is
+    // cleared during step it + 2, when every read of step it has passed the
+    // barrier of step it + 1, and set again only after that step's barrier.
+    // Every step finishes at least the lowest missing box, so the loop ends
+    // within kKarrasBlock steps.
     bool done = !local;
+    int slot = 0;
 #pragma unroll 1
     for (int it = 0; it < kKarrasBlock; ++it) {
-        bool now = false;
         if (!done) {
             const int cl = static_cast<int>(G - blk0), cr = cl + 1;
-            if ((hl || s_done[cl]) && (hr || s_done[cr])) {
+            const bool rl = hl || __hip_atomic_load(&s_done[cl], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+            const bool rr = hr || __hip_atomic_load(&s_done[cr], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+            if (rl && rr) {
                 const float4 l1 = hl ? s_pts[cl] : s_lo[cl], h1 = hl ? s_pts[cl] : s_hi[cl];
                 const float4 l2 = hr ? s_pts[cr] : s_lo[cr], h2 = hr ? s_pts[cr] : s_hi[cr];
                 s_lo[tid] = make_float4(fminf(l1.x, l2.x), fminf(l1.y, l2.y), fminf(l1.z, l2.z), 0.0f);
                 s_hi[tid] = make_float4(fmaxf(h1.x, h2.x), fmaxf(h1.y, h2.y), fmaxf(h1.z, h2.z), 0.0f);
-                now = true;
+                __hip_atomic_store(&s_done[tid], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                done = true;
             }
         }
-        __syncthreads();  // the boxes of this step are written before their flags
-        if (now) {
-            s_done[tid] = 1;
-            done = true;
-        }
-        if (!__syncthreads_or(!done)) break;
+        const int next = slot == 2 ? 0 : slot + 1;
+        if (!done) s_pending[slot] = 1;
+        if (tid == 0) s_pending[next] = 0;
+        __syncthreads();
+        if (s_pending[slot] == 0) break;
+        slot = next;
     }
     if (!active) return;
     // the record of node g: child boxes with (first leaf, split, last leaf)
