@@ -187,7 +187,7 @@ def main() -> int:
             "build_ms": el(0, 1), "radius_ms": el(1, 2), "knn_ms": el(2, 3), "normals_ms": el(3, 4),
             "step_ms": el(0, 4),
             "reruns": ix.knn_reruns(reset=True, stream=st.handle) / max(args.steps, 1),
-            "nn": nn.numpy(), "normals": normals.numpy(),
+            "nn": nn.numpy(st.handle), "normals": normals.numpy(st.handle),
         }
         assert np.isfinite(res["normals"]).all()
         log(f"{mode}: {res['ms_per_step']:.2f} ms/step (build {np.median(res['build_ms']):.2f}, radius "
@@ -290,7 +290,9 @@ def valu_roofline(mode: str, knn_ms: float, n: int, k: int):
     """The kNN's actual ceiling: VALU issue.  Wave instructions per call from
     the committed SQ_INSTS_VALU profile of this workload, divided by the kNN
     time measured in this run, against 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles
-    per wave64 instruction."""
+    per wave64 instruction: the single-rate issue peak, which the kNN's
+    compares and selects run at (FP32 add/mul/fma and integer add/and/xor issue
+    at about twice it on gfx950, so a mix rich in those can exceed 1.0)."""
     f = ROOT / "profiles" / "knn_scene8m_valu.json"
     if not (f.exists() and n == 8_388_608 and k == 16):
         return None
@@ -300,7 +302,7 @@ def valu_roofline(mode: str, knn_ms: float, n: int, k: int):
     peak = prof["peak_wave_instructions_per_s"]
     return {"kernel": "sx_knn (tile + deferred-query kernels)", "bound": "valu", "achieved": achieved, "peak": peak,
             "unit": "wave-instructions/s", "frac": achieved / peak, "instructions_per_call": per_call,
-            "source": "profiles/knn_scene8m_valu.json (SQ_INSTS_VALU)"}
+            "peak_note": prof.get("peak_note", ""), "source": "profiles/knn_scene8m_valu.json (SQ_INSTS_VALU)"}
 
 
 def normal_errors(sx, ix, nn, normals, k, st) -> dict:
@@ -308,8 +310,8 @@ def normal_errors(sx, ix, nn, normals, k, st) -> dict:
     FP32 normals against the FP64 path on the same neighbours (FP64 covariance
     + FP64 Jacobi on the GPU, itself pinned to the CPU FP64 PCA in the tests),
     angle up to sign."""
-    n32 = normals.numpy().astype(np.float64)
-    n64 = ix.normals(nn, k, precision=sx.SX_PREC_FP64, stream=st.handle).numpy().astype(np.float64)
+    n32 = normals.numpy(st.handle).astype(np.float64)
+    n64 = ix.normals(nn, k, precision=sx.SX_PREC_FP64, stream=st.handle).numpy(st.handle).astype(np.float64)
     err = np.degrees(np.arccos(np.clip(np.abs(np.sum(n32 * n64, axis=1)), 0.0, 1.0)))
     return {"reference": "FP64 covariance + FP64 Jacobi (GPU), same neighbours", "median_deg": float(np.median(err)),
             "p99_deg": float(np.percentile(err, 99)), "max_deg": float(err.max()),
@@ -321,8 +323,8 @@ def per_class_errors(sx, ix, nn, normals, cls, dirs, k, st) -> dict:
     covariance + FP64 Jacobi on the GPU, itself pinned to the CPU FP64 PCA in
     the tests).  A line's normal is only defined up to a rotation about the
     line, so for lines the error is the deviation from perpendicular."""
-    n32 = normals.numpy().astype(np.float64)
-    n64 = ix.normals(nn, k, precision=sx.SX_PREC_FP64, stream=st.handle).numpy().astype(np.float64)
+    n32 = normals.numpy(st.handle).astype(np.float64)
+    n64 = ix.normals(nn, k, precision=sx.SX_PREC_FP64, stream=st.handle).numpy(st.handle).astype(np.float64)
     rep = ix.fetch()["rep"]
     ucls, udir = cls[rep], dirs[rep].astype(np.float64)
     vs64 = np.degrees(np.arccos(np.clip(np.abs(np.sum(n32 * n64, axis=1)), 0.0, 1.0)))
@@ -395,7 +397,7 @@ def bench_batched(sx, args, dev) -> int:
     b.deferred(reset=True, stream=st.handle)
     graph.launch(st.handle)
     deferred = b.deferred(reset=True, stream=st.handle)
-    nrm = normals.numpy()
+    nrm = normals.numpy(st.handle)
     valid = np.zeros(S, bool)
     for c in range(n_clouds):
         valid[tab["base"][c]:tab["base"][c] + tab["size"][c]] = True

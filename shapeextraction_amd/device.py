@@ -42,8 +42,14 @@ class DeviceArray:
         return d
 
     def numpy(self, stream=None) -> np.ndarray:
+        """Host copy.  With a stream, the copy follows that stream's work.
+        Without one, every stream's work is waited for first: the library's
+        streams are non-blocking, so the null stream alone would not order
+        the copy after a kernel launched on one of them."""
         out = np.empty(self.shape, self.dtype)
         if self.nbytes:
+            if stream is None:
+                _lib.call("sx_device_sync")
             _lib.call("sx_memcpy_dtoh", out.ctypes.data, self.ptr, self.nbytes, stream)
         return out
 
@@ -74,10 +80,13 @@ class DeviceArray:
 
 
 def fetch(ptr: int, count: int, dtype, stream=None) -> np.ndarray:
-    """Copy `count` elements of `dtype` from a raw device pointer."""
+    """Copy `count` elements of `dtype` from a raw device pointer (without a
+    stream, after every stream's work, as DeviceArray.numpy)."""
     dt = np.dtype(dtype)
     out = np.empty(int(count), dt)
     if count:
+        if stream is None:
+            _lib.call("sx_device_sync")
         _lib.call("sx_memcpy_dtoh", out.ctypes.data, ptr, out.nbytes, stream)
     return out
 

@@ -370,6 +370,22 @@ def test_normals_fp64_mode(gpu, O):
     assert p99 < 1e-3
 
 
+def test_host_copy_waits_for_nonblocking_stream(gpu):
+    """A host copy without a stream follows work launched on a library stream
+    (those streams are non-blocking, so the null stream alone would not order
+    the copy): the slow FP64 normals of 1M points, read at once, are whole."""
+    x, y, z = gpu.gen_scene(2, 1_000_000)
+    st = gpu.Stream()
+    ix = gpu.Index(len(x), 63).build(x, y, z)
+    nn = ix.knn(16)
+    for _ in range(3):
+        n64 = ix.normals(nn, 16, precision=gpu.SX_PREC_FP64, stream=st.handle)
+        early = n64.numpy()
+        st.sync()
+        assert np.array_equal(early, n64.numpy(st.handle))
+        assert np.all(np.abs(np.linalg.norm(early, axis=1) - 1.0) < 1e-3)
+
+
 def test_normals_degenerate_classes(gpu, O):
     x, y, z, cls, d = gpu.gen_degenerate(5, 600_000)
     ix = gpu.Index(len(x), 63).build(x, y, z)
