@@ -272,7 +272,14 @@ constexpr int kTileRanges = kRangeFlush - 1 + 2 * kWave;  // one expansion adds 
 // 32-neighbourhoods.
 template <int K>
 constexpr int kWindow = K >= 32 ? 96 : 32;
-constexpr int kSegBudget = 8192;                         // candidates one segment may stream before deferring
+// Candidates one segment may stream before its lanes are deferred.  The
+// check runs after each flush, so a segment that overruns has already
+// streamed about this many candidates to every lane; the deferred-query
+// kernel searches such queries for less.  Measured on the 8M scene, k = 16:
+// plain kNN 4.84 / 4.65 / 4.38 / 4.17 / 4.25 / 4.58 ms at 8192 / 6144 / 4096
+// / 2048 / 1536 / 1024 (flat from 1792 to 2048).
+template <int K>
+constexpr int kSegBudget = 2048 * K / 16;
 constexpr float kSegGap = 8.0f;                          // segment break: gap > kSegGap x mean radius
 constexpr float kRadiusStretch = 2.25f;                  // learned: accept up to 1.5x the predicted radius
 constexpr float kLearnedCap = 1.21f;                     // learned: radius capped at 1.1x the predicted one
@@ -712,7 +719,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
                                        q, in_seg, tau, top, seen, s_cand[w]);
             nr = 0;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            if (seen > kSegBudget) overflow = true;
+            if (seen > kSegBudget<K>) overflow = true;
             if (overflow || sp == 0) break;
         }
         if (overflow) lost = lost || in_seg;  // the segment's lanes are deferred
