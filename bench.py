@@ -289,19 +289,21 @@ def main() -> int:
 def valu_roofline(mode: str, knn_ms: float, n: int, k: int):
     """The kNN's actual ceiling: VALU issue.  Wave instructions per call from
     the committed SQ_INSTS_VALU profile of this workload, divided by the kNN
-    time measured in this run, against 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles
-    per wave64 instruction: the single-rate issue peak, which the kNN's
-    compares and selects run at (FP32 add/mul/fma and integer add/and/xor issue
-    at about twice it on gfx950, so a mix rich in those can exceed 1.0)."""
+    time measured in this run.  `peak` is the dual issue rate measured for
+    two-source FP32/integer ops (the ceiling for any instruction mix);
+    `frac_single_rate` compares against 256 CUs x 4 SIMDs x 2.4 GHz / 4
+    cycles, the rate of the compares and selects that dominate the kNN."""
     f = ROOT / "profiles" / "knn_scene8m_valu.json"
     if not (f.exists() and n == 8_388_608 and k == 16):
         return None
     prof = json.loads(f.read_text())
     per_call = prof["valu_wave_instructions_per_call"][mode]
     achieved = per_call / (knn_ms / 1000.0)
-    peak = prof["peak_wave_instructions_per_s"]
+    single = prof["peak_wave_instructions_per_s"]
+    peak = prof["peak_dual_rate_wave_instructions_per_s"]
     return {"kernel": "sx_knn (tile + deferred-query kernels)", "bound": "valu", "achieved": achieved, "peak": peak,
-            "unit": "wave-instructions/s", "frac": achieved / peak, "instructions_per_call": per_call,
+            "unit": "wave-instructions/s", "frac": achieved / peak, "peak_single_rate": single,
+            "frac_single_rate": achieved / single, "instructions_per_call": per_call,
             "peak_note": prof.get("peak_note", ""), "source": "profiles/knn_scene8m_valu.json (SQ_INSTS_VALU)"}
 
 
