@@ -52,11 +52,31 @@ int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn
 // counts[] slots
 enum { kBSlots = 0, kBUnique = 1, kBBad = 2, kBDeferred = 3, kBQueue = 4 };
 
+// Cloud of each element in code order: a binary search of the cloud offsets,
+// staged in LDS when they fit (each block handles kCloudKeysItems elements
+// per thread, so the staging is amortised); a gather of per-input cloud ids
+// would touch one cache line per element instead.
+constexpr int kCloudKeysLds = 2048;
+constexpr int kCloudKeysItems = 8;
 __global__ void __launch_bounds__(256) cloud_keys_kernel(const uint32_t* __restrict__ perm, int64_t n,
                                                          const int64_t* __restrict__ off, int n_clouds,
                                                          uint64_t* __restrict__ keys) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) keys[i] = static_cast<uint64_t>(cloud_of(off, n_clouds, perm[i]));
+    __shared__ int64_t s_off[kCloudKeysLds];
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x * kCloudKeysItems + threadIdx.x;
+    if (n_clouds <= kCloudKeysLds) {
+        for (int c = threadIdx.x; c < n_clouds; c += blockDim.x) s_off[c] = off[c];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kCloudKeysItems; ++r) {
+            const int64_t i = i0 + static_cast<int64_t>(r) * blockDim.x;
+            if (i < n) keys[i] = static_cast<uint64_t>(cloud_of(s_off, n_clouds, perm[i]));
+        }
+    } else {
+        for (int r = 0; r < kCloudKeysItems; ++r) {
+            const int64_t i = i0 + static_cast<int64_t>(r) * blockDim.x;
+            if (i < n) keys[i] = static_cast<uint64_t>(cloud_of(off, n_clouds, perm[i]));
+        }
+    }
 }
 
 __global__ void __launch_bounds__(256) gather_codes_kernel(const uint64_t* __restrict__ codes,
@@ -235,7 +255,8 @@ int batch_build(sx_batch* b, const float* x, const float* y, const float* z, hip
     SX_TRY(sx::seg_encode_launch(x, y, z, n, b->off, nc, b->bounds, cb, b->codes_in, st));
     // (cloud, code, input index) order: by code, then stably by cloud
     SX_TRY(sx::sort_pairs_launch(b->codes_in, nullptr, n, cb, b->codes_a, b->perm_a, b->ws, b->sort_algo, st));
-    sx::cloud_keys_kernel<<<sx::grid_for(n, 256), 256, 0, st>>>(b->perm_a, n, b->off, nc, b->keys_c);
+    sx::cloud_keys_kernel<<<sx::grid_for(n, 256 * sx::kCloudKeysItems), 256, 0, st>>>(b->perm_a, n, b->off, nc,
+                                                                                      b->keys_c);
     SX_LAUNCH_CHECK("cloud_keys_kernel");
     SX_TRY(sx::sort_pairs_launch(b->keys_c, b->perm_a, n, cloud_bits(nc), b->keys_s, b->perm, b->ws, b->sort_algo,
                                  st));

@@ -284,23 +284,34 @@ int seg_bbox_launch(const float* x, const float* y, const float* z, const int64_
     return SX_OK;
 }
 
+constexpr int kSegEncodeItems = 8;
 __global__ void __launch_bounds__(kEncodeBlock) seg_encode_kernel(const float* __restrict__ x,
                                                                   const float* __restrict__ y,
                                                                   const float* __restrict__ z, int64_t n,
                                                                   const int64_t* __restrict__ off, int n_clouds,
                                                                   const float* __restrict__ bounds, int code_bits,
                                                                   uint64_t* __restrict__ codes) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = cloud_of(off, n_clouds, i);
+    // kSegEncodeItems points per thread; the clouds of the block's first and
+    // last point are found once per block (uniform: scalar loads), and a
+    // point searches only when the block straddles a cloud boundary
+    const int64_t span = static_cast<int64_t>(blockDim.x) * kSegEncodeItems;
+    const int64_t b0 = static_cast<int64_t>(blockIdx.x) * span;
+    const int64_t b1 = (b0 + span < n ? b0 + span : n) - 1;
+    const int c0 = cloud_of(off, n_clouds, b0), c1 = cloud_of(off, n_clouds, b1);
     const float fscale = static_cast<float>(axis_scale(code_bits));
-    codes[i] = encode_one(x[i], y[i], z[i], bounds[4 * c], bounds[4 * c + 2], fscale);
+#pragma unroll
+    for (int r = 0; r < kSegEncodeItems; ++r) {
+        const int64_t i = b0 + static_cast<int64_t>(r) * blockDim.x + threadIdx.x;
+        if (i >= n) break;
+        const int c = c0 == c1 ? c0 : c0 + cloud_of(off + c0, c1 - c0 + 1, i);
+        codes[i] = encode_one(x[i], y[i], z[i], bounds[4 * c], bounds[4 * c + 2], fscale);
+    }
 }
 
 int seg_encode_launch(const float* x, const float* y, const float* z, int64_t n, const int64_t* off, int n_clouds,
                       const float* bounds, int code_bits, uint64_t* codes, hipStream_t st) {
     if (n <= 0) return SX_OK;
-    seg_encode_kernel<<<grid_for(n, kEncodeBlock), kEncodeBlock, 0, st>>>(x, y, z, n, off, n_clouds, bounds,
+    seg_encode_kernel<<<grid_for(n, kEncodeBlock * kSegEncodeItems), kEncodeBlock, 0, st>>>(x, y, z, n, off, n_clouds, bounds,
                                                                             code_bits, codes);
     SX_LAUNCH_CHECK("seg_encode_kernel");
     return SX_OK;
