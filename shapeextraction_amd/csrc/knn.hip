@@ -534,11 +534,12 @@ __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, co
     }
 }
 
-// Six waves per SIMD for K <= 16 (the compiler otherwise settles a few
-// registers above the 80 that allow it); larger lists keep their natural
-// allocation.
+// Seven waves per SIMD for K <= 16: 72 VGPRs, a few values spilled to scratch
+// (measured on the 8M scene, plain kNN: 4.11-4.12 ms at seven waves, 4.15-4.19
+// at six with 80 VGPRs, 4.17 at eight with 64); larger lists keep their
+// natural allocation.
 template <int K, bool kLearned>
-__global__ void __launch_bounds__(kKnnBlock) __attribute__((amdgpu_waves_per_eu(K <= 16 ? 6 : 1, 8)))
+__global__ void __launch_bounds__(kKnnBlock) __attribute__((amdgpu_waves_per_eu(K <= 16 ? 7 : 1, 8)))
 knn_tile_kernel(const float4* __restrict__ pts,
                                                              const TravNode* __restrict__ nodes,
                                                              const int32_t* __restrict__ d_counts, int64_t q_start,
