@@ -536,10 +536,12 @@ __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, co
 
 // Seven waves per SIMD for K <= 16: 72 VGPRs, a few values spilled to scratch
 // (measured on the 8M scene, plain kNN: 4.11-4.12 ms at seven waves, 4.15-4.19
-// at six with 80 VGPRs, 4.17 at eight with 64); larger lists keep their
-// natural allocation.
+// at six with 80 VGPRs, 4.17 at eight with 64).  Five for K = 24 (4M
+// degenerate scene: 4.44 ms against 4.81 at the natural 107 VGPRs, four
+// waves).  K = 32 keeps its natural allocation (123 VGPRs; forced to five
+// waves it spills heavily: 35.4 ms against 22.1 on the 16M mixture).
 template <int K, bool kLearned>
-__global__ void __launch_bounds__(kKnnBlock) __attribute__((amdgpu_waves_per_eu(K <= 16 ? 7 : 1, 8)))
+__global__ void __launch_bounds__(kKnnBlock) __attribute__((amdgpu_waves_per_eu(K <= 16 ? 7 : (K <= 24 ? 5 : 1), 8)))
 knn_tile_kernel(const float4* __restrict__ pts,
                                                              const TravNode* __restrict__ nodes,
                                                              const int32_t* __restrict__ d_counts, int64_t q_start,
