@@ -41,6 +41,8 @@ extern "C" {
 #define SX_ERANGE 4   /* a data-dependent size exceeded the capacity given          */
 #define SX_EDATA 5    /* invalid input data (non-finite coordinates)                */
 #define SX_ESTATE 6   /* object used before it was built                            */
+#define SX_EINTERNAL 7 /* a device-side invariant broke (a sort look-back gave up);
+                          the results of that build are invalid                    */
 
 /* Karras root rule.  The reference gives the root l = N-1 with N = n_unique-1
  * (src/RadixTree.cu:128-131 with :353-362), so its root spans leaves
@@ -142,6 +144,17 @@ int sx_morton_decode(const uint64_t* d_codes, const int32_t* d_count, int64_t n_
  * (src/RadixTree.cu:316-335), which sorts keys only. */
 int sx_sort_pairs(const uint64_t* d_keys_in, const uint32_t* d_vals_in, int64_t n, int key_bits,
                   uint64_t* d_keys_out, uint32_t* d_vals_out, void* ws, size_t* ws_bytes, void* stream);
+/* Status of the last sx_sort_pairs that used workspace `ws` for n keys
+ * (synchronises `stream`): SX_EINTERNAL when one of its single-pass tiles
+ * gave up waiting for an earlier tile's prefix (the forward-progress
+ * invariant broke and the output is invalid), else SX_OK.  No reference
+ * counterpart (cub reports nothing either, src/RadixTree.cu:316-335). */
+int sx_sort_check(const void* ws, int64_t n, void* stream);
+/* Spins a sort tile waits for an earlier tile before giving up (default
+ * 2^22).  A process-wide setting read at launch; a test hook for the
+ * give-up path above (0 gives up at the first wait, and every kernel still
+ * exits). */
+int sx_sort_set_spin_limit(uint32_t spins);
 
 /* Duplicate removal over sorted codes: d_unique[0..U) are the distinct codes,
  * d_rep[u] the value (input index) of the first element of run u,
@@ -217,9 +230,26 @@ int sx_index_destroy(sx_index* index);
 /* Builds the index from device SoA coordinates; asynchronous on `stream`. */
 int sx_index_build(sx_index* index, const float* d_x, const float* d_y, const float* d_z, int64_t n,
                    void* stream);
-/* Orders `stream` after the last sx_index_build (whatever stream it ran on),
- * synchronises it, validates counts and fills *view. */
+/* Builds the index from n strictly increasing Morton codes (device, at most
+ * the index's code bits wide) of points under the cube [min_coord,
+ * min_coord + range]: no bounds, encode, sort or duplicate removal; the
+ * permutation and representatives are the identity, the points are the
+ * decoded codes.  The path of the reference's Octree host-data constructor
+ * (src/Octree.cu:179-192), whose points are the decoded unique points of an
+ * earlier build in Morton order.  Codes that break the order are reported by
+ * sx_index_get_view (SX_EDATA) and leave the index empty. */
+int sx_index_build_codes(sx_index* index, const uint64_t* d_codes, int64_t n, float min_coord, float range,
+                         void* stream);
+/* Orders `stream` after the last build (whatever stream it ran on),
+ * synchronises it, validates counts and fills *view.  SX_EINTERNAL when the
+ * build's sort failed (see sx_sort_check). */
 int sx_index_get_view(sx_index* index, sx_index_view* view, void* stream);
+/* Rows in input order: output row i (i < n_input) = row unique_of_input[i] of
+ * d_rows, rows of row_bytes (a multiple of 4) per unique point as written by
+ * sx_normals / sx_knn for the whole index.  Input points that share a code
+ * share its row.  The reference returns unique points in Morton order only
+ * (src/main.cu:119-120); this is the scatter back to the caller's order. */
+int sx_index_to_input_order(sx_index* index, const void* d_rows, int64_t row_bytes, void* d_out, void* stream);
 /* Bucket size of the kNN descent: a child holding at most `bucket` points is
  * scanned linearly instead of descended (default 16; tuning knob, no effect
  * on results).  No reference counterpart (the reference descends to single
@@ -231,6 +261,14 @@ int sx_index_set_algorithms(sx_index* index, int knn_algo, int sort_algo);
  * or the seeded search of SX_KNN_RADIUS) and searched again one lane per
  * query, since the last reset (synchronises `stream`). */
 int sx_index_knn_reruns(sx_index* index, int64_t* reruns, int reset, void* stream);
+
+/* Query entry points (sx_knn, sx_knn_points, sx_normals, sx_estimate_normals,
+ * sx_radius_*) order their stream after the last build.  Query positions are
+ * checked against the unique count once a view has read it, else against the
+ * input count; rows of positions at or beyond the unique count are written as
+ * -1 (neighbours) or NaN (normals).  An index is used from one stream at a
+ * time: its calls share the deferred-query queue and counters, so calls that
+ * overlap on different streams race. */
 
 /* Exact k nearest neighbours of the index points [q_start, q_start+n_q)
  * among all index points, k in {8, 16, 24, 32}.  Row q of d_out holds k

@@ -550,10 +550,11 @@ int64_t orc_octree_cells(const uint64_t* codes, int64_t U, int code_bits, int ro
     return c;
 }
 
-// Exact kNN by brute force: for queries q in [q0, q0+nq) of pts (3*U floats),
-// the k points with the smallest (distance2, index); the query itself
-// included; rows padded with their first entry when U < k.
-void orc_knn_brute(const float* pts, int64_t U, int64_t q0, int64_t nq, int k, int32_t* out) {
+// Exact kNN by brute force: for each of the nq query points qpts (3 floats
+// each), the k points of pts (3*U floats) with the smallest (distance2,
+// index), distance2 in the uncontracted order of Point::distance2
+// (src/PointCloud.hpp:30-35); rows padded with their first entry when U < k.
+static void knn_brute_rows(const float* pts, int64_t U, const float* qpts, int64_t nq, int k, int32_t* out) {
     std::vector<float> X(U), Y(U), Z(U);
     for (int64_t i = 0; i < U; ++i) { X[i] = pts[3 * i]; Y[i] = pts[3 * i + 1]; Z[i] = pts[3 * i + 2]; }
 #pragma omp parallel
@@ -562,7 +563,7 @@ void orc_knn_brute(const float* pts, int64_t U, int64_t q0, int64_t nq, int k, i
         std::vector<uint64_t> heap;
 #pragma omp for schedule(dynamic, 16)
         for (int64_t t = 0; t < nq; ++t) {
-            const float* q = pts + 3 * (q0 + t);
+            const float* q = qpts + 3 * t;
             const float qx = q[0], qy = q[1], qz = q[2];
             heap.clear();
             float thr = std::numeric_limits<float>::infinity();
@@ -587,6 +588,17 @@ void orc_knn_brute(const float* pts, int64_t U, int64_t q0, int64_t nq, int k, i
             write_row(heap, k, out + t * k);
         }
     }
+}
+
+// Index points [q0, q0+nq) as the queries: the query itself is included.
+void orc_knn_brute(const float* pts, int64_t U, int64_t q0, int64_t nq, int k, int32_t* out) {
+    knn_brute_rows(pts, U, pts + 3 * q0, nq, k, out);
+}
+
+// Arbitrary query points (the query API of Octree::knnSearch<k>,
+// src/Octree.hpp:214-242).
+void orc_knn_brute_points(const float* pts, int64_t U, const float* queries, int64_t nq, int k, int32_t* out) {
+    knn_brute_rows(pts, U, queries, nq, k, out);
 }
 
 // Exact kNN over a uniform grid (independent of any tree): points bucketed by

@@ -54,6 +54,7 @@ def lib() -> C.CDLL:
             "orc_octree": (I64, [P, I64, I, F, F, P, P, P, P, P, P, I64]),
             "orc_octree_cells": (I64, [P, I64, I, I, P]),
             "orc_knn_brute": (None, [P, I64, I64, I64, I, P]),
+            "orc_knn_brute_points": (None, [P, I64, P, I64, I, P]),
             "orc_knn_grid": (None, [P, I64, P, I64, I, P]),
             "orc_knn_reference": (None, [P, P, I64, I64, I64, I, P]),
             "orc_pca_f64": (None, [P, P, I64, I, P, P]),
@@ -175,6 +176,15 @@ def knn_brute(points, k: int, q_start: int = 0, n_q: int | None = None) -> np.nd
     n_q = U - q_start if n_q is None else n_q
     out = np.empty((n_q, k), np.int32)
     lib().orc_knn_brute(_p(pts), U, q_start, n_q, k, _p(out))
+    return out
+
+
+def knn_brute_points(points, queries, k: int) -> np.ndarray:
+    """Exact kNN of arbitrary query points (n x 3) among points, by brute force."""
+    pts = _c(points, np.float32).reshape(-1, 3)
+    q = _c(queries, np.float32).reshape(-1, 3)
+    out = np.empty((len(q), k), np.int32)
+    lib().orc_knn_brute_points(_p(pts), len(pts), _p(q), len(q), k, _p(out))
     return out
 
 

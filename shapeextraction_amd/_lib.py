@@ -12,7 +12,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().with_name("libshapes_hip.so")
 HEADER_PATH = Path(__file__).resolve().parent.parent / "include" / "shapes_c.h"
 
-SX_OK, SX_EINVAL, SX_ENOMEM, SX_EHIP, SX_ERANGE, SX_EDATA, SX_ESTATE = range(7)
+SX_OK, SX_EINVAL, SX_ENOMEM, SX_EHIP, SX_ERANGE, SX_EDATA, SX_ESTATE, SX_EINTERNAL = range(8)
 SX_ROOT_TEXTBOOK, SX_ROOT_REFERENCE = 0, 1
 SX_KNN_PLAIN, SX_KNN_RADIUS = 0, 1
 SX_PREC_FP32, SX_PREC_FP64 = 0, 1
@@ -27,6 +27,7 @@ STATUS_NAMES = {
     SX_ERANGE: "SX_ERANGE",
     SX_EDATA: "SX_EDATA",
     SX_ESTATE: "SX_ESTATE",
+    SX_EINTERNAL: "SX_EINTERNAL",
 }
 
 
@@ -132,6 +133,8 @@ SIGNATURES = {
     "sx_morton_encode": (I, [P, P, P, I64, P, I, P, P]),
     "sx_morton_decode": (I, [P, P, I64, P, I, P, P]),
     "sx_sort_pairs": (I, [P, P, I64, I, P, P, P, PSZ, P]),
+    "sx_sort_check": (I, [P, I64, P]),
+    "sx_sort_set_spin_limit": (I, [C.c_uint32]),
     "sx_unique": (I, [P, P, I64, P, P, P, P, P, PSZ, P]),
     "sx_build_radix_tree": (I, [P, P, I64, I, I, P, P, P, P, P, P, P, P]),
     "sx_build_octree": (I, [P, P, I64, I, P, P, P, P, P, P, P, I64, P, P, PSZ, P]),
@@ -139,7 +142,9 @@ SIGNATURES = {
     "sx_index_create": (I, [C.POINTER(C.c_void_p), I64, I, I, F]),
     "sx_index_destroy": (I, [P]),
     "sx_index_build": (I, [P, P, P, P, I64, P]),
+    "sx_index_build_codes": (I, [P, P, I64, F, F, P]),
     "sx_index_get_view": (I, [P, C.POINTER(IndexView), P]),
+    "sx_index_to_input_order": (I, [P, P, I64, P, P]),
     "sx_index_set_bucket": (I, [P, I]),
     "sx_index_set_algorithms": (I, [P, I, I]),
     "sx_index_knn_reruns": (I, [P, C.POINTER(C.c_int64), I, P]),

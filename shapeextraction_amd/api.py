@@ -147,14 +147,27 @@ def morton_decode(codes: DeviceArray, bounds: DeviceArray, code_bits: int, count
     return pts
 
 
-def sort_pairs(keys: DeviceArray, key_bits: int, vals: DeviceArray | None = None, stream=None):
+def sort_pairs(keys: DeviceArray, key_bits: int, vals: DeviceArray | None = None, stream=None,
+               check: bool = False):
+    """Stable sort of (key, value) pairs.  check=True also waits for the sort
+    and raises SxError(SX_EINTERNAL) if one of its tiles gave up waiting."""
     n = len(keys)
     ko = DeviceArray((n,), np.uint64)
     vo = DeviceArray((n,), np.uint32)
     vp = vals.ptr if vals is not None else None
     ws = _ws("sx_sort_pairs", keys.ptr, vp, n, key_bits, ko.ptr, vo.ptr, stream=stream)
     call("sx_sort_pairs", keys.ptr, vp, n, key_bits, ko.ptr, vo.ptr, ws.ptr, C.byref(C.c_size_t(ws.nbytes)), stream)
+    if check:
+        call("sx_sort_check", ws.ptr, n, stream)
     return ko, vo
+
+
+def set_sort_spin_limit(spins: int) -> None:
+    """Spins a sort tile waits for an earlier one before giving up (test hook)."""
+    call("sx_sort_set_spin_limit", int(spins))
+
+
+DEFAULT_SORT_SPIN_LIMIT = 1 << 22
 
 
 def unique(sorted_keys: DeviceArray, vals: DeviceArray | None, stream=None):
@@ -249,6 +262,36 @@ class Index:
         call("sx_index_build", self.handle, dx.ptr, dy.ptr, dz.ptr, len(dx), stream)
         self._view = None
         return self
+
+    def build_codes(self, codes, min_coord: float, range_: float, stream=None) -> "Index":
+        """Index of strictly increasing Morton codes under the cube
+        [min_coord, min_coord + range_] (no sort, no duplicate removal)."""
+        dc = _dev(codes, np.uint64)
+        self._inputs = (dc,)
+        call("sx_index_build_codes", self.handle, dc.ptr, len(dc), float(min_coord), float(range_), stream)
+        self._view = None
+        return self
+
+    def to_input_order(self, rows: DeviceArray, stream=None) -> DeviceArray:
+        """Rows per unique point (normals, neighbour rows) gathered into input order."""
+        v = self._view or self.view(stream)
+        row_shape = rows.shape[1:]
+        out = DeviceArray((int(v.n_input),) + tuple(row_shape), rows.dtype)
+        row_bytes = rows.nbytes // max(rows.shape[0], 1)
+        call("sx_index_to_input_order", self.handle, rows.ptr, row_bytes, out.ptr, stream)
+        return out
+
+    def knn_points(self, queries, k: int, out: DeviceArray | None = None, stream=None) -> DeviceArray:
+        """Exact k nearest index points of arbitrary query points (n x 3 float32)."""
+        if isinstance(queries, DeviceArray):
+            dq = queries
+        else:
+            dq = DeviceArray.from_numpy(np.ascontiguousarray(queries, np.float32).reshape(-1, 3))
+        n_q = dq.shape[0]
+        out = out if out is not None else DeviceArray((n_q, k), np.int32)
+        self._queries = dq
+        call("sx_knn_points", self.handle, dq.ptr, n_q, k, out.ptr, stream)
+        return out
 
     def view(self, stream=None) -> _lib.IndexView:
         v = _lib.IndexView()
@@ -461,7 +504,7 @@ class BatchGraph:
 
 __all__ = [
     "Index", "Batch", "BatchGraph", "Workload", "WORKLOADS", "OTNODE_DTYPE", "bbox", "morton_encode", "morton_decode", "sort_pairs",
-    "unique", "radix_tree", "octree", "check_counts", "gen_sphere", "gen_scene", "gen_gmm_scene",
+    "unique", "radix_tree", "octree", "check_counts", "set_sort_spin_limit", "DEFAULT_SORT_SPIN_LIMIT", "gen_sphere", "gen_scene", "gen_gmm_scene",
     "gen_batch_sizes", "gen_degenerate", "fnv64", "SX_KNN_PLAIN", "SX_KNN_RADIUS", "SX_PREC_FP32", "SX_PREC_FP64",
     "SX_ROOT_TEXTBOOK", "SX_ROOT_REFERENCE", "SX_KNN_ALGO_TILES", "SX_KNN_ALGO_LANE", "SX_SORT_ONESWEEP",
     "SX_SORT_THREE_KERNEL",

@@ -40,9 +40,6 @@ HIP_SOURCES = [
     "batch.hip",
 ]
 CXX_SOURCES = ["datagen.cpp"]
-# standalone measurement program (VALU issue rates), not linked into the library
-PROBE_SOURCE = "valu_probe.hip"
-PROBE = PKG / "valu_probe"
 
 ARCH = os.environ.get("SX_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -86,8 +83,6 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
         obj = BUILD / (src + ".o")
         objs.append(obj)
         cmds.append(["g++", *CXX_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
-    if (CSRC / PROBE_SOURCE).exists():
-        cmds.append([HIPCC, *HIP_FLAGS, str(CSRC / PROBE_SOURCE), "-o", str(PROBE)])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         futs = {ex.submit(_run, c): c for c in cmds}
         for f in cf.as_completed(futs):

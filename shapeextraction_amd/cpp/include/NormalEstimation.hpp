@@ -40,6 +40,9 @@ struct NormalOptions {
 };
 
 std::vector<Point> estimate_normals(const OT::Octree& octree, int start_idx, int n, int k, const NormalOptions& opt);
+// Normals of every input point of the cloud the octree was built from, in
+// that cloud's order (duplicates share their code's normal).
+std::vector<Point> estimate_normals_input_order(const OT::Octree& octree, int k, const NormalOptions& opt);
 
 }  // namespace shapes
 
@@ -59,5 +62,14 @@ public:
                                               const shapes::NormalOptions& opt) {
         static_assert(k == 8 || k == 16 || k == 24 || k == 32, "k must be 8, 16, 24 or 32");
         return shapes::estimate_normals(octree, start_idx, n, k, opt);
+    }
+    // Not in the reference, which returns Morton order (src/main.cu:119-120):
+    // one normal per point of the input cloud, in its order, ready for
+    // PointCloud::setNormals.
+    template <int k>
+    static std::vector<Point> estimateNormalsInputOrder(const OT::Octree& octree,
+                                                        const shapes::NormalOptions& opt = shapes::NormalOptions()) {
+        static_assert(k == 8 || k == 16 || k == 24 || k == 32, "k must be 8, 16, 24 or 32");
+        return shapes::estimate_normals_input_order(octree, k, opt);
     }
 };

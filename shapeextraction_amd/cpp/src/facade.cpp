@@ -2,6 +2,7 @@
 // NormalEstimation, PointCloud, MortonUtils, DataIO) over the C ABI of
 // libshapes_hip.so.  No HIP headers: every device operation goes through
 // include/shapes_c.h.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -181,24 +182,90 @@ OT::Octree::Octree(const RT::RadixTree& radix_tree) {
                   "sx_memcpy_dtoh");
 }
 
+namespace {
+// The raw grid coordinate whose decoded value (codeToPoint's arithmetic) is
+// exactly v under the cube (mn, range), or -1 when v is not a decoded grid
+// value.  The encode formula gives the candidate; decoding is monotone, so
+// the search only looks a few steps either side of it.
+int64_t snap_to_grid(float v, float mn, float range) {
+    const float fs = static_cast<float>(axis_scale());
+    float t = (v - mn) / range;
+    float sc = fs * t;
+    if (!(sc >= 0.0f)) sc = 0.0f;
+    if (sc > fs) sc = fs;
+    const int64_t r0 = static_cast<int64_t>(sc);
+    for (int64_t d = 0; d <= 3; ++d) {
+        for (int64_t r : {r0 - d, r0 + d}) {
+            if (r < 0 || r > static_cast<int64_t>(axis_scale())) continue;
+            const float u = static_cast<float>(static_cast<uint32_t>(r)) / fs;
+            const float w = u * range;
+            if (w + mn == v) return r;
+        }
+    }
+    return -1;
+}
+
+Code_t interleave_raw(uint64_t rx, uint64_t ry, uint64_t rz) {
+    Code_t code = 0;
+    for (int i = 0; i < CODE_LEN / 3; ++i)
+        code |= (((rx >> i) & 1u) << (3 * i)) | (((ry >> i) & 1u) << (3 * i + 1)) | (((rz >> i) & 1u) << (3 * i + 2));
+    return code;
+}
+}  // namespace
+
 OT::Octree::Octree(std::shared_ptr<std::vector<OTNode>> nodes, const int nn, std::shared_ptr<std::vector<Point>> points,
                    const int np) {
     if (!nodes || !points || nn < 0 || np <= 0 || static_cast<size_t>(nn) > nodes->size() ||
         static_cast<size_t>(np) > points->size())
         throw shapes::Error(SX_EINVAL, "Octree: bad host data");
+    if (nn == 0 && np > 1) throw shapes::Error(SX_EINVAL, "Octree: host points without octree nodes");
     h_points = points;
-    std::vector<float> x(np), y(np), z(np);
-    for (int i = 0; i < np; ++i) { x[i] = (*points)[i].x; y[i] = (*points)[i].y; z[i] = (*points)[i].z; }
-    index_ = shapes::build_index(x.data(), y.data(), z.data(), static_cast<size_t>(np), SX_ROOT_TEXTBOOK);
+    if (nn == 0) {
+        // one point: no node carries a cube; index the point itself
+        const Point& p = (*points)[0];
+        index_ = shapes::build_index(&p.x, &p.y, &p.z, 1, SX_ROOT_TEXTBOOK);
+    } else {
+        // The root's cell is the whole cube: cell_size = range
+        // (src/Octree.cu:257-261), and its corner is the cube minimum on the
+        // axis where some point sits at raw coordinate 0 (every other corner
+        // coordinate decodes a raw prefix >= 0, so it is not smaller).
+        const OTNode& root = (*nodes)[0];
+        const float mn = std::min(root.corner.x, std::min(root.corner.y, root.corner.z));
+        const float range = root.cell_size;
+        if (!(range > 0.0f) || !std::isfinite(range) || !std::isfinite(mn))
+            throw shapes::Error(SX_EINVAL, "Octree: the root node does not describe a cube");
+        // The points are the decoded unique points of that cube in Morton
+        // order: recover their codes exactly and index those.
+        std::vector<Code_t> codes(static_cast<size_t>(np));
+        for (int i = 0; i < np; ++i) {
+            const Point& p = (*points)[i];
+            const int64_t rx = snap_to_grid(p.x, mn, range), ry = snap_to_grid(p.y, mn, range),
+                          rz = snap_to_grid(p.z, mn, range);
+            if (rx < 0 || ry < 0 || rz < 0)
+                throw shapes::Error(SX_EINVAL, "Octree: host point " + std::to_string(i) +
+                                                   " is not a decoded grid point of the octree's cube");
+            codes[i] = interleave_raw(static_cast<uint64_t>(rx), static_cast<uint64_t>(ry), static_cast<uint64_t>(rz));
+            if (i > 0 && codes[i] <= codes[i - 1])
+                throw shapes::Error(SX_EINVAL, "Octree: host points are not distinct grid points in Morton order (point " +
+                                                   std::to_string(i) + ")");
+        }
+        index_ = std::make_shared<shapes::IndexHandle>(static_cast<int64_t>(np), CODE_LEN, SX_ROOT_TEXTBOOK);
+        DeviceBuffer dc(codes.size() * sizeof(Code_t));
+        shapes::check(sx_memcpy_htod(dc.ptr, codes.data(), codes.size() * sizeof(Code_t), nullptr), "sx_memcpy_htod");
+        shapes::check(sx_index_build_codes(index_->index, static_cast<const uint64_t*>(dc.ptr), np, mn, range, nullptr),
+                      "sx_index_build_codes");
+        shapes::check(sx_index_get_view(index_->index, &index_->view, nullptr), "sx_index_get_view");
+    }
     const sx_index_view& v = index_->view;
-    // The search returns positions in the index's own Morton order; host data
-    // from an Octree is already in that order, which is checked here.
-    std::vector<uint32_t> rep(static_cast<size_t>(v.n_unique));
-    shapes::check(sx_memcpy_dtoh(rep.data(), v.rep, rep.size() * sizeof(uint32_t), nullptr), "sx_memcpy_dtoh");
-    bool identity = v.n_unique == np;
-    for (size_t i = 0; identity && i < rep.size(); ++i) identity = rep[i] == i;
-    if (!identity)
-        throw shapes::Error(SX_EINVAL, "Octree: host points are not distinct points in their own Morton order");
+    // The search runs on the index points; they must be the caller's points,
+    // bit for bit and in the same order.
+    std::vector<Point> decoded(static_cast<size_t>(v.n_unique));
+    shapes::check(sx_memcpy_dtoh(decoded.data(), v.points, decoded.size() * sizeof(Point), nullptr), "sx_memcpy_dtoh");
+    bool same = v.n_unique == np;
+    for (int i = 0; same && i < np; ++i)
+        same = std::memcmp(&decoded[i], &(*points)[i], sizeof(Point)) == 0;
+    if (!same)
+        throw shapes::Error(SX_EINVAL, "Octree: the indexed points differ from the host points");
     n_pts = np;
     u_points = reinterpret_cast<Point*>(const_cast<float*>(v.points));
     n_nodes = nn;
@@ -292,14 +359,13 @@ RadiusModel::RadiusModel(const std::string& path) {
 
 RadiusModel::~RadiusModel() { sx_mlp_destroy(mlp_); }
 
-std::vector<Point> estimate_normals(const OT::Octree& octree, int start_idx, int n, int k, const NormalOptions& opt) {
-    if (!octree.index()) throw Error(SX_ESTATE, "estimateNormals: empty octree");
-    if (start_idx < 0 || n < 0 || start_idx + n > octree.n_pts)
-        throw Error(SX_EINVAL, "estimateNormals: range outside the octree points");
-    std::vector<Point> out(static_cast<size_t>(n));
-    if (n == 0) return out;
+namespace {
+// kNN + normals of the octree points [start_idx, start_idx + n) into the
+// device buffer `normals` (n rows of 12 bytes).
+void normals_on_device(const OT::Octree& octree, int start_idx, int n, int k, const NormalOptions& opt,
+                       DeviceBuffer& normals) {
     sx_index* ix = octree.index()->index;
-    DeviceBuffer nn(sizeof(int32_t) * static_cast<size_t>(n) * k), normals(sizeof(Point) * static_cast<size_t>(n));
+    DeviceBuffer nn(sizeof(int32_t) * static_cast<size_t>(n) * k);
     int mode = SX_KNN_PLAIN;
     std::unique_ptr<DeviceBuffer> feats, r2;
     if (opt.radius_model) {
@@ -317,7 +383,31 @@ std::vector<Point> estimate_normals(const OT::Octree& octree, int start_idx, int
     check(sx_normals(ix, static_cast<const int32_t*>(nn.ptr), start_idx, n, k, opt.precision, opt.viewpoint.x,
                      opt.viewpoint.y, opt.viewpoint.z, static_cast<float*>(normals.ptr), nullptr, nullptr),
           "sx_normals");
+}
+}  // namespace
+
+std::vector<Point> estimate_normals(const OT::Octree& octree, int start_idx, int n, int k, const NormalOptions& opt) {
+    if (!octree.index()) throw Error(SX_ESTATE, "estimateNormals: empty octree");
+    if (start_idx < 0 || n < 0 || start_idx + n > octree.n_pts)
+        throw Error(SX_EINVAL, "estimateNormals: range outside the octree points");
+    std::vector<Point> out(static_cast<size_t>(n));
+    if (n == 0) return out;
+    DeviceBuffer normals(sizeof(Point) * static_cast<size_t>(n));
+    normals_on_device(octree, start_idx, n, k, opt, normals);
     check(sx_memcpy_dtoh(out.data(), normals.ptr, sizeof(Point) * out.size(), nullptr), "sx_memcpy_dtoh");
+    return out;
+}
+
+std::vector<Point> estimate_normals_input_order(const OT::Octree& octree, int k, const NormalOptions& opt) {
+    if (!octree.index()) throw Error(SX_ESTATE, "estimateNormals: empty octree");
+    const sx_index_view& v = octree.index()->view;
+    std::vector<Point> out(static_cast<size_t>(v.n_input));
+    DeviceBuffer normals(sizeof(Point) * static_cast<size_t>(octree.n_pts));
+    normals_on_device(octree, 0, octree.n_pts, k, opt, normals);
+    DeviceBuffer in_order(sizeof(Point) * out.size());
+    check(sx_index_to_input_order(octree.index()->index, normals.ptr, sizeof(Point), in_order.ptr, nullptr),
+          "sx_index_to_input_order");
+    check(sx_memcpy_dtoh(out.data(), in_order.ptr, sizeof(Point) * out.size(), nullptr), "sx_memcpy_dtoh");
     return out;
 }
 

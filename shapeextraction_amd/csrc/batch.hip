@@ -37,7 +37,7 @@ int seg_decode_launch(const uint64_t* codes, const int32_t* d_slots, int64_t n_m
                       int code_bits, float4* pts16, hipStream_t st);
 size_t sort_ws_bytes(int64_t n);
 int sort_pairs_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n, int key_bits, uint64_t* keys_out,
-                      uint32_t* vals_out, void* ws, int algo, hipStream_t st);
+                      uint32_t* vals_out, void* ws, int algo, hipStream_t st, int32_t* err);
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
                   int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav, const float4* pts,
@@ -49,8 +49,9 @@ int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn
                    int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st,
                    CloudMap cm);
 
-// counts[] slots
-enum { kBSlots = 0, kBUnique = 1, kBBad = 2, kBDeferred = 3, kBQueue = 4 };
+// counts[] slots (the first four are cleared by every build; kBSortErr is
+// the sorts' look-back error flag)
+enum { kBSlots = 0, kBUnique = 1, kBBad = 2, kBSortErr = 3, kBDeferred = 4, kBQueue = 5 };
 
 // Cloud of each element in code order: a binary search of the cloud offsets,
 // staged in LDS when they fit (each block handles kCloudKeysItems elements
@@ -120,8 +121,10 @@ __global__ void __launch_bounds__(kScanBlock) cloud_table_kernel(const int64_t* 
         carry += total;
     }
     if (threadIdx.x == 0) {
-        counts[kBSlots] = carry;
-        counts[kBUnique] = n_clouds > 0 ? uos[off[n_clouds] - 1] + 1 : 0;
+        // a failed sort leaves no slots: no later stage runs on unsorted keys
+        const bool ok = counts[kBSortErr] == 0;
+        counts[kBSlots] = ok ? carry : 0;
+        counts[kBUnique] = ok && n_clouds > 0 ? uos[off[n_clouds] - 1] + 1 : 0;
     }
 }
 
@@ -254,12 +257,13 @@ int batch_build(sx_batch* b, const float* x, const float* y, const float* z, hip
     SX_TRY(sx::seg_bbox_launch(x, y, z, b->off, nc, b->bounds, b->counts + sx::kBBad, b->bbox_partial, st));
     SX_TRY(sx::seg_encode_launch(x, y, z, n, b->off, nc, b->bounds, cb, b->codes_in, st));
     // (cloud, code, input index) order: by code, then stably by cloud
-    SX_TRY(sx::sort_pairs_launch(b->codes_in, nullptr, n, cb, b->codes_a, b->perm_a, b->ws, b->sort_algo, st));
+    SX_TRY(sx::sort_pairs_launch(b->codes_in, nullptr, n, cb, b->codes_a, b->perm_a, b->ws, b->sort_algo, st,
+                                 b->counts + sx::kBSortErr));
     sx::cloud_keys_kernel<<<sx::grid_for(n, 256 * sx::kCloudKeysItems), 256, 0, st>>>(b->perm_a, n, b->off, nc,
                                                                                       b->keys_c);
     SX_LAUNCH_CHECK("cloud_keys_kernel");
     SX_TRY(sx::sort_pairs_launch(b->keys_c, b->perm_a, n, cloud_bits(nc), b->keys_s, b->perm, b->ws, b->sort_algo,
-                                 st));
+                                 st, b->counts + sx::kBSortErr));
     sx::gather_codes_kernel<<<sx::grid_for(n, 256), 256, 0, st>>>(b->codes_in, b->perm, n, b->codes_s);
     SX_LAUNCH_CHECK("gather_codes_kernel");
     SX_TRY(sx::device_scan(sx::BatchUniqueIn{b->keys_s, b->codes_s}, sx::BatchUniqueOut{b->uos}, nullptr, n, nullptr,
@@ -391,6 +395,9 @@ int sx_batch_get_view(sx_batch* b, sx_batch_view* view, void* stream) {
     int32_t h[8];
     SX_HIP(hipMemcpyAsync(h, b->counts, sizeof(h), hipMemcpyDeviceToHost, st));
     SX_HIP(hipStreamSynchronize(st));
+    if (h[sx::kBSortErr] != 0)
+        return fail(SX_EINTERNAL, "sx_batch_get_view: a sort of the build gave up waiting in a look-back; "
+                                  "the batch is empty and must be rebuilt");
     if (h[sx::kBBad] != 0) return fail(SX_EDATA, "sx_batch_get_view: non-finite input coordinates");
     view->n_clouds = b->n_clouds;
     view->n_input = b->n_input;

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "sx_internal.hpp"
 
@@ -29,9 +30,9 @@ int decode_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
                   float* pts12, float4* pts16, hipStream_t st);
 size_t sort_ws_bytes(int64_t n);
 int sort_pairs_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n, int key_bits, uint64_t* keys_out,
-                      uint32_t* vals_out, void* ws, int algo, hipStream_t st);
+                      uint32_t* vals_out, void* ws, int algo, hipStream_t st, int32_t* err);
 int unique_launch(const uint64_t* sorted, const uint32_t* vals, int64_t n, uint64_t* uniq, uint32_t* rep,
-                  int32_t* uos, int32_t* uoi, int32_t* d_count, void* ws, hipStream_t st);
+                  int32_t* uos, int32_t* uoi, int32_t* d_count, void* ws, hipStream_t st, const int32_t* sort_err);
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
                   int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav, const float4* pts,
@@ -53,9 +54,11 @@ int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn
                    CloudMap cm = CloudMap());
 }  // namespace sx
 
-// counts[] slots: unique points, octree nodes, non-finite inputs, queries the
-// kNN deferred to the per-lane search (cumulative), the last call's queue size
-enum { kCntUnique = 0, kCntOct = 1, kCntBad = 2, kCntReruns = 3, kCntQueue = 4 };
+// counts[] slots: unique points, octree nodes, non-finite inputs, the sort's
+// look-back error flag (the first four are cleared by every build), queries
+// the kNN deferred to the per-lane search (cumulative), the last call's queue
+// size
+enum { kCntUnique = 0, kCntOct = 1, kCntBad = 2, kCntSortErr = 3, kCntReruns = 4, kCntQueue = 5 };
 
 
 
@@ -99,6 +102,8 @@ struct sx_index {
     // host state
     int64_t n_input = 0;
     bool built = false;
+    bool from_codes = false;  // the last build took sorted codes (sx_index_build_codes)
+    bool uoi_valid = false;   // unique_of_input holds the last build's map
     bool counts_valid = false;
     hipEvent_t built_event = nullptr;  // recorded after every build
     int64_t n_unique = 0;
@@ -147,6 +152,9 @@ int check_index(const sx_index* ix) {
     return SX_OK;
 }
 
+// Query positions are checked against the unique count once a view has read
+// it, else against the input count; the kernels give positions at or beyond
+// the unique count rows of -1 (neighbours) or NaN (normals).
 int check_queries(const sx_index* ix, int64_t q_start, int64_t n_q) {
     const int64_t limit = ix->counts_valid ? ix->n_unique : ix->n_input;
     if (q_start < 0 || n_q < 0 || q_start + n_q > limit)
@@ -157,6 +165,21 @@ int check_queries(const sx_index* ix, int64_t q_start, int64_t n_q) {
 
 bool valid_k(int k) { return k == 8 || k == 16 || k == 24 || k == 32; }
 
+// unique_of_input of the last build (scattered from the sorted positions the
+// first time a caller needs it).
+int input_order_map(sx_index* ix, hipStream_t st) {
+    if (ix->uoi_valid) return SX_OK;
+    SX_TRY(sx::scatter_i32_launch(ix->uos, ix->perm, ix->n_input, ix->uoi, st));
+    ix->uoi_valid = true;
+    return SX_OK;
+}
+
+// Orders `stream` after the last build, whichever stream it ran on.
+int after_build(const sx_index* ix, void* stream) {
+    SX_HIP(hipStreamWaitEvent(sx::as_stream(stream), ix->built_event, 0));
+    return SX_OK;
+}
+
 }  // namespace
 
 namespace sx {
@@ -164,8 +187,9 @@ namespace sx {
 int index_feature_arrays(sx_index* ix, const TravNode** trav, const int2** up, const int2** oct_aux,
                          const uint8_t** pn, const int32_t** leaf_parent, const int32_t** leaf_oct,
                          const sx_otnode** oct, int64_t* oct_cap, const float** bounds, const int32_t** counts,
-                         int64_t* limit) {
+                         int64_t* limit, void* stream) {
     SX_TRY(check_index(ix));
+    SX_TRY(after_build(ix, stream));
     // textbook root: the octree links give the counts and the API's prefix
     // array the depth; otherwise the traversal tree's (parent, prefix) pairs
     const bool textbook = ix->root_mode == SX_ROOT_TEXTBOOK;
@@ -183,6 +207,80 @@ int index_feature_arrays(sx_index* ix, const TravNode** trav, const int2** up, c
     return SX_OK;
 }
 }  // namespace sx
+
+namespace sx {
+
+// Build from given codes: copy them, identity permutation and
+// representatives, and count codes that are not strictly increasing or do
+// not fit in code_bits into *bad.
+__global__ void __launch_bounds__(256) codes_in_kernel(const uint64_t* __restrict__ in, int64_t n, int code_bits,
+                                                       uint64_t* __restrict__ ucodes, uint32_t* __restrict__ perm,
+                                                       uint32_t* __restrict__ rep, int32_t* __restrict__ uos,
+                                                       int32_t* bad) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool wrong = false;
+    if (i < n) {
+        const uint64_t c = in[i];
+        wrong = (c >> code_bits) != 0 || (i > 0 && in[i - 1] >= c);
+        ucodes[i] = c;
+        perm[i] = static_cast<uint32_t>(i);
+        rep[i] = static_cast<uint32_t>(i);
+        uos[i] = static_cast<int32_t>(i);
+    }
+    const uint64_t m = __ballot(wrong);
+    if (m && lane_id() == static_cast<unsigned>(__ffsll(static_cast<unsigned long long>(m)) - 1))
+        atomicAdd(bad, __popcll(m));
+}
+
+// The cube and the unique count of a build from codes; no points when a code
+// was rejected (every later stage is sized by the count).
+__global__ void codes_count_kernel(int64_t n, float mn, float range, float* bounds, int32_t* counts, int bad_slot) {
+    bounds[0] = mn;
+    bounds[1] = __fadd_rn(mn, range);
+    bounds[2] = range;
+    counts[0] = counts[bad_slot] ? 0 : static_cast<int32_t>(n);
+}
+
+// dst row r = src row map[r] (rows of `words` 32-bit words).
+__global__ void __launch_bounds__(256) gather_rows_kernel(const uint32_t* __restrict__ src, int64_t words,
+                                                          const int32_t* __restrict__ map, int64_t n_rows,
+                                                          uint32_t* __restrict__ dst) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_rows * words) return;
+    const int64_t r = i / words, w = i - r * words;
+    dst[i] = src[static_cast<int64_t>(map[r]) * words + w];
+}
+
+}  // namespace sx
+
+namespace {
+// Decode, radix tree (+ traversal boxes) and octree of the unique codes,
+// sized by counts[kCntUnique] (capacity n).
+int build_trees(sx_index* ix, int64_t n, hipStream_t st) {
+    const int cb = ix->code_bits;
+    SX_TRY(sx::decode_launch(ix->ucodes, ix->counts + kCntUnique, n, ix->bounds, cb, ix->pts12, ix->pts16, st));
+    if (ix->root_mode == SX_ROOT_TEXTBOOK) {
+        SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, ix->pn, ix->lc, ix->hll,
+                                 ix->hlr, ix->parent, ix->node_end, ix->leaf_parent, nullptr, ix->trav, ix->pts16,
+                                 ix->bounds, st));
+    } else {
+        SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_REFERENCE, ix->pn, ix->lc,
+                                 ix->hll, ix->hlr, ix->parent, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                 st));
+        SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, nullptr, nullptr,
+                                 nullptr, nullptr, nullptr, nullptr, ix->leaf_parent, ix->up, ix->trav, ix->pts16,
+                                 ix->bounds, st));
+    }
+    // with the textbook root the octree and the traversal tree are one tree:
+    // record each octree node's parent and point count for the features
+    const bool textbook = ix->root_mode == SX_ROOT_TEXTBOOK;
+    SX_TRY(sx::octree_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, ix->bounds, ix->pn, ix->lc, ix->hll, ix->hlr,
+                             ix->parent, ix->oct, ix->oct_cap, ix->counts + kCntOct, ix->leaf_oct, ix->ws, st,
+                             textbook ? ix->node_end : nullptr, textbook ? ix->oct_aux : nullptr));
+    return SX_OK;
+}
+
+}  // namespace
 
 using sx::fail;
 
@@ -259,36 +357,47 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
     hipStream_t st = sx::as_stream(stream);
     ix->built = false;
     ix->counts_valid = false;
+    ix->from_codes = false;
+    ix->uoi_valid = false;
     const int cb = ix->code_bits;
     // unique / octree / non-finite counts; the deferred-query total accumulates
     // across builds until sx_index_knn_reruns resets it
     SX_TRY(sx::fill_i32(ix->counts, 0, kCntReruns, st));
     SX_TRY(sx::bbox_launch(d_x, d_y, d_z, n, ix->bounds, ix->counts + kCntBad, ix->ws_bbox, st));
     SX_TRY(sx::encode_launch(d_x, d_y, d_z, n, ix->bounds, cb, ix->codes_in, st));
-    SX_TRY(sx::sort_pairs_launch(ix->codes_in, nullptr, n, cb, ix->codes_sorted, ix->perm, ix->ws, ix->sort_algo, st));
+    SX_TRY(sx::sort_pairs_launch(ix->codes_in, nullptr, n, cb, ix->codes_sorted, ix->perm, ix->ws, ix->sort_algo, st,
+                                 ix->counts + kCntSortErr));
+    // a failed sort leaves the unique count at 0: no later stage runs on it
     SX_TRY(sx::unique_launch(ix->codes_sorted, ix->perm, n, ix->ucodes, ix->rep, ix->uos, nullptr,
-                             ix->counts + kCntUnique, ix->ws, st));
-    SX_TRY(sx::decode_launch(ix->ucodes, ix->counts + kCntUnique, n, ix->bounds, cb, ix->pts12, ix->pts16, st));
-    if (ix->root_mode == SX_ROOT_TEXTBOOK) {
-        SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, ix->pn, ix->lc, ix->hll,
-                                 ix->hlr, ix->parent, ix->node_end, ix->leaf_parent, nullptr, ix->trav, ix->pts16,
-                                 ix->bounds, st));
-    } else {
-        SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_REFERENCE, ix->pn, ix->lc,
-                                 ix->hll, ix->hlr, ix->parent, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                 st));
-        SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, nullptr, nullptr,
-                                 nullptr, nullptr, nullptr, nullptr, ix->leaf_parent, ix->up, ix->trav, ix->pts16,
-                                 ix->bounds, st));
-    }
-    // with the textbook root the octree and the traversal tree are one tree:
-    // record each octree node's parent and point count for the features
-    const bool textbook = ix->root_mode == SX_ROOT_TEXTBOOK;
-    SX_TRY(sx::octree_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, ix->bounds, ix->pn, ix->lc, ix->hll, ix->hlr,
-                             ix->parent, ix->oct, ix->oct_cap, ix->counts + kCntOct, ix->leaf_oct, ix->ws, st,
-                             textbook ? ix->node_end : nullptr, textbook ? ix->oct_aux : nullptr));
+                             ix->counts + kCntUnique, ix->ws, st, ix->counts + kCntSortErr));
+    SX_TRY(build_trees(ix, n, st));
     ix->n_input = n;
     SX_HIP(hipEventRecord(ix->built_event, sx::as_stream(stream)));
+    ix->built = true;
+    return SX_OK;
+}
+
+int sx_index_build_codes(sx_index* ix, const uint64_t* d_codes, int64_t n, float min_coord, float range,
+                         void* stream) {
+    if (!ix) return fail(SX_EINVAL, "sx_index_build_codes: null index");
+    if (n <= 0 || n > ix->cap) return fail(SX_EINVAL, "sx_index_build_codes: n must be in [1, capacity]");
+    if (!d_codes) return fail(SX_EINVAL, "sx_index_build_codes: null codes");
+    if (!std::isfinite(min_coord) || !std::isfinite(range) || range < 0.0f)
+        return fail(SX_EINVAL, "sx_index_build_codes: the cube must be finite with range >= 0");
+    hipStream_t st = sx::as_stream(stream);
+    ix->built = false;
+    ix->counts_valid = false;
+    ix->from_codes = true;
+    ix->uoi_valid = false;
+    SX_TRY(sx::fill_i32(ix->counts, 0, kCntReruns, st));
+    sx::codes_in_kernel<<<sx::grid_for(n, 256), 256, 0, st>>>(d_codes, n, ix->code_bits, ix->ucodes, ix->perm,
+                                                               ix->rep, ix->uos, ix->counts + kCntBad);
+    SX_LAUNCH_CHECK("codes_in_kernel");
+    sx::codes_count_kernel<<<1, 1, 0, st>>>(n, min_coord, range, ix->bounds, ix->counts, kCntBad);
+    SX_LAUNCH_CHECK("codes_count_kernel");
+    SX_TRY(build_trees(ix, n, st));
+    ix->n_input = n;
+    SX_HIP(hipEventRecord(ix->built_event, st));
     ix->built = true;
     return SX_OK;
 }
@@ -296,10 +405,18 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
 int sx_index_get_view(sx_index* ix, sx_index_view* view, void* stream) {
     SX_TRY(check_index(ix));
     if (!view) return fail(SX_EINVAL, "sx_index_get_view: null view");
-    // order this stream after the build, whichever stream the build ran on
-    SX_HIP(hipStreamWaitEvent(sx::as_stream(stream), ix->built_event, 0));
-    // the input-order map is only built when a caller asks for the view
-    SX_TRY(sx::scatter_i32_launch(ix->uos, ix->perm, ix->n_input, ix->uoi, sx::as_stream(stream)));
+    SX_TRY(after_build(ix, stream));
+    SX_TRY(input_order_map(ix, sx::as_stream(stream)));
+    int32_t flags[2] = {0, 0};  // non-finite inputs (rejected codes), sort error
+    SX_HIP(hipMemcpyAsync(flags, ix->counts + kCntBad, sizeof(flags), hipMemcpyDeviceToHost, sx::as_stream(stream)));
+    SX_HIP(hipStreamSynchronize(sx::as_stream(stream)));
+    if (flags[1])
+        return fail(SX_EINTERNAL, "sx_index_get_view: the build's sort gave up waiting in a look-back; "
+                                  "the index is empty and must be rebuilt");
+    if (ix->from_codes && flags[0])
+        return fail(SX_EDATA, "sx_index_get_view: " + std::to_string(flags[0]) +
+                                  " codes given to sx_index_build_codes are not strictly increasing or do not fit "
+                                  "in the index's code bits; the index is empty");
     int64_t nu = 0, no = 0;
     SX_TRY(sx_check_counts(ix->counts + kCntUnique, ix->n_input, ix->counts + kCntOct, ix->oct_cap,
                            ix->counts + kCntBad, &nu, &no, stream));
@@ -329,6 +446,22 @@ int sx_index_get_view(sx_index* ix, sx_index_view* view, void* stream) {
     return SX_OK;
 }
 
+int sx_index_to_input_order(sx_index* ix, const void* d_rows, int64_t row_bytes, void* d_out, void* stream) {
+    SX_TRY(check_index(ix));
+    if (row_bytes <= 0 || row_bytes % 4 != 0)
+        return fail(SX_EINVAL, "sx_index_to_input_order: row_bytes must be a positive multiple of 4");
+    if (!d_rows || !d_out || (reinterpret_cast<uintptr_t>(d_rows) | reinterpret_cast<uintptr_t>(d_out)) % 4 != 0)
+        return fail(SX_EINVAL, "sx_index_to_input_order: rows must be 4-byte aligned device buffers");
+    hipStream_t st = sx::as_stream(stream);
+    SX_TRY(after_build(ix, stream));
+    SX_TRY(input_order_map(ix, st));
+    const int64_t words = row_bytes / 4;
+    sx::gather_rows_kernel<<<sx::grid_for(ix->n_input * words, 256), 256, 0, st>>>(
+        static_cast<const uint32_t*>(d_rows), words, ix->uoi, ix->n_input, static_cast<uint32_t*>(d_out));
+    SX_LAUNCH_CHECK("gather_rows_kernel");
+    return SX_OK;
+}
+
 int sx_index_knn_reruns(sx_index* ix, int64_t* reruns, int reset, void* stream) {
     if (!ix || !reruns) return fail(SX_EINVAL, "sx_index_knn_reruns: null argument");
     int32_t h = 0;
@@ -348,6 +481,7 @@ int sx_knn(sx_index* ix, int64_t q_start, int64_t n_q, int k, int mode, const fl
     if (mode == SX_KNN_RADIUS && !d_radius2) return fail(SX_EINVAL, "sx_knn: radius mode needs d_radius2");
     SX_TRY(check_queries(ix, q_start, n_q));
     if (n_q > 0 && !d_out) return fail(SX_EINVAL, "sx_knn: null output");
+    SX_TRY(after_build(ix, stream));
     const float* r2 = mode == SX_KNN_RADIUS ? d_radius2 : nullptr;
     if (ix->knn_algo == SX_KNN_ALGO_TILES)
         return sx::knn_tiles_launch(ix->pts16, ix->trav, ix->counts, q_start, n_q, k, r2, ix->bucket, d_out,
@@ -361,6 +495,7 @@ int sx_knn_points(sx_index* ix, const float* d_queries, int64_t n_q, int k, int3
     SX_TRY(check_index(ix));
     if (!valid_k(k)) return fail(SX_EINVAL, "sx_knn_points: k must be 8, 16, 24 or 32");
     if (n_q < 0 || (n_q > 0 && (!d_queries || !d_out))) return fail(SX_EINVAL, "sx_knn_points: bad arguments");
+    SX_TRY(after_build(ix, stream));
     return sx::knn_launch(ix->pts16, ix->trav, ix->counts, 0, n_q, d_queries, k, nullptr, ix->bucket, d_out, nullptr,
                           sx::as_stream(stream));
 }
@@ -371,6 +506,7 @@ int sx_normals(sx_index* ix, const int32_t* d_nn, int64_t q_start, int64_t n_q, 
     if (!valid_k(k)) return fail(SX_EINVAL, "sx_normals: k must be 8, 16, 24 or 32");
     SX_TRY(check_queries(ix, q_start, n_q));
     if (n_q > 0 && (!d_nn || !d_normals)) return fail(SX_EINVAL, "sx_normals: null pointer");
+    SX_TRY(after_build(ix, stream));
     return sx::normals_launch(ix->pts16, ix->counts, d_nn, q_start, n_q, k, precision, vx, vy, vz, d_normals, d_flags,
                               sx::as_stream(stream));
 }

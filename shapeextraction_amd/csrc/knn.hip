@@ -85,6 +85,13 @@ __device__ __forceinline__ void write_row(int32_t* row, const TopK<K>& top, int 
     }
 }
 
+// Row of a query position at or beyond the unique count: every entry -1.
+template <int K>
+__device__ __forceinline__ void write_missing_row(int32_t* row) {
+#pragma unroll
+    for (int s = 0; s < K; s += 4) *reinterpret_cast<int4*>(row + s) = make_int4(-1, -1, -1, -1);
+}
+
 // One query's exact search.  State machine with one record load per step:
 //   down at `node`: pick the nearer child; a bucket child is scanned (next
 //     step) and counts as returned-from; an internal child is entered.
@@ -181,7 +188,10 @@ __global__ void __launch_bounds__(kKnnBlock) knn_kernel(const float4* __restrict
         qz = queries[3 * t + 2];
     } else {
         const int64_t qi = q_start + t;
-        if (qi >= total) return;
+        if (qi >= total) {  // past the unique count (range checked against the input count)
+            write_missing_row<K>(out + t * K);
+            return;
+        }
         const int2 cloud = cm.at(qi, total);
         base = cloud.x;
         U = cloud.y;
@@ -556,9 +566,13 @@ knn_tile_kernel(const float4* __restrict__ pts,
     const int w = static_cast<int>(threadIdx.x) / kWave;
     const int lane = static_cast<int>(lane_id());
     const int total = d_counts[0];
-    int64_t q_end = q_start + n_q;
+    const int64_t q_req = q_start + n_q;
+    int64_t q_end = q_req;
     if (q_end > total) q_end = total;
     const int64_t tile0 = q_start + (static_cast<int64_t>(xcd_block()) * kTileWaves + w) * kWave;
+    // positions asked for at or beyond the unique count get rows of -1 (the
+    // range is checked against the input count until the counts are read)
+    if (tile0 + lane >= total && tile0 + lane < q_req) write_missing_row<K>(out + (tile0 + lane - q_start) * K);
     if (tile0 >= q_end) return;  // whole wave idle (no block barriers below)
     // the tile's cloud: points and radix nodes [cbase, cbase + U)
     const int2 cloud = cm.at(tile0, total);

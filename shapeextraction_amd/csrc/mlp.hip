@@ -719,7 +719,7 @@ namespace sx {
 int index_feature_arrays(sx_index* ix, const TravNode** trav, const int2** up, const int2** oct_aux,
                          const uint8_t** pn, const int32_t** leaf_parent, const int32_t** leaf_oct,
                          const sx_otnode** oct, int64_t* oct_cap, const float** bounds, const int32_t** counts,
-                         int64_t* limit);
+                         int64_t* limit, void* stream);
 }
 
 using sx::fail;
@@ -890,7 +890,7 @@ int sx_radius_features(sx_index* index, int64_t q_start, int64_t n_q, float* d_f
     int64_t limit;
     int64_t oct_cap;
     SX_TRY(sx::index_feature_arrays(index, &trav, &up, &aux, &pn, &lp, &lo, &oct, &oct_cap, &bounds, &counts,
-                                    &limit));
+                                    &limit, stream));
     if (q_start < 0 || n_q < 0 || q_start + n_q > limit) return fail(SX_EINVAL, "sx_radius_features: bad query range");
     if (n_q == 0) return SX_OK;
     if (!d_feats || reinterpret_cast<uintptr_t>(d_feats) % 16 != 0)
@@ -914,7 +914,7 @@ int sx_radius_predict(sx_index* index, sx_mlp* mlp, int64_t q_start, int64_t n_q
     int64_t limit;
     int64_t oct_cap;
     SX_TRY(sx::index_feature_arrays(index, &trav, &up, &aux, &pn, &lp, &lo, &oct, &oct_cap, &bounds, &counts,
-                                    &limit));
+                                    &limit, stream));
     if (q_start < 0 || n_q < 0 || q_start + n_q > limit) return fail(SX_EINVAL, "sx_radius_predict: bad query range");
     if (n_q == 0) return SX_OK;
     if (!d_radius2) return fail(SX_EINVAL, "sx_radius_predict: null output");

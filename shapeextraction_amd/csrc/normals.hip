@@ -148,7 +148,11 @@ __global__ void __launch_bounds__(kNormalsBlock) normals_kernel(const float4* __
     if (t >= n_q) return;
     const int total = d_counts[0];
     const int64_t qi = q_start + t;
-    if (qi >= total) return;
+    if (qi >= total) {  // past the unique count (range checked against the input count): no normal
+        normals[3 * t] = normals[3 * t + 1] = normals[3 * t + 2] = __builtin_nanf("");
+        if (flags) flags[t] = 0;
+        return;
+    }
     // rows hold indices relative to the query's cloud
     const int2 cloud = cm.at(qi, total);
     if (qi - cloud.x >= cloud.y) return;  // padding slot of a batched cloud

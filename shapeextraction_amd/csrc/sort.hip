@@ -28,6 +28,8 @@
 // contiguous 1024-key span so that ranks follow input order.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "sx_internal.hpp"
 
 namespace sx {
@@ -150,6 +152,7 @@ inline int64_t sort_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
 // ---------------------------------------------------------------------------
 constexpr uint32_t kFlagAgg = 1u << 30, kFlagPrefix = 2u << 30, kCountMask = (1u << 30) - 1;
 constexpr int kMaxPasses = 8;
+constexpr uint32_t kDefaultSpinLimit = 1u << 22;
 
 __global__ void __launch_bounds__(kSortBlock) onesweep_hist_kernel(const uint64_t* __restrict__ keys, int64_t n,
                                                                    int passes, uint32_t* __restrict__ ghist) {
@@ -191,6 +194,7 @@ __global__ void __launch_bounds__(kPassBlock) __attribute__((amdgpu_waves_per_eu
                                                                    const uint32_t* __restrict__ vals_in, int64_t n,
                                                                    int shift, const uint32_t* __restrict__ gbase,
                                                                    uint32_t* status, uint32_t* tile_counter,
+                                                                   uint32_t spin_limit, int32_t* err,
                                                                    uint64_t* __restrict__ keys_out,
                                                                    uint32_t* __restrict__ vals_out) {
     __shared__ uint32_t cnt[kPassWaves][kRadix];
@@ -296,8 +300,11 @@ __global__ void __launch_bounds__(kPassBlock) __attribute__((amdgpu_waves_per_eu
             if (waiting) {
                 // Tile t holds an earlier ticket, so it is running and will
                 // publish; the bound only guards against a broken invariant.
-                if (++spins > (1u << 22)) {
-                    atomicOr(tile_counter + kMaxPasses, 1u);
+                // Giving up leaves this tile's prefix short (its keys land
+                // inside [0, n) but collide with others'): *err is raised so
+                // that the caller's status reports the failed sort.
+                if (++spins > spin_limit) {
+                    atomicOr(err, 1);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
@@ -341,8 +348,21 @@ size_t onesweep_ws_bytes(int64_t n, int passes) {
     return c.off;
 }
 
+// The look-back error flag inside a workspace laid out for n keys.
+int32_t* onesweep_ws_flag(void* ws, int64_t n) {
+    Carver c(ws);
+    c.take<uint64_t>(static_cast<size_t>(n));
+    c.take<uint32_t>(static_cast<size_t>(n));
+    c.take<uint32_t>(kMaxPasses * kRadix);
+    return reinterpret_cast<int32_t*>(c.take<uint32_t>(kMaxPasses + 1) + kMaxPasses);
+}
+
+// Spins a look-back waits for an earlier tile before it gives up (a host
+// setting, read at launch).
+std::atomic<uint32_t> g_sort_spin_limit{kDefaultSpinLimit};
+
 int onesweep_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n, int key_bits, uint64_t* keys_out,
-                    uint32_t* vals_out, void* ws, hipStream_t st) {
+                    uint32_t* vals_out, void* ws, int32_t* err, hipStream_t st) {
     const int passes = (key_bits + 7) / 8;
     const int64_t tiles = sort_tiles(n);
     Carver c(ws);
@@ -361,6 +381,8 @@ int onesweep_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n,
     SX_LAUNCH_CHECK("onesweep_hist_kernel");
     onesweep_base_kernel<<<passes, kRadix, 0, st>>>(ghist);
     SX_LAUNCH_CHECK("onesweep_base_kernel");
+    if (!err) err = reinterpret_cast<int32_t*>(counters + kMaxPasses);
+    const uint32_t spin_limit = g_sort_spin_limit.load();
     const uint64_t* src_k = keys_in;
     const uint32_t* src_v = vals_in;
     for (int p = 0; p < passes; ++p) {
@@ -369,7 +391,7 @@ int onesweep_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n,
         uint32_t* dst_v = to_out ? vals_out : vals_tmp;
         onesweep_pass_kernel<<<static_cast<unsigned>(tiles), kPassBlock, 0, st>>>(
             src_k, src_v, n, 8 * p, ghist + p * kRadix, status + static_cast<size_t>(p) * kRadix * tiles,
-            counters + p, dst_k, dst_v);
+            counters + p, spin_limit, err, dst_k, dst_v);
         SX_LAUNCH_CHECK("onesweep_pass_kernel");
         src_k = dst_k;
         src_v = dst_v;
@@ -425,11 +447,13 @@ size_t sort_ws_bytes(int64_t n) {
     return n < (1ll << 30) ? onesweep_ws_bytes(n, kMaxPasses) : sort_ws_bytes_3k(n);
 }
 
+// err (nullable): device int32 that a failed onesweep look-back sets to
+// nonzero (without it the flag stays in the workspace, onesweep_ws_flag).
 int sort_pairs_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n, int key_bits, uint64_t* keys_out,
-                      uint32_t* vals_out, void* ws, int algo, hipStream_t st) {
+                      uint32_t* vals_out, void* ws, int algo, hipStream_t st, int32_t* err) {
     if (n <= 0) return SX_OK;
     if (n < (1ll << 30) && algo == SX_SORT_ONESWEEP)
-        return onesweep_launch(keys_in, vals_in, n, key_bits, keys_out, vals_out, ws, st);
+        return onesweep_launch(keys_in, vals_in, n, key_bits, keys_out, vals_out, ws, err, st);
     return sort_3k_launch(keys_in, vals_in, n, key_bits, keys_out, vals_out, ws, st);
 }
 
@@ -452,5 +476,21 @@ extern "C" int sx_sort_pairs(const uint64_t* d_keys_in, const uint32_t* d_vals_i
     if (d_keys_in == d_keys_out || (d_vals_in && d_vals_in == d_vals_out))
         return fail(SX_EINVAL, "sx_sort_pairs: input and output buffers must differ");
     return sx::sort_pairs_launch(d_keys_in, d_vals_in, n, key_bits, d_keys_out, d_vals_out, ws, SX_SORT_ONESWEEP,
-                                 sx::as_stream(stream));
+                                 sx::as_stream(stream), nullptr);
+}
+
+extern "C" int sx_sort_check(const void* ws, int64_t n, void* stream) {
+    if (!ws || n < 0) return fail(SX_EINVAL, "sx_sort_check: null workspace or negative n");
+    if (n == 0 || n >= (1ll << 30)) return SX_OK;  // no look-back ran
+    int32_t h = 0;
+    hipStream_t st = sx::as_stream(stream);
+    SX_HIP(hipMemcpyAsync(&h, sx::onesweep_ws_flag(const_cast<void*>(ws), n), sizeof(h), hipMemcpyDeviceToHost, st));
+    SX_HIP(hipStreamSynchronize(st));
+    return h ? fail(SX_EINTERNAL, "sx_sort_pairs: a look-back gave up waiting for an earlier tile; the sort is invalid")
+             : SX_OK;
+}
+
+extern "C" int sx_sort_set_spin_limit(uint32_t spins) {
+    sx::g_sort_spin_limit.store(spins);
+    return SX_OK;
 }

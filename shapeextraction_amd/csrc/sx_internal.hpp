@@ -185,9 +185,11 @@ __global__ void __launch_bounds__(kScanBlock) scan_reduce_kernel(In in, const in
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
 }
 
-// One block: exclusive scan of the tile sums in place, total to *d_total.
+// One block: exclusive scan of the tile sums in place, total to *d_total
+// (0 instead when *d_zero_if is nonzero: a failed earlier stage).
 template <int kBlock = kScanBlock>
-__global__ void __launch_bounds__(kBlock) scan_tiles_kernel(int* tile_sums, int n_tiles, int32_t* d_total) {
+__global__ void __launch_bounds__(kBlock) scan_tiles_kernel(int* tile_sums, int n_tiles, int32_t* d_total,
+                                                            const int32_t* d_zero_if) {
     int carry = 0;
     for (int base = 0; base < n_tiles; base += kBlock) {
         const int i = base + static_cast<int>(threadIdx.x);
@@ -197,7 +199,7 @@ __global__ void __launch_bounds__(kBlock) scan_tiles_kernel(int* tile_sums, int 
         if (i < n_tiles) tile_sums[i] = carry + ex;
         carry += total;
     }
-    if (threadIdx.x == 0 && d_total) *d_total = carry;
+    if (threadIdx.x == 0 && d_total) *d_total = (d_zero_if && *d_zero_if) ? 0 : carry;
 }
 
 template <typename In, typename Out>
@@ -231,14 +233,15 @@ inline int64_t scan_tiles(int64_t n_max) { return (n_max + kScanTile - 1) / kSca
 inline size_t scan_ws_bytes(int64_t n_max) { return align_up(static_cast<size_t>(scan_tiles(n_max) + 1) * sizeof(int)); }
 
 template <typename In, typename Out>
-int device_scan(In in, Out out, const int32_t* d_n, int64_t n_max, int32_t* d_total, void* ws, hipStream_t st) {
+int device_scan(In in, Out out, const int32_t* d_n, int64_t n_max, int32_t* d_total, void* ws, hipStream_t st,
+                const int32_t* d_zero_if = nullptr) {
     if (n_max <= 0) return fail(SX_EINVAL, "device_scan: empty range");
     const int64_t tiles = scan_tiles(n_max);
     if (tiles > (1ll << 30)) return fail(SX_EINVAL, "device_scan: too many tiles");
     int* tile_sums = static_cast<int*>(ws);
     scan_reduce_kernel<In><<<static_cast<unsigned>(tiles), kScanBlock, 0, st>>>(in, d_n, n_max, tile_sums);
     SX_LAUNCH_CHECK("scan_reduce_kernel");
-    scan_tiles_kernel<kScanBlock><<<1, kScanBlock, 0, st>>>(tile_sums, static_cast<int>(tiles), d_total);
+    scan_tiles_kernel<kScanBlock><<<1, kScanBlock, 0, st>>>(tile_sums, static_cast<int>(tiles), d_total, d_zero_if);
     SX_LAUNCH_CHECK("scan_tiles_kernel");
     scan_apply_kernel<In, Out><<<static_cast<unsigned>(tiles), kScanBlock, 0, st>>>(in, out, d_n, n_max, tile_sums);
     SX_LAUNCH_CHECK("scan_apply_kernel");

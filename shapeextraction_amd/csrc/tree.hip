@@ -41,9 +41,14 @@ struct UniqueOut {
     }
 };
 
+// sort_err (nullable): the sort's error flag; when it is set the unique count
+// is written as 0, so that every later stage (sized by that count) does no
+// work on keys that are not sorted.
 int unique_launch(const uint64_t* sorted, const uint32_t* vals, int64_t n, uint64_t* uniq, uint32_t* rep,
-                  int32_t* uos, int32_t* uoi, int32_t* d_count, void* ws, hipStream_t st) {
-    return device_scan(UniqueIn{sorted}, UniqueOut{sorted, vals, uniq, rep, uos, uoi}, nullptr, n, d_count, ws, st);
+                  int32_t* uos, int32_t* uoi, int32_t* d_count, void* ws, hipStream_t st,
+                  const int32_t* sort_err = nullptr) {
+    return device_scan(UniqueIn{sorted}, UniqueOut{sorted, vals, uniq, rep, uos, uoi}, nullptr, n, d_count, ws, st,
+                       sort_err);
 }
 
 struct Box3 {

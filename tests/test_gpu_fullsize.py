@@ -1,0 +1,148 @@
+"""Oracle parity at the full sizes of the benchmark workloads, through the C
+ABI: the 8,388,608-point scene (k = 16), the 16,777,216-point Gaussian
+mixture (k = 32, plain and with the committed learned radius) and the
+4,194,304-point degenerate structures (k = 24).
+
+The tile kNN's behaviour depends on scale and density (segment budgets,
+deferred queries, stack overflows, XCD tile order), so these sizes are where a
+regression would hide.  Every stage is compared bit for bit with the oracle
+(codes, sort permutation, unique codes and representatives, input-order map,
+decoded points, radix tree arrays, octree nodes byte for byte); 20,000
+sampled kNN rows with the grid oracle (an exact method independent of any
+tree); every row with the size-independent properties of an exact result
+(the query first, (distance, index) ascending); normals against FP64 PCA on a
+sample."""
+import numpy as np
+import pytest
+
+from conftest import rng
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "scene8m": ("scene", 2, 8_388_608, 16),
+    "gmm16m": ("gmm", 3, 16_777_216, 32),
+    "degenerate4m": ("degenerate", 5, 4_194_304, 24),
+}
+
+
+def _generate(sx, kind, seed, n):
+    if kind == "scene":
+        return sx.gen_scene(seed, n)
+    if kind == "gmm":
+        return sx.gen_gmm_scene(seed, n)
+    return sx.gen_degenerate(seed, n)[:3]
+
+
+def _dist2(pts, q, rows):
+    """Squared distances in the kernels' operation order (no contraction)."""
+    d = pts[rows] - pts[q][:, None, :]
+    xx, yy, zz = d[..., 0] * d[..., 0], d[..., 1] * d[..., 1], d[..., 2] * d[..., 2]
+    return (xx + yy) + zz
+
+
+def _check_rows_exact_form(pts, nn, chunk=1 << 19):
+    """Every row: distance 0 first (the query, or an equal decoded point of
+    lower index), keys (distance, index) strictly ascending."""
+    U, k = nn.shape
+    for a in range(0, U, chunk):
+        b = min(U, a + chunk)
+        q = np.arange(a, b)
+        rows = nn[a:b]
+        d = _dist2(pts, q, rows)
+        assert (d[:, 0] == 0).all() and (rows[:, 0] <= q).all(), "a row does not start at its query"
+        db = d.view(np.uint32).astype(np.uint64)
+        keys = (db << np.uint64(32)) | rows.astype(np.uint64)
+        assert (keys[:, 1:] > keys[:, :-1]).all(), "a row is not ascending by (distance, index)"
+
+
+def _check_stages(f, O, x, y, z):
+    b, bad = O.bbox(x, y, z)
+    assert bad == 0
+    codes = O.encode(x, y, z, b[0], b[2], 63)
+    sc, perm = O.sort_pairs(codes)
+    uq, rep, uoi = O.unique(sc, perm)
+    U = len(uq)
+    assert f["n_unique"] == U
+    assert f["min_coord"] == b[0] and f["max_coord"] == b[1]
+    assert np.array_equal(f["sorted_perm"], perm)
+    assert np.array_equal(f["codes"], uq)
+    assert np.array_equal(f["rep"], rep)
+    assert np.array_equal(f["unique_of_input"], uoi)
+    pts = O.decode(uq, b[0], b[2], 63)
+    assert np.array_equal(f["points"], pts)
+    del codes, sc, perm, rep, uoi
+    rt = O.karras(uq, 63, 0)
+    for key in ("prefix_n", "left_child", "has_leaf_left", "has_leaf_right"):
+        assert np.array_equal(f[key][: U - 1], rt[key][: U - 1]), key
+    assert np.array_equal(f["parent"], rt["parent"])
+    nodes = O.octree(uq, 63, b[0], b[2], rt)
+    assert f["n_oct"] == len(nodes)
+    assert f["oct_nodes"].tobytes() == nodes.tobytes()
+    return pts
+
+
+@pytest.fixture(scope="module")
+def built(gpu):
+    """One built index per workload, kept for the module (they are large)."""
+    cache = {}
+
+    def get(name):
+        if name not in cache:
+            cache.clear()
+            kind, seed, n, k = CASES[name]
+            x, y, z = _generate(gpu, kind, seed, n)
+            ix = gpu.Index(n, 63).build(x, y, z)
+            cache[name] = (x, y, z, ix, ix.fetch())
+        return cache[name]
+
+    yield get
+    cache.clear()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fullsize_stages_and_knn(gpu, O, built, name):
+    kind, seed, n, k = CASES[name]
+    x, y, z, ix, f = built(name)
+    pts = _check_stages(f, O, x, y, z)
+    U = f["n_unique"]
+    ix.knn_reruns(reset=True)
+    nn_d = ix.knn(k)
+    nn = nn_d.numpy()
+    deferred = ix.knn_reruns()
+    print(f"[{name}] U={U} octree nodes={f['n_oct']} deferred {deferred} ({deferred / U:.2%})")
+    q = np.sort(rng(seed + 1000).choice(U, 20_000, replace=False))
+    assert np.array_equal(nn[q], O.knn_grid(pts, k, q))
+    _check_rows_exact_form(pts, nn)
+    # normals of the timed configuration (FP32) against FP64 PCA on a sample
+    nrm = ix.normals(nn_d, k).numpy()
+    s = np.sort(rng(seed + 2000).choice(U, 200_000, replace=False))
+    n64, _ = O.pca_f64(pts, nn[s])
+    err = O.angular_error_deg(nrm[s], n64)
+    med, p99 = float(np.median(err)), float(np.percentile(err, 99))
+    print(f"[{name}] normals vs FP64 PCA: median {med:.2e} deg, p99 {p99:.2e} deg")
+    if kind != "degenerate":  # lines have no unique normal (checked per class in test_gpu_pipeline)
+        assert med <= 0.1 and p99 <= 1.0
+
+
+@pytest.mark.parametrize("name,k", [("gmm16m", 32), ("scene8m", 16)])
+def test_fullsize_learned_radius_rows(gpu, O, built, name, k):
+    """The learned radius (committed model, features + MLP in one kernel,
+    then the seeded tile search) gives the plain rows exactly, and defers
+    fewer queries than the plain search."""
+    from shapeextraction_amd import learned
+
+    x, y, z, ix, f = built(name)
+    U = f["n_unique"]
+    model = learned.RadiusModel.load(learned.default_model_path(k))
+    ix.knn_reruns(reset=True)
+    plain = ix.knn(k).numpy()
+    d_plain = ix.knn_reruns()
+    r2 = model.predict_index_radius2(ix)
+    got = ix.knn(k, mode=gpu.SX_KNN_RADIUS, radius2=r2).numpy()
+    d_learned = ix.knn_reruns()
+    print(f"[{name} learned] deferred plain {d_plain} learned {d_learned}")
+    assert np.array_equal(got, plain)
+    q = np.sort(rng(77).choice(U, 10_000, replace=False))
+    assert np.array_equal(got[q], O.knn_grid(f["points"], k, q))
+    assert d_learned < d_plain
