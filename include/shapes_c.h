@@ -303,6 +303,17 @@ int sx_normals(sx_index* index, const int32_t* d_nn, int64_t q_start, int64_t n_
 int sx_estimate_normals(sx_index* index, int64_t q_start, int64_t n_q, int k, int knn_mode,
                         const float* d_radius2, int precision, int32_t* d_nn, float* d_normals, void* stream);
 
+/* sx_estimate_normals without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The neighbour rows
+ * live in a fixed 64 MiB device array of the library (one per GPU, held until
+ * the process ends), so n_q * k <= 2^24 (e.g. 1,048,576 queries at k = 16);
+ * larger ranges are split by the caller (q_start, n_q).  Every call on a GPU
+ * shares that array: calls that overlap on one GPU (different streams or
+ * threads) race.  Results equal sx_estimate_normals with d_nn == NULL. */
+int sx_estimate_normals_noalloc(sx_index* index, int64_t q_start, int64_t n_q, int k, int knn_mode,
+                                const float* d_radius2, int precision, float* d_normals, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Learned search radius (no reference counterpart: the reference search has  */
 /* no radius, src/Octree.hpp:152-155).  A 8-64-64-1 MLP, BF16 weights, FP32   */

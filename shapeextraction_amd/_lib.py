@@ -152,6 +152,7 @@ SIGNATURES = {
     "sx_knn_points": (I, [P, P, I64, I, P, P]),
     "sx_normals": (I, [P, P, I64, I64, I, I, F, F, F, P, P, P]),
     "sx_estimate_normals": (I, [P, I64, I64, I, I, P, I, P, P, P]),
+    "sx_estimate_normals_noalloc": (I, [P, I64, I64, I, I, P, I, P, P]),
     "sx_mlp_create": (I, [C.POINTER(C.c_void_p), I64]),
     "sx_mlp_destroy": (I, [P]),
     "sx_mlp_num_params": (I, []),

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "sx_internal.hpp"
 
@@ -207,6 +208,39 @@ int index_feature_arrays(sx_index* ix, const TravNode** trav, const int2** up, c
     return SX_OK;
 }
 }  // namespace sx
+
+// Neighbour rows of sx_estimate_normals_noalloc: a file-scope device array,
+// one copy per GPU for the life of the process (64 MiB), shared by every
+// call on that GPU.  Host code never passes the symbol itself to a kernel:
+// it passes the device address hipGetSymbolAddress gives for the current
+// device (noalloc_rows below).
+constexpr int64_t kNoAllocEntries = int64_t(1) << 24;  // n_q * k int32 entries at most
+static __device__ __align__(256) int32_t scratch[kNoAllocEntries];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (no allocation).
+int noalloc_rows(int32_t** out) {
+    int dev = 0;
+    SX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return sx::fail(SX_EINVAL, "sx_estimate_normals_noalloc: device id too large");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return sx::fail(SX_EHIP, std::string("sx_estimate_normals_noalloc: hipGetSymbolAddress failed: ") +
+                                         hipGetErrorString(e));
+        g_scratch[dev] = static_cast<int32_t*>(ptr);
+    }
+    *out = g_scratch[dev];
+    return SX_OK;
+}
+}  // namespace
 
 namespace sx {
 
@@ -444,6 +478,25 @@ int sx_index_get_view(sx_index* ix, sx_index_view* view, void* stream) {
     view->d_counts = ix->counts;
     view->d_bounds = ix->bounds;
     return SX_OK;
+}
+
+int sx_estimate_normals_noalloc(sx_index* ix, int64_t q_start, int64_t n_q, int k, int knn_mode,
+                                const float* d_radius2, int precision, float* d_normals, void* stream) {
+    SX_TRY(check_index(ix));
+    if (!valid_k(k)) return fail(SX_EINVAL, "sx_estimate_normals_noalloc: k must be 8, 16, 24 or 32");
+    if (n_q < 0 || n_q * k > kNoAllocEntries)
+        return fail(SX_EINVAL, "sx_estimate_normals_noalloc: n_q * k must be in [0, 2^24] (" +
+                                   std::to_string(n_q) + " x " + std::to_string(k) +
+                                   "); split the range into calls of at most 2^24 / k queries");
+    SX_TRY(check_queries(ix, q_start, n_q));
+    if (n_q == 0) return SX_OK;
+    if (!d_normals) return fail(SX_EINVAL, "sx_estimate_normals_noalloc: null normals");
+    int32_t* nn = nullptr;
+    SX_TRY(noalloc_rows(&nn));
+    // every row [0, n_q) is written by the kNN (rows past the unique count
+    // as -1), so nothing of the previous call's contents needs clearing
+    SX_TRY(sx_knn(ix, q_start, n_q, k, knn_mode, d_radius2, nn, stream));
+    return sx_normals(ix, nn, q_start, n_q, k, precision, 0.0f, 0.0f, 0.0f, d_normals, nullptr, stream);
 }
 
 int sx_index_to_input_order(sx_index* ix, const void* d_rows, int64_t row_bytes, void* d_out, void* stream) {
