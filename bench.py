@@ -47,26 +47,51 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, workload_n: int, k: int) -> dict:
     """The oracle's restatement of the reference pipeline on a bounded sample
-    of the same workload (same generator and seed), on the host cores."""
+    of the same workload (same generator and seed), on the host cores: one
+    untimed warm-up on a small cloud, then the median of three timed runs."""
     from oracle import oracle as O
 
     import shapeextraction_amd as sx
 
-    n = min(int(os.environ.get("SX_CPU_SAMPLE", 4_194_304)), workload_n)  # ~10 s on 16 host threads
+    n = min(int(os.environ.get("SX_CPU_SAMPLE", 2_097_152)), workload_n)  # ~5 s per run on 16 host threads
+    runs = int(os.environ.get("SX_CPU_RUNS", 3))
     x, y, z = sx.Workload(w.name, n, w.k, w.code_bits, w.seed, w.kind).generate()
-    t0 = time.perf_counter()
-    U, _, stage = O.pipeline(x, y, z, w.code_bits, k, knn_mode=0)
-    dt = time.perf_counter() - t0
+    O.pipeline(x[:65536].copy(), y[:65536].copy(), z[:65536].copy(), w.code_bits, k, knn_mode=0)
+    times, stages = [], []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        U, _, stage = O.pipeline(x, y, z, w.code_bits, k, knn_mode=0)
+        times.append(time.perf_counter() - t0)
+        stages.append(stage)
+    order = np.argsort(times)
+    mid = int(order[len(order) // 2])
+    dt = times[mid]
+    names = ["bounds+encode", "sort", "unique", "radix tree", "octree", "decode", "kNN (32-slot queue)", "normals"]
     return {
         "value": n / dt,
         "unit": "points/s",
         "cores": O.threads(),
         "kind": "port",
         "sample": f"{n}-point {w.kind} cloud from the same generator and seed as the {workload_n}-point "
-                  f"workload; reference pipeline restated on the host (32-slot queue kNN, FP32 Jacobi), "
-                  f"{dt:.1f} s, stage ms {[round(float(s), 1) for s in stage]}",
+                  f"workload (1/{workload_n // n} of it; every stage is linear or N log N in the points, the kNN "
+                  f"{100 * stages[mid][6] / (1000 * dt):.0f}% of the time and linear); reference pipeline restated "
+                  f"on the host (32-slot queue kNN, FP32 Jacobi); median of {runs} runs "
+                  f"({', '.join(f'{t:.2f}' for t in times)} s)",
+        "stage_ms": {nm: round(float(v), 1) for nm, v in zip(names, stages[mid])},
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
     }
 
 

@@ -149,6 +149,36 @@ def test_reference_root_differs_in_degenerate_case(O):
     assert not _tree_equal({k: v[:4] for k, v in kt.items()}, {k: v[:4] for k, v in kr.items()})
 
 
+def test_reference_root_hand_derived(O):
+    """constructTree (src/RadixTree.cu:99-194) stepped by hand with the root's
+    range end l = N-1, N = n_unique - 1 internal nodes (:128-131)."""
+    # the root's right subtree is exactly the last two leaves: the reference
+    # root spans [0, 2], so its right child is leaf 2 and leaf 3's node 2 has
+    # no parent; the textbook root spans [0, 3]
+    c = np.array([1, 2, 1 << 62, (1 << 62) + 1], np.uint64)
+    ref, txt = O.karras(c, 63, 1), O.karras(c, 63, 0)
+    assert list(ref["prefix_n"][:3]) == [0, 61, 62] and list(txt["prefix_n"][:3]) == [0, 61, 62]
+    assert list(ref["left_child"][:3]) == [1, 0, 2] and list(txt["left_child"][:3]) == [1, 0, 2]
+    assert list(ref["has_leaf_left"][:3]) == [0, 1, 1] and list(txt["has_leaf_left"][:3]) == [0, 1, 1]
+    assert list(ref["has_leaf_right"][:3]) == [1, 1, 1]
+    assert list(txt["has_leaf_right"][:3]) == [0, 1, 1]
+    assert list(ref["parent"][:3]) == [-1, 0, -1] and list(txt["parent"][:3]) == [-1, 0, 0]
+    # the last code alone is the right subtree: the reference root covers
+    # [0, 2] with delta 61, splits after leaf 0, and node 2 (range [0, 2]
+    # seen from its other end) claims node 1 as well
+    c = np.array([1, 2, 3, 1 << 62], np.uint64)
+    ref, txt = O.karras(c, 63, 1), O.karras(c, 63, 0)
+    assert list(ref["prefix_n"][:3]) == [61, 62, 61]
+    assert list(ref["left_child"][:3]) == [0, 1, 0]
+    assert list(ref["has_leaf_left"][:3]) == [1, 1, 1]
+    assert list(ref["has_leaf_right"][:3]) == [0, 1, 0]
+    assert ref["parent"][0] == -1 and ref["parent"][1] in (0, 2)
+    assert list(txt["prefix_n"][:3]) == [0, 62, 61]
+    assert list(txt["left_child"][:3]) == [2, 1, 0]
+    assert list(txt["has_leaf_left"][:3]) == [0, 1, 1] and list(txt["has_leaf_right"][:3]) == [1, 1, 0]
+    assert list(txt["parent"][:3]) == [-1, 2, 0]
+
+
 def test_radix_tree_invariants(O, sx):
     x, y, z = sx.gen_scene(21, 20000)
     uq, _ = _sorted_unique_codes(O, x, y, z, 63)
@@ -265,6 +295,83 @@ def test_reference_knn_emulation_recall(O, sx):
     ex = O.knn_brute(pts, 16, 0, 3000)
     recall = np.mean([len(set(r) & set(e)) / 16 for r, e in zip(ref, ex)])
     assert recall > 0.5  # diagnostic: the reference search is approximate
+
+
+def _otnodes(O, n):
+    nodes = np.zeros(n, O.OTNODE_DTYPE)
+    nodes["cell_size"] = 1.0
+    nodes["corner"] = -100.0
+    return nodes
+
+
+def _three_squares(lo, count, skip=()):
+    """Integer points (a, b, c) with distinct squared norms >= lo, ascending."""
+    out, t = [], lo
+    while len(out) < count:
+        hit = next(((a, b, c) for a in range(13) for b in range(a + 1) for c in range(b + 1)
+                    if a * a + b * b + c * c == t), None)
+        if hit is not None and t not in skip:
+            out.append(hit)
+        t += 1
+    return out
+
+
+def test_reference_result_queue_hand_derived(O):
+    """The k-slot result queue of knnSearchKernel (src/Octree.hpp:136-193):
+    when full, an item replaces the maximum if that maximum's priority is >=
+    its own, else it is dropped (src/dequeue.hpp:60-68); an equal priority
+    replaces.  Root (leaves 0-6, d2 0 1 2 4 5 6 9, then node 1) and node 1
+    (leaves 7-11, d2 10 3 9 11 12) visited in that order; hand-derived queue:
+    {0..6} + 7 -> full; 8 (3) evicts 7 (10); 9 (9) evicts 6 (9, equal);
+    10 and 11 dropped.  Rows ascending: 0 1 2 8 3 4 5 9 - the exact search
+    (ties by index) ends with 6 instead."""
+    pts = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (2, 0, 0), (2, 1, 0), (2, 1, 1), (3, 0, 0),
+                    (3, 1, 0), (1, 1, 1), (2, 2, 1), (3, 1, 1), (2, 2, 2)], np.float32)
+    nodes = _otnodes(O, 2)
+    nodes["children"][0, :7] = np.arange(7)
+    nodes["children"][0, 7] = 1
+    nodes["child_leaf_mask"][0] = 0x7F
+    nodes["child_node_mask"][0] = 0x80
+    nodes["children"][1, :5] = np.arange(7, 12)
+    nodes["child_leaf_mask"][1] = 0x1F
+    nodes["corner"][1] = (0.5, -100.0, -100.0)  # box distance 0.5
+    got = O.knn_reference(nodes, pts, 8, 0, 1)[0]
+    assert list(got) == [0, 1, 2, 8, 3, 4, 5, 9]
+    assert list(O.knn_brute(pts, 8, 0, 1)[0]) == [0, 1, 2, 8, 3, 4, 5, 6]
+
+
+def test_reference_search_queue_overflow_hand_derived(O):
+    """The 32-slot search queue (SEARCH_Q_SIZE, src/Octree.hpp:17) drops
+    nodes when full.  Root: child nodes C1-C7 (box distance 1 + i/64) and
+    leaf 0 (the query, at the origin); C_i: grandchildren g = 8(i-1) + s (box
+    distance 2 + g/64), each holding one point 1 + g.  Best first, C1-C3
+    queue 24 grandchildren (28 items with C4-C7); C4 adds g24-g28, then the
+    queue is full and g29-g31 exceed its maximum (g28): dropped; C5, C6, C7
+    each free one slot for g32, g40, g48 and drop the rest.  The points of
+    g29 (d2 1) and g30 (d2 3) are never seen; g48's (d2 2) evicts the
+    largest result.  Rows: 0, 49 (g48), 1 (g0, d2 50), 2-6 (g1-g5)."""
+    n_gc = 56
+    nodes = _otnodes(O, 8 + n_gc)
+    nodes["children"][0, :7] = np.arange(1, 8)
+    nodes["children"][0, 7] = 0
+    nodes["child_node_mask"][0] = 0x7F
+    nodes["child_leaf_mask"][0] = 0x80
+    for i in range(1, 8):
+        nodes["corner"][i] = (1.0 + i / 64.0, -100.0, -100.0)
+        nodes["children"][i] = 8 + 8 * (i - 1) + np.arange(8)
+        nodes["child_node_mask"][i] = 0xFF
+    for g in range(n_gc):
+        nodes["corner"][8 + g] = (2.0 + g / 64.0, -100.0, -100.0)
+        nodes["children"][8 + g, 0] = 1 + g
+        nodes["child_leaf_mask"][8 + g] = 1
+    special = {29: (1, 0, 0), 30: (1, 1, 1), 48: (1, 1, 0), 0: (5, 4, 3)}  # d2 1, 3, 2, 50
+    far = iter(_three_squares(100, n_gc))
+    pts = np.zeros((1 + n_gc, 3), np.float32)
+    for g in range(n_gc):
+        pts[1 + g] = special.get(g) or next(far)
+    got = O.knn_reference(nodes, pts, 8, 0, 1)[0]
+    assert list(got) == [0, 49, 1, 2, 3, 4, 5, 6]
+    assert list(O.knn_brute(pts, 8, 0, 1)[0]) == [0, 30, 49, 31, 1, 2, 3, 4]
 
 
 # ---------------------------------------------------------------- eigen / PCA
