@@ -282,6 +282,9 @@ constexpr int kTileRanges = kRangeFlush - 1 + 2 * kWave;  // one expansion adds 
 // 32-neighbourhoods.
 template <int K>
 constexpr int kWindow = K >= 32 ? 96 : 32;
+// LDS slots of the staged window: the tile and kWindow points either side.
+template <int K>
+constexpr int kWinSlots = kWave + 2 * kWindow<K>;
 // Candidates one segment may stream before its lanes are deferred.  The
 // check runs after each flush, so a segment that overruns has already
 // streamed about this many candidates to every lane; the deferred-query
@@ -508,6 +511,92 @@ __device__ __forceinline__ void window_merge(const float4* cand, int n, float4 q
     }
 }
 
+// ---------------------------------------------------------------------------
+// Window selection on 32-bit coarse keys.  The window is at most 256 points
+// staged in LDS; point t of it gets the key (distance bits with the low PB
+// bits cleared) | t, PB = log2(window slots).  A coarse key orders by the
+// distance rounded down to 2^PB ulps and then by window position, i.e. by
+// index, so a compare-exchange is one min and one max (a 64-bit key needs a
+// compare and four selects).  The K smallest coarse keys L then give the
+// exact K smallest (distance, index) keys whenever no discarded point shares
+// the coarse distance of L's last entry: every point of a smaller coarse
+// distance has a smaller exact distance, and there are fewer than K of them.
+// The exact keys of L are rebuilt from the staged points and sorted when two
+// entries with one coarse distance came out of order; an ambiguous boundary
+// sends the wave to the exact 64-bit merge.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void cswap32(uint32_t& x, uint32_t& y) {
+    const uint32_t a = x, b = y;
+    x = a < b ? a : b;
+    y = a < b ? b : a;
+}
+
+template <int N>
+__device__ __forceinline__ void sort_keys32(uint32_t (&x)[N]) {
+    constexpr OddEvenNet<N> net{};
+#pragma unroll
+    for (int c = 0; c < net.n; ++c) cswap32(x[net.a[c]], x[net.b[c]]);
+}
+
+// Coarse keys of the K nearest of the kSlots staged points win[0, kSlots)
+// (points past the window at +inf) into e (ascending); *dmin receives the
+// smallest key the selection discarded.
+template <int K, int kSlots>
+__device__ __forceinline__ void coarse_window(const float4* win, float4 q, uint32_t (&e)[K], uint32_t& dmin) {
+    constexpr int C = K <= 8 ? 8 : 16;
+    constexpr uint32_t kPosMask = kSlots - 1;
+    static_assert((kSlots & (kSlots - 1)) == 0 && kSlots % C == 0, "coarse_window: slots in whole chunks");
+#pragma unroll
+    for (int s = 0; s < K; ++s) e[s] = 0xFFFFFFFFu;
+    dmin = 0xFFFFFFFFu;
+#pragma unroll 1
+    for (int c0 = 0; c0 < kSlots; c0 += C) {
+        uint32_t x[C];
+#pragma unroll
+        for (int t = 0; t < C; ++t) {
+            const float4 p = win[c0 + t];
+            x[t] = (__float_as_uint(dist2(q.x, q.y, q.z, p.x, p.y, p.z)) & ~kPosMask) | static_cast<uint32_t>(c0 + t);
+        }
+        sort_keys32<C>(x);
+        // e[K-1-t] = min(e[K-1-t], x[t]) keeps the K smallest of both as a
+        // bitonic sequence; the maxima are the discarded keys
+#pragma unroll
+        for (int t = 0; t < C; ++t) {
+            const uint32_t a = e[K - 1 - t], b = x[t];
+            e[K - 1 - t] = a < b ? a : b;
+            const uint32_t hi = a < b ? b : a;
+            dmin = hi < dmin ? hi : dmin;
+        }
+#pragma unroll
+        for (int d = K / 2; d >= 1; d >>= 1)
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                if ((j & d) == 0) cswap32(e[j], e[j + d]);
+    }
+}
+
+// Exact (distance, index) keys of the coarse selection, in ascending order
+// (a slot past the window keeps its +inf distance and index, as in the exact
+// merge).
+// Returns false when the selection may differ from the exact one (a
+// discarded point shares the coarse distance of the last entry).
+template <int K, int kSlots>
+__device__ __forceinline__ bool exact_window(const float4* win, int wlo, float4 q, const uint32_t (&e)[K], uint32_t dmin,
+                                             TopK<K>& top) {
+    constexpr uint32_t kPosMask = kSlots - 1;
+    bool unsorted = false;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        const int t = static_cast<int>(e[s] & kPosMask);  // every entry is a slot: kSlots > K
+        const float4 p = win[t];
+        const uint64_t key = make_key(dist2(q.x, q.y, q.z, p.x, p.y, p.z), wlo + t);
+        top.e[s] = key;
+        if (s > 0) unsorted = unsorted || top.e[s - 1] > key;
+    }
+    if (__ballot(unsorted)) sort_keys<K>(top.e);  // equal coarse distances out of exact order (rare)
+    return e[K - 1] >= 0x7F800000u || (dmin & ~kPosMask) != (e[K - 1] & ~kPosMask);
+}
+
 // Stage the lanes' candidates (p, j) with ok set into cand[0, n), in lane
 // order, followed by one point at +inf; returns n (uniform).
 __device__ __forceinline__ int stage_candidates(float4* cand, float4 p, int j, bool ok) {
@@ -559,9 +648,17 @@ knn_tile_kernel(const float4* __restrict__ pts,
                                                              int bucket, int32_t* __restrict__ out,
                                                              int2* __restrict__ queue, int32_t* queue_count,
                                                              CloudMap cm) {
-    __shared__ int s_stack[kTileWaves][kTileStack];
-    __shared__ int2 s_range[kTileWaves][kTileRanges];
-    __shared__ int s_pre[kTileWaves][kTileRanges];
+    // per wave: the window's points, then (same memory) the range query's
+    // stack, leaf ranges and their size prefix
+    union TileScratch {
+        struct {
+            int stack[kTileStack];
+            int2 range[kTileRanges];
+            int pre[kTileRanges];
+        } q;
+        float4 win[kWinSlots<K>];
+    };
+    __shared__ TileScratch s_scr[kTileWaves];
     __shared__ float4 s_cand[kTileWaves][kWave + 1];
     const int w = static_cast<int>(threadIdx.x) / kWave;
     const int lane = static_cast<int>(lane_id());
@@ -580,9 +677,9 @@ knn_tile_kernel(const float4* __restrict__ pts,
     const int64_t qi = tile0 + lane;
     const bool valid = qi < q_end && qi < cend;
     const float4 q = valid ? pts[qi] : make_float4(0.f, 0.f, 0.f, 0.f);
-    int* stk = s_stack[w];
-    int2* rng = s_range[w];
-    int* pre = s_pre[w];
+    int* stk = s_scr[w].q.stack;
+    int2* rng = s_scr[w].q.range;
+    int* pre = s_scr[w].q.pre;
     TopK<K> top;
     top.init();
 
@@ -592,23 +689,45 @@ knn_tile_kernel(const float4* __restrict__ pts,
         constexpr int hw = kWindow<K>;
         wlo = static_cast<int>(tile0 - hw > cbase ? tile0 - hw : cbase);
         whi = static_cast<int>(tile0 + kWave - 1 + hw < cend - 1 ? tile0 + kWave - 1 + hw : cend - 1);
-        for (int base = wlo; base <= whi; base += kWave) {
-            const int j = base + lane;
-            if constexpr (kNetWindow<K>) {
-                // points past the window sit at +inf: their keys (bits
-                // 0x7F800000..) follow every finite one, so they reach the
-                // list only when the window holds fewer than K points, and
-                // such a lane never joins (its k-th distance is not finite)
-                const float4 p = j <= whi ? pts[j] : make_float4(INFINITY, INFINITY, INFINITY, 0.f);
-                s_cand[w][lane] = make_float4(p.x, p.y, p.z, __int_as_float(j));
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                window_merge<K>(s_cand[w], whi - base + 1 < kWave ? whi - base + 1 : kWave, q, top);
-            } else {
+        if constexpr (kNetWindow<K>) {
+            // the whole window in LDS; slots past it hold points at +inf,
+            // whose keys follow every finite one, so they reach the list only
+            // when the window holds fewer than K points, and such a lane
+            // never joins (its k-th distance is not finite)
+            constexpr int kSlots = kWinSlots<K>;
+            float4* win = s_scr[w].win;
+#pragma unroll
+            for (int t = lane; t < kSlots; t += kWave)
+                win[t] = wlo + t <= whi ? pts[wlo + t] : make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            bool exact_sel;
+            {
+                uint32_t ce[K];
+                uint32_t dmin;
+                coarse_window<K, kSlots>(win, q, ce, dmin);
+                exact_sel = exact_window<K, kSlots>(win, wlo, q, ce, dmin, top);
+            }
+            if (__ballot(!exact_sel)) {
+                // a boundary the coarse keys cannot settle: the exact merge,
+                // 64 staged points at a time
+                top.init();
+                for (int base = 0; base < kSlots; base += kWave) {
+                    const float4 p = win[base + lane];
+                    s_cand[w][lane] = make_float4(p.x, p.y, p.z, __int_as_float(wlo + base + lane));
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    window_merge<K>(s_cand[w], kWave, q, top);
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        } else {
+            for (int base = wlo; base <= whi; base += kWave) {
+                const int j = base + lane;
                 const float4 p = j <= whi ? pts[j] : make_float4(0.f, 0.f, 0.f, 0.f);
                 const int n = stage_candidates(s_cand[w], p, j, j <= whi);
                 broadcast_insert<K>(s_cand[w], n, q, true, 0xFFFFFFFFu, top);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
     }
     uint32_t r2bits = top.kth_bits();  // >= 0x7F800000 when the window holds fewer than K points
