@@ -282,9 +282,14 @@ constexpr int kTileRanges = kRangeFlush - 1 + 2 * kWave;  // one expansion adds 
 // 32-neighbourhoods.
 template <int K>
 constexpr int kWindow = K >= 32 ? 96 : 32;
-// LDS slots of the staged window: the tile and kWindow points either side.
+// Points of the staged window (the tile and kWindow points either side) and
+// its LDS slots (a power of two: the slot number fills the low bits of a
+// coarse key).
 template <int K>
-constexpr int kWinSlots = kWave + 2 * kWindow<K>;
+constexpr int kWinPoints = kWave + 2 * kWindow<K>;
+constexpr int pow2_ceil(int v) { return v <= 1 ? 1 : 2 * pow2_ceil((v + 1) / 2); }
+template <int K>
+constexpr int kWinSlots = pow2_ceil(kWinPoints<K>);
 // Candidates one segment may stream before its lanes are deferred.  The
 // check runs after each flush, so a segment that overruns has already
 // streamed about this many candidates to every lane; the deferred-query
@@ -470,9 +475,11 @@ __device__ __forceinline__ int range_prefix(const int2* rng, int* pre, int nr) {
 
 // Broadcast the n candidates staged in cand[0, n) (x, y, z, index bits; two
 // per step, uniform LDS reads) to every lane; a lane inserts a candidate when
-// on, its squared distance is <= tau and it beats the lane's k-th key.
+// on, it beats the lane's k-th key and (kTau) its squared distance is <= tau.
+// Without kTau the distance test is implied: a lane's radius is its k-th
+// distance when the stream starts, and the list only improves.
 // cand[n] must hold a point at +inf when n is odd.
-template <int K>
+template <int K, bool kTau>
 __device__ __forceinline__ void broadcast_insert(const float4* cand, int n, float4 q, bool on, uint32_t tau,
                                                  TopK<K>& top) {
     for (int i = 0; i < n; i += 2) {
@@ -483,8 +490,8 @@ __device__ __forceinline__ void broadcast_insert(const float4* cand, int n, floa
         const int j0 = __float_as_int(c0.w), j1 = __float_as_int(c1.w);
         const uint64_t k0 = (static_cast<uint64_t>(d0) << 32) | static_cast<uint32_t>(j0);
         const uint64_t k1 = (static_cast<uint64_t>(d1) << 32) | static_cast<uint32_t>(j1);
-        if (on && d0 <= tau && k0 < top.e[K - 1]) top.insert(k0);
-        if (on && d1 <= tau && k1 < top.e[K - 1]) top.insert(k1);
+        if (on && (!kTau || d0 <= tau) && k0 < top.e[K - 1]) top.insert(k0);
+        if (on && (!kTau || d1 <= tau) && k1 < top.e[K - 1]) top.insert(k1);
     }
 }
 
@@ -541,16 +548,17 @@ __device__ __forceinline__ void sort_keys32(uint32_t (&x)[N]) {
 // Coarse keys of the K nearest of the kSlots staged points win[0, kSlots)
 // (points past the window at +inf) into e (ascending); *dmin receives the
 // smallest key the selection discarded.
-template <int K, int kSlots>
+template <int K, int kSlots, int kPoints>
 __device__ __forceinline__ void coarse_window(const float4* win, float4 q, uint32_t (&e)[K], uint32_t& dmin) {
     constexpr int C = K <= 8 ? 8 : 16;
     constexpr uint32_t kPosMask = kSlots - 1;
-    static_assert((kSlots & (kSlots - 1)) == 0 && kSlots % C == 0, "coarse_window: slots in whole chunks");
+    static_assert((kSlots & (kSlots - 1)) == 0 && kPoints % C == 0 && kPoints <= kSlots,
+                  "coarse_window: points in whole chunks within the slots");
 #pragma unroll
     for (int s = 0; s < K; ++s) e[s] = 0xFFFFFFFFu;
     dmin = 0xFFFFFFFFu;
 #pragma unroll 1
-    for (int c0 = 0; c0 < kSlots; c0 += C) {
+    for (int c0 = 0; c0 < kPoints; c0 += C) {
         uint32_t x[C];
 #pragma unroll
         for (int t = 0; t < C; ++t) {
@@ -612,7 +620,7 @@ __device__ __forceinline__ int stage_candidates(float4* cand, float4 p, int j, b
 // coalesced); those inside E are staged and broadcast.  Indices in
 // [skip_lo, skip_hi] (the window) were seen by every lane already and are
 // not staged.
-template <int K>
+template <int K, bool kTau>
 __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, const int2* rng, int* pre, int nr,
                                               int skip_lo, int skip_hi, float ex0, float ey0, float ez0, float ex1,
                                               float ey1, float ez1, float4 q, bool on, uint32_t tau, TopK<K>& top,
@@ -628,7 +636,7 @@ __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, co
         if (j >= 0) p = pts[j];
         const bool in_e = j >= 0 && p.x >= ex0 && p.x <= ex1 && p.y >= ey0 && p.y <= ey1 && p.z >= ez0 && p.z <= ez1;
         const int n = stage_candidates(cand, p, j, in_e);
-        broadcast_insert<K>(cand, n, q, on, tau, top);
+        broadcast_insert<K, kTau>(cand, n, q, on, tau, top);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
 }
@@ -695,6 +703,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
             // when the window holds fewer than K points, and such a lane
             // never joins (its k-th distance is not finite)
             constexpr int kSlots = kWinSlots<K>;
+            constexpr int kPoints = kWinPoints<K>;
             float4* win = s_scr[w].win;
 #pragma unroll
             for (int t = lane; t < kSlots; t += kWave)
@@ -704,7 +713,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
             {
                 uint32_t ce[K];
                 uint32_t dmin;
-                coarse_window<K, kSlots>(win, q, ce, dmin);
+                coarse_window<K, kSlots, kPoints>(win, q, ce, dmin);
                 exact_sel = exact_window<K, kSlots>(win, wlo, q, ce, dmin, top);
             }
             if (__ballot(!exact_sel)) {
@@ -725,7 +734,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
                 const int j = base + lane;
                 const float4 p = j <= whi ? pts[j] : make_float4(0.f, 0.f, 0.f, 0.f);
                 const int n = stage_candidates(s_cand[w], p, j, j <= whi);
-                broadcast_insert<K>(s_cand[w], n, q, true, 0xFFFFFFFFu, top);
+                broadcast_insert<K, false>(s_cand[w], n, q, true, 0xFFFFFFFFu, top);
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
         }
@@ -851,7 +860,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
             if (overflow || nr == 0) break;
-            stream_ranges<K>(pts, rng, pre, nr, wlo, whi, ex0, ey0, ez0, ex1, ey1, ez1,
+            stream_ranges<K, kLearned>(pts, rng, pre, nr, wlo, whi, ex0, ey0, ez0, ex1, ey1, ez1,
                                        q, in_seg, tau, top, seen, s_cand[w]);
             nr = 0;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
