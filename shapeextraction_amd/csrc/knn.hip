@@ -706,8 +706,10 @@ knn_tile_kernel(const float4* __restrict__ pts,
             constexpr int kPoints = kWinPoints<K>;
             float4* win = s_scr[w].win;
 #pragma unroll
-            for (int t = lane; t < kSlots; t += kWave)
+            for (int t0 = 0; t0 < kSlots; t0 += kWave) {
+                const int t = t0 + lane;
                 win[t] = wlo + t <= whi ? pts[wlo + t] : make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+            }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             bool exact_sel;
             {
