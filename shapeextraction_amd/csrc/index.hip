@@ -33,8 +33,7 @@ size_t sort_ws_bytes(int64_t n);
 int sort_pairs_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n, int key_bits, uint64_t* keys_out,
                       uint32_t* vals_out, void* ws, int algo, hipStream_t st, int32_t* err);
 int unique_launch(const uint64_t* sorted, const uint32_t* vals, int64_t n, uint64_t* uniq, uint32_t* rep,
-                  int32_t* uos, int32_t* uoi, int32_t* d_count, void* ws, hipStream_t st, const int32_t* sort_err,
-                  const float* bounds, int code_bits, float* pts12, float4* pts16);
+                  int32_t* uos, int32_t* uoi, int32_t* d_count, void* ws, hipStream_t st, const int32_t* sort_err);
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
                   int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav, const float4* pts,
@@ -289,12 +288,11 @@ __global__ void __launch_bounds__(256) gather_rows_kernel(const uint32_t* __rest
 }  // namespace sx
 
 namespace {
-// Decode (unless the unique step did), radix tree (+ traversal boxes) and
-// octree of the unique codes, sized by counts[kCntUnique] (capacity n).
-int build_trees(sx_index* ix, int64_t n, hipStream_t st, bool decoded) {
+// Decode, radix tree (+ traversal boxes) and octree of the unique codes,
+// sized by counts[kCntUnique] (capacity n).
+int build_trees(sx_index* ix, int64_t n, hipStream_t st) {
     const int cb = ix->code_bits;
-    if (!decoded)
-        SX_TRY(sx::decode_launch(ix->ucodes, ix->counts + kCntUnique, n, ix->bounds, cb, ix->pts12, ix->pts16, st));
+    SX_TRY(sx::decode_launch(ix->ucodes, ix->counts + kCntUnique, n, ix->bounds, cb, ix->pts12, ix->pts16, st));
     if (ix->root_mode == SX_ROOT_TEXTBOOK) {
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, ix->pn, ix->lc, ix->hll,
                                  ix->hlr, ix->parent, ix->node_end, ix->leaf_parent, nullptr, ix->trav, ix->pts16,
@@ -404,11 +402,9 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
     SX_TRY(sx::sort_pairs_launch(ix->codes_in, nullptr, n, cb, ix->codes_sorted, ix->perm, ix->ws, ix->sort_algo, st,
                                  ix->counts + kCntSortErr));
     // a failed sort leaves the unique count at 0: no later stage runs on it
-    // the unique step also decodes the points (no separate decode pass)
     SX_TRY(sx::unique_launch(ix->codes_sorted, ix->perm, n, ix->ucodes, ix->rep, ix->uos, nullptr,
-                             ix->counts + kCntUnique, ix->ws, st, ix->counts + kCntSortErr, ix->bounds, cb,
-                             ix->pts12, ix->pts16));
-    SX_TRY(build_trees(ix, n, st, true));
+                             ix->counts + kCntUnique, ix->ws, st, ix->counts + kCntSortErr));
+    SX_TRY(build_trees(ix, n, st));
     ix->n_input = n;
     SX_HIP(hipEventRecord(ix->built_event, sx::as_stream(stream)));
     ix->built = true;
@@ -433,7 +429,7 @@ int sx_index_build_codes(sx_index* ix, const uint64_t* d_codes, int64_t n, float
     SX_LAUNCH_CHECK("codes_in_kernel");
     sx::codes_count_kernel<<<1, 1, 0, st>>>(n, min_coord, range, ix->bounds, ix->counts, kCntBad);
     SX_LAUNCH_CHECK("codes_count_kernel");
-    SX_TRY(build_trees(ix, n, st, false));
+    SX_TRY(build_trees(ix, n, st));
     ix->n_input = n;
     SX_HIP(hipEventRecord(ix->built_event, st));
     ix->built = true;

@@ -23,15 +23,6 @@ struct UniqueIn {
     __device__ int operator()(int64_t i) const { return (i == 0 || s[i] != s[i - 1]) ? 1 : 0; }
 };
 
-// Decoded points of the unique codes, written by the unique step itself
-// (pts12 == nullptr: no decode).
-struct UniqueDecode {
-    const float* bounds = nullptr;
-    int code_bits = 63;
-    float* pts12 = nullptr;
-    float4* pts16 = nullptr;
-};
-
 struct UniqueOut {
     const uint64_t* s;
     const uint32_t* vals;
@@ -39,24 +30,10 @@ struct UniqueOut {
     uint32_t* rep;
     int32_t* uos;  // unique id of each sorted position
     int32_t* uoi;  // unique id of each input index (needs vals)
-    UniqueDecode dec;
     __device__ void operator()(int64_t i, int ex, int v) const {
         if (v) {
-            const uint64_t c = s[i];
-            uniq[ex] = c;
+            uniq[ex] = s[i];
             if (rep) rep[ex] = vals ? vals[i] : static_cast<uint32_t>(i);
-            if (dec.pts12) {
-                // decodePoints (src/Octree.cu:14-24), as decode_kernel does it
-                const float mn = dec.bounds[0], range = dec.bounds[2];
-                const float fscale = static_cast<float>(axis_scale(dec.code_bits));
-                const float px = dequantize(compact3(c), fscale, range, mn);
-                const float py = dequantize(compact3(c >> 1), fscale, range, mn);
-                const float pz = dequantize(compact3(c >> 2), fscale, range, mn);
-                dec.pts12[3 * ex] = px;
-                dec.pts12[3 * ex + 1] = py;
-                dec.pts12[3 * ex + 2] = pz;
-                dec.pts16[ex] = make_float4(px, py, pz, 0.0f);
-            }
         }
         const int u = ex + v - 1;
         if (uos) uos[i] = u;
@@ -67,16 +44,11 @@ struct UniqueOut {
 // sort_err (nullable): the sort's error flag; when it is set the unique count
 // is written as 0, so that every later stage (sized by that count) does no
 // work on keys that are not sorted.
-// pts12/pts16 (nullable): the unique codes decoded under `bounds` in the same
-// pass (the decode kernel's arithmetic).
 int unique_launch(const uint64_t* sorted, const uint32_t* vals, int64_t n, uint64_t* uniq, uint32_t* rep,
                   int32_t* uos, int32_t* uoi, int32_t* d_count, void* ws, hipStream_t st,
-                  const int32_t* sort_err = nullptr, const float* bounds = nullptr, int code_bits = 63,
-                  float* pts12 = nullptr, float4* pts16 = nullptr) {
-    UniqueDecode dec;
-    if (pts12 && pts16 && bounds) dec = UniqueDecode{bounds, code_bits, pts12, pts16};
-    return device_scan(UniqueIn{sorted}, UniqueOut{sorted, vals, uniq, rep, uos, uoi, dec}, nullptr, n, d_count, ws,
-                       st, sort_err);
+                  const int32_t* sort_err = nullptr) {
+    return device_scan(UniqueIn{sorted}, UniqueOut{sorted, vals, uniq, rep, uos, uoi}, nullptr, n, d_count, ws, st,
+                       sort_err);
 }
 
 struct Box3 {
