@@ -270,6 +270,14 @@ int sx_index_knn_reruns(sx_index* index, int64_t* reruns, int reset, void* strea
  * time: its calls share the deferred-query queue and counters, so calls that
  * overlap on different streams race. */
 
+/* Work statistics of the tile kNN (SX_KNN_ALGO_TILES): while enabled, every
+ * sx_knn call adds the candidate points its tiles broadcast to their 64
+ * queries (the kernel's dominant work) to a device counter;
+ * sx_index_knn_candidates reads it (synchronises `stream`).  Disabled by
+ * default (one atomic per tile).  No reference counterpart. */
+int sx_index_set_knn_stats(sx_index* index, int enable);
+int sx_index_knn_candidates(sx_index* index, int64_t* candidates, int reset, void* stream);
+
 /* Exact k nearest neighbours of the index points [q_start, q_start+n_q)
  * among all index points, k in {8, 16, 24, 32}.  Row q of d_out holds k
  * indices ascending by (squared distance, index); the query itself is

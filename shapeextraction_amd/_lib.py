@@ -148,6 +148,8 @@ SIGNATURES = {
     "sx_index_set_bucket": (I, [P, I]),
     "sx_index_set_algorithms": (I, [P, I, I]),
     "sx_index_knn_reruns": (I, [P, C.POINTER(C.c_int64), I, P]),
+    "sx_index_set_knn_stats": (I, [P, I]),
+    "sx_index_knn_candidates": (I, [P, C.POINTER(C.c_int64), I, P]),
     "sx_knn": (I, [P, I64, I64, I, I, P, P, P]),
     "sx_knn_points": (I, [P, P, I64, I, P, P]),
     "sx_normals": (I, [P, P, I64, I64, I, I, F, F, F, P, P, P]),

@@ -357,6 +357,15 @@ class Index:
              stream)
         return out
 
+    def set_knn_stats(self, enable: bool = True):
+        call("sx_index_set_knn_stats", self.handle, 1 if enable else 0)
+
+    def knn_candidates(self, reset: bool = True, stream=None) -> int:
+        """Candidates the kNN tiles broadcast since the last reset (stats enabled)."""
+        r = C.c_int64(0)
+        call("sx_index_knn_candidates", self.handle, C.byref(r), 1 if reset else 0, stream)
+        return r.value
+
     def knn_reruns(self, reset: bool = True, stream=None) -> int:
         r = C.c_int64(0)
         call("sx_index_knn_reruns", self.handle, C.byref(r), 1 if reset else 0, stream)

@@ -44,7 +44,7 @@ int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
                   const float* bounds, hipStream_t st, CloudMap cm);
 int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                      int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
-                     int32_t* deferred_total, hipStream_t st, CloudMap cm, int queue_blocks);
+                     int32_t* deferred_total, hipStream_t st, CloudMap cm, int queue_blocks, int32_t* cand_total);
 int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn, int64_t q_start, int64_t n_q, int k,
                    int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st,
                    CloudMap cm);
@@ -289,7 +289,7 @@ int batch_build(sx_batch* b, const float* x, const float* y, const float* z, hip
 int batch_knn(sx_batch* b, int k, int32_t* out, int64_t n_rows, hipStream_t st) {
     return sx::knn_tiles_launch(b->pts16, b->trav, b->counts + sx::kBSlots, 0, n_rows, k, nullptr, b->bucket,
                                 out, b->queue, b->counts + sx::kBQueue, b->counts + sx::kBDeferred, st, b->map(),
-                                b->queue_blocks);
+                                b->queue_blocks, nullptr);
 }
 
 int batch_normals(sx_batch* b, const int32_t* nn, int k, int precision, float vx, float vy, float vz, float* normals,

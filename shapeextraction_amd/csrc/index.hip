@@ -48,7 +48,8 @@ int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts
                hipStream_t st, CloudMap cm = CloudMap());
 int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                      int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
-                     int32_t* deferred_total, hipStream_t st, CloudMap cm = CloudMap(), int queue_blocks = 0);
+                     int32_t* deferred_total, hipStream_t st, CloudMap cm = CloudMap(), int queue_blocks = 0,
+                     int32_t* cand_total = nullptr);
 int scatter_i32_launch(const int32_t* src, const uint32_t* idx, int64_t n, int32_t* dst, hipStream_t st);
 int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn, int64_t q_start, int64_t n_q, int k,
                    int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st,
@@ -59,7 +60,7 @@ int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn
 // look-back error flag (the first four are cleared by every build), queries
 // the kNN deferred to the per-lane search (cumulative), the last call's queue
 // size
-enum { kCntUnique = 0, kCntOct = 1, kCntBad = 2, kCntSortErr = 3, kCntReruns = 4, kCntQueue = 5 };
+enum { kCntUnique = 0, kCntOct = 1, kCntBad = 2, kCntSortErr = 3, kCntReruns = 4, kCntQueue = 5, kCntCand = 6 };
 
 
 
@@ -71,6 +72,7 @@ struct sx_index {
     int bucket = 16;
     int knn_algo = SX_KNN_ALGO_TILES;
     int sort_algo = SX_SORT_ONESWEEP;
+    bool knn_stats = false;  // count the candidates the kNN tiles broadcast
     void* mem = nullptr;
     float* bounds = nullptr;
     int32_t* counts = nullptr;
@@ -526,6 +528,23 @@ int sx_index_knn_reruns(sx_index* ix, int64_t* reruns, int reset, void* stream) 
     return SX_OK;
 }
 
+int sx_index_set_knn_stats(sx_index* ix, int enable) {
+    if (!ix) return fail(SX_EINVAL, "sx_index_set_knn_stats: null index");
+    ix->knn_stats = enable != 0;
+    return SX_OK;
+}
+
+int sx_index_knn_candidates(sx_index* ix, int64_t* candidates, int reset, void* stream) {
+    if (!ix || !candidates) return fail(SX_EINVAL, "sx_index_knn_candidates: null argument");
+    int32_t h = 0;
+    hipStream_t st = sx::as_stream(stream);
+    SX_HIP(hipMemcpyAsync(&h, ix->counts + kCntCand, 4, hipMemcpyDeviceToHost, st));
+    SX_HIP(hipStreamSynchronize(st));
+    *candidates = static_cast<uint32_t>(h);
+    if (reset) SX_TRY(sx::fill_i32(ix->counts + kCntCand, 0, 1, st));
+    return SX_OK;
+}
+
 int sx_knn(sx_index* ix, int64_t q_start, int64_t n_q, int k, int mode, const float* d_radius2, int32_t* d_out,
            void* stream) {
     SX_TRY(check_index(ix));
@@ -539,7 +558,8 @@ int sx_knn(sx_index* ix, int64_t q_start, int64_t n_q, int k, int mode, const fl
     if (ix->knn_algo == SX_KNN_ALGO_TILES)
         return sx::knn_tiles_launch(ix->pts16, ix->trav, ix->counts, q_start, n_q, k, r2, ix->bucket, d_out,
                                     ix->knn_queue, ix->counts + kCntQueue, ix->counts + kCntReruns,
-                                    sx::as_stream(stream));
+                                    sx::as_stream(stream), sx::CloudMap(), 0,
+                                    ix->knn_stats ? ix->counts + kCntCand : nullptr);
     return sx::knn_launch(ix->pts16, ix->trav, ix->counts, q_start, n_q, nullptr, k, r2, ix->bucket, d_out,
                           ix->counts + kCntReruns, sx::as_stream(stream));
 }

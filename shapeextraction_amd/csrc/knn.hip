@@ -624,7 +624,7 @@ template <int K, bool kTau>
 __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, const int2* rng, int* pre, int nr,
                                               int skip_lo, int skip_hi, float ex0, float ey0, float ez0, float ex1,
                                               float ey1, float ez1, float4 q, bool on, uint32_t tau, TopK<K>& top,
-                                              int& seen, float4* cand) {
+                                              int& seen, int& streamed, float4* cand) {
     const int lane = static_cast<int>(lane_id());
     const int total = range_prefix(rng, pre, nr);
     seen += total;
@@ -636,6 +636,7 @@ __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, co
         if (j >= 0) p = pts[j];
         const bool in_e = j >= 0 && p.x >= ex0 && p.x <= ex1 && p.y >= ey0 && p.y <= ey1 && p.z >= ez0 && p.z <= ez1;
         const int n = stage_candidates(cand, p, j, in_e);
+        streamed += n;
         broadcast_insert<K, kTau>(cand, n, q, on, tau, top);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
@@ -655,7 +656,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
                                                              int64_t n_q, const float* __restrict__ radius2,
                                                              int bucket, int32_t* __restrict__ out,
                                                              int2* __restrict__ queue, int32_t* queue_count,
-                                                             CloudMap cm) {
+                                                             int32_t* cand_total, CloudMap cm) {
     // per wave: the window's points, then (same memory) the range query's
     // stack, leaf ranges and their size prefix
     union TileScratch {
@@ -776,6 +777,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
                           (fabsf(q.x - gx) > gap || fabsf(q.y - gy) > gap || fabsf(q.z - gz) > gap));
     }
     bool lost = false;
+    int streamed = 0;   // candidates broadcast to the wave (reported when cand_total is given)
     uint32_t lim = 0u;  // squared radius within which the lane's list is proven complete
     while (todo) {
         const int s0 = static_cast<int>(__builtin_ctzll(todo));
@@ -863,7 +865,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
             }
             if (overflow || nr == 0) break;
             stream_ranges<K, kLearned>(pts, rng, pre, nr, wlo, whi, ex0, ey0, ez0, ex1, ey1, ez1,
-                                       q, in_seg, tau, top, seen, s_cand[w]);
+                                       q, in_seg, tau, top, seen, streamed, s_cand[w]);
             nr = 0;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             if (seen > kSegBudget<K>) overflow = true;
@@ -885,6 +887,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
         // when the list holds K real points (+inf bits otherwise)
         if (defer) queue[slot0 + rank_below(md)] = make_int2(static_cast<int32_t>(qi), static_cast<int>(top.kth_bits()));
     }
+    if (cand_total && lane == 0 && streamed) atomicAdd(cand_total, streamed);
 }
 
 // ---------------------------------------------------------------------------
@@ -1189,15 +1192,15 @@ __global__ void __launch_bounds__(kKnnBlock) knn_overflow_kernel(const float4* _
 template <int K>
 int knn_tiles_k(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                 const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
-                int32_t* deferred_total, hipStream_t st, CloudMap cm, int queue_blocks) {
+                int32_t* deferred_total, hipStream_t st, CloudMap cm, int queue_blocks, int32_t* cand_total) {
     const int64_t tiles = (n_q + kWave - 1) / kWave;
     const unsigned g = static_cast<unsigned>((tiles + kTileWaves - 1) / kTileWaves);
     if (radius2)
         knn_tile_kernel<K, true><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out,
-                                                         queue, queue_count, cm);
+                                                         queue, queue_count, cand_total, cm);
     else
         knn_tile_kernel<K, false><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, nullptr, bucket, out,
-                                                          queue, queue_count, cm);
+                                                          queue, queue_count, cand_total, cm);
     SX_LAUNCH_CHECK("knn_tile_kernel");
     constexpr int kGroups = kTileWaves * kGroupsPerWave;
     int64_t gq = (n_q + kGroups - 1) / kGroups;  // one group per query at most
@@ -1213,18 +1216,20 @@ int knn_tiles_k(const float4* pts, const TravNode* nodes, const int32_t* d_count
 }
 
 // Index queries through the tiles.  queue: n_q int2; queue_count: one int32,
-// cleared here; deferred_total accumulates the number of queued queries.
+// cleared here; deferred_total accumulates the number of queued queries;
+// cand_total (nullable) the candidates the tiles broadcast.
 int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                      int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
-                     int32_t* deferred_total, hipStream_t st, CloudMap cm = CloudMap(), int queue_blocks = 0) {
+                     int32_t* deferred_total, hipStream_t st, CloudMap cm = CloudMap(), int queue_blocks = 0,
+                     int32_t* cand_total = nullptr) {
     if (n_q <= 0) return SX_OK;
     if (reinterpret_cast<uintptr_t>(out) % 16 != 0) return fail(SX_EINVAL, "knn: output must be 16-byte aligned");
     SX_TRY(fill_i32(queue_count, 0, 1, st));
     switch (k) {
-        case 8: return knn_tiles_k<8>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks);
-        case 16: return knn_tiles_k<16>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks);
-        case 24: return knn_tiles_k<24>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks);
-        case 32: return knn_tiles_k<32>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks);
+        case 8: return knn_tiles_k<8>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks, cand_total);
+        case 16: return knn_tiles_k<16>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks, cand_total);
+        case 24: return knn_tiles_k<24>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks, cand_total);
+        case 32: return knn_tiles_k<32>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks, cand_total);
         default: return fail(SX_EINVAL, "knn: k must be 8, 16, 24 or 32");
     }
 }

@@ -144,3 +144,52 @@ def test_fullsize_learned_radius_rows(gpu, O, built, name, k):
     assert np.array_equal(got, plain)
     q = np.sort(rng(77).choice(U, 10_000, replace=False))
     assert np.array_equal(got[q], O.knn_grid(f["points"], k, q))
+
+
+@pytest.mark.parametrize("name,k", [("scene8m", 16), ("gmm16m", 32)])
+def test_learned_radius_gain(gpu, built, name, k):
+    """What the trained network buys: the seeded tile search broadcasts fewer
+    candidates than the plain one (its segment boxes shrink where the window
+    bound is loose), with the same rows.  Prints the candidates, the deferred
+    queries and the times (median of five calls; learned = features + MLP +
+    kNN), which are measured here but not asserted."""
+    from shapeextraction_amd import learned
+
+    x, y, z, ix, f = built(name)
+    U = f["n_unique"]
+    model = learned.RadiusModel.load(learned.default_model_path(k))
+    st = gpu.Stream()
+    nn = gpu.DeviceArray((U, k), np.int32)
+    r2 = gpu.DeviceArray((U,), np.float32)
+
+    def run(mode):
+        ix.set_knn_stats(True)
+        ix.knn_candidates(reset=True, stream=st.handle)
+        ix.knn_reruns(reset=True, stream=st.handle)
+        if mode == "learned":
+            model.predict_index_radius2(ix, out=r2, stream=st.handle)
+        ix.knn(k, mode=gpu.SX_KNN_RADIUS if mode == "learned" else gpu.SX_KNN_PLAIN,
+               radius2=r2 if mode == "learned" else None, out=nn, stream=st.handle)
+        cand = ix.knn_candidates(reset=True, stream=st.handle)
+        deferred = ix.knn_reruns(reset=True, stream=st.handle)
+        rows = nn.numpy(st.handle)
+        ix.set_knn_stats(False)
+        times = []
+        for _ in range(5):
+            e0, e1 = gpu.Event(), gpu.Event()
+            e0.record(st.handle)
+            if mode == "learned":
+                model.predict_index_radius2(ix, out=r2, stream=st.handle)
+            ix.knn(k, mode=gpu.SX_KNN_RADIUS if mode == "learned" else gpu.SX_KNN_PLAIN,
+                   radius2=r2 if mode == "learned" else None, out=nn, stream=st.handle)
+            e1.record(st.handle)
+            st.sync()
+            times.append(e0.elapsed_ms(e1))
+        return cand, deferred, float(np.median(times)), rows
+
+    cp, dp, tp, rows_p = run("plain")
+    cl, dl, tl, rows_l = run("learned")
+    print(f"[{name} k={k}] plain: {cp} candidates, {dp} deferred, {tp:.3f} ms | learned (radius + kNN): "
+          f"{cl} candidates ({cl / cp:.3f}x), {dl} deferred, {tl:.3f} ms ({tp / tl:.3f}x)")
+    assert np.array_equal(rows_l, rows_p)
+    assert cl < cp
