@@ -60,19 +60,40 @@ struct TopK {
     // keys that rank in the upper half of their lists: the lower half is
     // shifted only when some active lane's key belongs there (a uniform
     // branch), which halves the typical cost of a wave-wide insert.
-    __device__ __forceinline__ void insert_wave(uint64_t key) {
-        constexpr int H = K / 2;
-        if (__ballot(key < e[H - 1])) {
-            insert(key);
-            return;
-        }
+    // Shift slots [lo, hi): the insert's formula per slot, highest first (a
+    // slot reads its lower neighbour before that neighbour is shifted).
+    template <int lo, int hi>
+    __device__ __forceinline__ void shift_slots(uint64_t key) {
 #pragma unroll
-        for (int s = K - 1; s > H; --s) {
-            const uint64_t prev = e[s - 1];
+        for (int s = hi - 1; s >= lo; --s) {
             const uint64_t cur = e[s];
-            e[s] = key < prev ? prev : (key < cur ? key : cur);
+            e[s] = s > 0 && key < e[s > 0 ? s - 1 : 0] ? e[s > 0 ? s - 1 : 0] : (key < cur ? key : cur);
         }
-        e[H] = key < e[H] ? key : e[H];
+    }
+    // insert() for the active lanes of a wave, half the list at a time: the
+    // lower half is shifted only when some active lane's key ranks there.
+    __device__ __forceinline__ void insert_wave_half(uint64_t key) {
+        const bool lower = __ballot(key < e[K / 2 - 1]) != 0;
+        shift_slots<K / 2, K>(key);
+        if (lower) shift_slots<0, K / 2>(key);
+    }
+    // insert() for the active lanes of a wave, one quarter of the list at a
+    // time from the top: a lower quarter is shifted only when some active
+    // lane's key ranks in or below it (uniform branches), so a wave whose
+    // inserted keys all rank near the end of their lists shifts a quarter.
+    __device__ __forceinline__ void insert_wave(uint64_t key) {
+        constexpr int Q = K / 4;
+        const bool q2 = __ballot(key < e[3 * Q - 1]) != 0;
+        shift_slots<3 * Q, K>(key);
+        if (q2) {
+            const bool q1 = __ballot(key < e[2 * Q - 1]) != 0;
+            shift_slots<2 * Q, 3 * Q>(key);
+            if (q1) {
+                const bool q0 = __ballot(key < e[Q - 1]) != 0;
+                shift_slots<Q, 2 * Q>(key);
+                if (q0) shift_slots<0, Q>(key);
+            }
+        }
     }
 };
 
@@ -508,8 +529,15 @@ __device__ __forceinline__ void broadcast_insert(const float4* cand, int n, floa
         const int j0 = __float_as_int(c0.w), j1 = __float_as_int(c1.w);
         const uint64_t k0 = (static_cast<uint64_t>(d0) << 32) | static_cast<uint32_t>(j0);
         const uint64_t k1 = (static_cast<uint64_t>(d1) << 32) | static_cast<uint32_t>(j1);
-        if (on && (!kTau || d0 <= tau) && k0 < top.e[K - 1]) top.insert_wave(k0);
-        if (on && (!kTau || d1 <= tau) && k1 < top.e[K - 1]) top.insert_wave(k1);
+        // (the seeded search with K <= 16 at seven waves keeps more values
+        // live: there the half-wise insert spills less than the quarter-wise)
+        constexpr bool kHalf = kTau && K <= 16;
+        if (on && (!kTau || d0 <= tau) && k0 < top.e[K - 1]) {
+            if constexpr (kHalf) top.insert_wave_half(k0); else top.insert_wave(k0);
+        }
+        if (on && (!kTau || d1 <= tau) && k1 < top.e[K - 1]) {
+            if constexpr (kHalf) top.insert_wave_half(k1); else top.insert_wave(k1);
+        }
     }
 }
 
