@@ -16,6 +16,7 @@ sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -220,8 +221,49 @@ def main() -> int:
             f"{np.median(res['normals_ms']):.2f}), reruns/step {res['reruns']:.0f}")
         return res
 
+    def host_io(runs=5):
+        """The plain step with the coordinates copied in from pinned host
+        memory and the normals copied back: the rate a caller holding host
+        buffers sees (never `value`)."""
+        nb_in, nb_out = 4 * n, 12 * U
+        bufs = []
+        for nb in (nb_in, nb_in, nb_in, nb_out):
+            p = ctypes.c_void_p()
+            sx._lib.call("sx_host_malloc", ctypes.byref(p), nb)
+            bufs.append(p)
+        try:
+            for a, p in zip((x, y, z), bufs):
+                ctypes.memmove(p, a.ctypes.data, nb_in)
+            times, h2d, d2h = [], [], []
+            for _ in range(runs + 1):
+                e = [sx.Event() for _ in range(3)]
+                st.sync()
+                t0 = time.perf_counter()
+                e[0].record(st.handle)
+                for d, p in zip((dx, dy, dz), bufs):
+                    sx._lib.call("sx_memcpy_htod", d.ptr, p, nb_in, st.handle)
+                e[1].record(st.handle)
+                step("plain")
+                e[2].record(st.handle)
+                sx._lib.call("sx_memcpy_dtoh", bufs[3], normals.ptr, nb_out, st.handle)  # synchronises
+                times.append(time.perf_counter() - t0)
+                h2d.append(e[0].elapsed_ms(e[1]))
+                d2h.append((times[-1] * 1000.0) - e[0].elapsed_ms(e[2]))
+            times, h2d, d2h = times[1:], h2d[1:], d2h[1:]  # first run warms the copy path
+        finally:
+            for p in bufs:
+                sx._lib.call("sx_host_free", p)
+        ms = float(np.median(times)) * 1000.0
+        res = {"ms_per_step": ms, "points_per_s": n / (ms / 1000.0), "h2d_ms": float(np.median(h2d)),
+               "d2h_ms": float(np.median(d2h)), "bytes_in": 3 * nb_in, "bytes_out": nb_out,
+               "note": "pinned host xyz -> HBM, plain step, normals -> pinned host; median of "
+                       f"{runs} runs (d2h includes the wait for the step's last kernel)"}
+        log(f"host buffers: {ms:.2f} ms/step (h2d {res['h2d_ms']:.2f}, d2h {res['d2h_ms']:.2f})")
+        return res
+
     modes = ["plain"] + (["learned"] if model is not None else [])
     results = {m: measure(m) for m in modes}
+    pcie = host_io()
     if "learned" in results:
         assert np.array_equal(results["learned"]["nn"], results["plain"]["nn"]), "learned kNN differs from plain"
         assert np.array_equal(results["learned"]["normals"], results["plain"]["normals"])
@@ -298,6 +340,7 @@ def main() -> int:
             "achieved_GBs": pipe_bytes / (ms_per_step / 1000.0) / 1e9,
             "frac": pipe_bytes / (ms_per_step / 1000.0) / 1e9 / HBM_PEAK_GBS,
         },
+        "host_buffers": pcie,
     }
     if cls is not None:
         line["per_class_angular_error_deg"] = per_class_errors(sx, ix, nn, normals, cls, dirs, k, st)

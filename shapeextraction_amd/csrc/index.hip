@@ -59,7 +59,7 @@ int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn
 // counts[] slots: unique points, octree nodes, non-finite inputs, the sort's
 // look-back error flag (the first four are cleared by every build), queries
 // the kNN deferred to the per-lane search (cumulative), the last call's queue
-// size
+// size, candidates the tile search broadcast (cumulative, when counting is on)
 enum { kCntUnique = 0, kCntOct = 1, kCntBad = 2, kCntSortErr = 3, kCntReruns = 4, kCntQueue = 5, kCntCand = 6 };
 
 
