@@ -42,8 +42,8 @@ int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
                   int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav, const float4* pts,
                   const float* bounds, hipStream_t st, CloudMap cm);
-int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
-                     int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
+int knn_tiles_launch(const float4* pts, const TravNode* nodes, ParentRef par, const int32_t* d_counts, int64_t q_start,
+                     int64_t n_q, int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
                      int32_t* deferred_total, hipStream_t st, CloudMap cm, int queue_blocks, int32_t* cand_total);
 int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn, int64_t q_start, int64_t n_q, int k,
                    int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st,
@@ -287,7 +287,9 @@ int batch_build(sx_batch* b, const float* x, const float* y, const float* z, hip
 // Rows are written for slots < min(n_rows, slot count): n_rows is the row
 // capacity of the caller's buffers.
 int batch_knn(sx_batch* b, int k, int32_t* out, int64_t n_rows, hipStream_t st) {
-    return sx::knn_tiles_launch(b->pts16, b->trav, b->counts + sx::kBSlots, 0, n_rows, k, nullptr, b->bucket,
+    sx::ParentRef par;
+    par.p = b->parent;
+    return sx::knn_tiles_launch(b->pts16, b->trav, par, b->counts + sx::kBSlots, 0, n_rows, k, nullptr, b->bucket,
                                 out, b->queue, b->counts + sx::kBQueue, b->counts + sx::kBDeferred, st, b->map(),
                                 b->queue_blocks, nullptr);
 }

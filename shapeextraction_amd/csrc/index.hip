@@ -43,11 +43,11 @@ int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
                   const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
                   const int32_t* parent, sx_otnode* nodes, int64_t cap, int32_t* d_oct_count, int32_t* leaf_oct,
                   void* ws, hipStream_t st, const int32_t* node_end = nullptr, int2* aux = nullptr);
-int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
-               const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
+int knn_launch(const float4* pts, const TravNode* nodes, ParentRef par, const int32_t* d_counts, int64_t q_start,
+               int64_t n_q, const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
                hipStream_t st, CloudMap cm = CloudMap());
-int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
-                     int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
+int knn_tiles_launch(const float4* pts, const TravNode* nodes, ParentRef par, const int32_t* d_counts, int64_t q_start,
+                     int64_t n_q, int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
                      int32_t* deferred_total, hipStream_t st, CloudMap cm = CloudMap(), int queue_blocks = 0,
                      int32_t* cand_total = nullptr);
 int scatter_i32_launch(const int32_t* src, const uint32_t* idx, int64_t n, int32_t* dst, hipStream_t st);
@@ -147,6 +147,21 @@ size_t index_layout(sx_index* ix, void* base) {
     const size_t ws = std::max({sx::sort_ws_bytes(ix->cap), sx::scan_ws_bytes(ix->cap), sx::octree_ws_bytes(ix->cap)});
     ix->ws = c.take<char>(ws);
     return c.off;
+}
+
+// Parents of the traversal tree: the radix tree's own array with the
+// textbook root; with the reference root the traversal tree is the separate
+// textbook tree, whose parents are the .x of its (parent, prefix) pairs.
+sx::ParentRef trav_parents(const sx_index* ix) {
+    sx::ParentRef r;
+    if (ix->root_mode == SX_ROOT_TEXTBOOK) {
+        r.p = ix->parent;
+        r.stride = 1;
+    } else {
+        r.p = reinterpret_cast<const int32_t*>(ix->up);
+        r.stride = 2;
+    }
+    return r;
 }
 
 int check_index(const sx_index* ix) {
@@ -556,11 +571,11 @@ int sx_knn(sx_index* ix, int64_t q_start, int64_t n_q, int k, int mode, const fl
     SX_TRY(after_build(ix, stream));
     const float* r2 = mode == SX_KNN_RADIUS ? d_radius2 : nullptr;
     if (ix->knn_algo == SX_KNN_ALGO_TILES)
-        return sx::knn_tiles_launch(ix->pts16, ix->trav, ix->counts, q_start, n_q, k, r2, ix->bucket, d_out,
+        return sx::knn_tiles_launch(ix->pts16, ix->trav, trav_parents(ix), ix->counts, q_start, n_q, k, r2, ix->bucket, d_out,
                                     ix->knn_queue, ix->counts + kCntQueue, ix->counts + kCntReruns,
                                     sx::as_stream(stream), sx::CloudMap(), 0,
                                     ix->knn_stats ? ix->counts + kCntCand : nullptr);
-    return sx::knn_launch(ix->pts16, ix->trav, ix->counts, q_start, n_q, nullptr, k, r2, ix->bucket, d_out,
+    return sx::knn_launch(ix->pts16, ix->trav, trav_parents(ix), ix->counts, q_start, n_q, nullptr, k, r2, ix->bucket, d_out,
                           ix->counts + kCntReruns, sx::as_stream(stream));
 }
 
@@ -569,7 +584,7 @@ int sx_knn_points(sx_index* ix, const float* d_queries, int64_t n_q, int k, int3
     if (!valid_k(k)) return fail(SX_EINVAL, "sx_knn_points: k must be 8, 16, 24 or 32");
     if (n_q < 0 || (n_q > 0 && (!d_queries || !d_out))) return fail(SX_EINVAL, "sx_knn_points: bad arguments");
     SX_TRY(after_build(ix, stream));
-    return sx::knn_launch(ix->pts16, ix->trav, ix->counts, 0, n_q, d_queries, k, nullptr, ix->bucket, d_out, nullptr,
+    return sx::knn_launch(ix->pts16, ix->trav, trav_parents(ix), ix->counts, 0, n_q, d_queries, k, nullptr, ix->bucket, d_out, nullptr,
                           sx::as_stream(stream));
 }
 

@@ -141,7 +141,7 @@ __device__ __forceinline__ void write_missing_row(int32_t* row) {
 // gamma and the right child's gamma+1, for leaves and internal nodes alike.
 // The cloud spans points (and radix nodes) [base, base + U); its root is node base.
 template <int K>
-__device__ void knn_search(const float4* __restrict__ pts, const TravNode* __restrict__ nodes, int base, int U,
+__device__ void knn_search(const float4* __restrict__ pts, const TravNode* __restrict__ nodes, ParentRef par, int base, int U,
                            float qx, float qy, float qz, uint32_t r2bits, int bucket, TopK<K>& top) {
     bool scan = U <= bucket;
     bool done = scan;
@@ -156,8 +156,10 @@ __device__ void knn_search(const float4* __restrict__ pts, const TravNode* __res
         }
         const uint32_t kb = top.kth_bits();
         const uint32_t bound = kb < r2bits ? kb : r2bits;
-        const TravNode n = nodes[down ? node : p];
-        const int a = w_int(n.lo_l.w), g = w_int(n.hi_l.w), b = w_int(n.lo_r.w), parent = w_int(n.hi_r.w);
+        const int cur = down ? node : p;
+        const TravNode n = nodes[cur];
+        const int parent = par.at(cur);
+        const int a = w_int(n.lo_l.w), g = w_int(n.hi_l.w), b = w_int(n.lo_r.w);
         const float dl = box_dist2(qx, qy, qz, n.lo_l, n.hi_l);
         const float dr = box_dist2(qx, qy, qz, n.lo_r, n.hi_r);
         const bool lf = dl <= dr;
@@ -209,7 +211,7 @@ constexpr int kKnnBlock = 256;
 
 template <int K, bool kExternal>
 __global__ void __launch_bounds__(kKnnBlock) knn_kernel(const float4* __restrict__ pts,
-                                                        const TravNode* __restrict__ nodes,
+                                                        const TravNode* __restrict__ nodes, ParentRef par,
                                                         const int32_t* __restrict__ d_counts, int64_t q_start,
                                                         int64_t n_q, const float* __restrict__ queries,
                                                         const float* __restrict__ radius2, int bucket,
@@ -250,7 +252,7 @@ __global__ void __launch_bounds__(kKnnBlock) knn_kernel(const float4* __restrict
     // A seeded search that finds fewer than K points within R is not proven
     // exact: it is repeated unseeded (one loop so the search is inlined once).
     for (int attempt = 0; attempt < 2; ++attempt) {
-        knn_search<K>(pts, nodes, base, U, qx, qy, qz, r2bits, bucket, top);
+        knn_search<K>(pts, nodes, par, base, U, qx, qy, qz, r2bits, bucket, top);
         if (r2bits == 0xFFFFFFFFu || top.kth_bits() <= r2bits) break;
         if (reruns) atomicAdd(reruns, 1);
         top.init();
@@ -260,15 +262,15 @@ __global__ void __launch_bounds__(kKnnBlock) knn_kernel(const float4* __restrict
 }
 
 template <bool kExternal>
-int knn_dispatch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
+int knn_dispatch(const float4* pts, const TravNode* nodes, ParentRef par, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                  const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
                  hipStream_t st, CloudMap cm) {
     const unsigned g = grid_for(n_q, kKnnBlock);
     switch (k) {
-        case 8: knn_kernel<8, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns, cm); break;
-        case 16: knn_kernel<16, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns, cm); break;
-        case 24: knn_kernel<24, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns, cm); break;
-        case 32: knn_kernel<32, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns, cm); break;
+        case 8: knn_kernel<8, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, par, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns, cm); break;
+        case 16: knn_kernel<16, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, par, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns, cm); break;
+        case 24: knn_kernel<24, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, par, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns, cm); break;
+        case 32: knn_kernel<32, kExternal><<<g, kKnnBlock, 0, st>>>(pts, nodes, par, d_counts, q_start, n_q, queries, radius2, bucket, out, reruns, cm); break;
         default: return fail(SX_EINVAL, "knn: k must be 8, 16, 24 or 32");
     }
     SX_LAUNCH_CHECK("knn_kernel");
@@ -277,15 +279,16 @@ int knn_dispatch(const float4* pts, const TravNode* nodes, const int32_t* d_coun
 
 // queries == nullptr: the index points [q_start, q_start + n_q) are the
 // queries; otherwise n_q external points (12-byte AoS).
-int knn_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
-               const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
+int knn_launch(const float4* pts, const TravNode* nodes, ParentRef par, const int32_t* d_counts, int64_t q_start,
+               int64_t n_q, const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
                hipStream_t st, CloudMap cm = CloudMap()) {
     if (n_q <= 0) return SX_OK;
     if (reinterpret_cast<uintptr_t>(out) % 16 != 0) return fail(SX_EINVAL, "knn: output must be 16-byte aligned");
     if (queries)
-        return knn_dispatch<true>(pts, nodes, d_counts, q_start, n_q, queries, k, radius2, bucket, out, reruns, st,
+        return knn_dispatch<true>(pts, nodes, par, d_counts, q_start, n_q, queries, k, radius2, bucket, out, reruns, st,
                                   CloudMap());
-    return knn_dispatch<false>(pts, nodes, d_counts, q_start, n_q, nullptr, k, radius2, bucket, out, reruns, st, cm);
+    return knn_dispatch<false>(pts, nodes, par, d_counts, q_start, n_q, nullptr, k, radius2, bucket, out, reruns, st,
+                               cm);
 }
 
 
@@ -1214,7 +1217,7 @@ __global__ void __launch_bounds__(kKnnBlock) __attribute__((amdgpu_waves_per_eu(
 // knn_search bounded by the entry's k-th distance bound.
 template <int K>
 __global__ void __launch_bounds__(kKnnBlock) knn_overflow_kernel(const float4* __restrict__ pts,
-                                                                 const TravNode* __restrict__ nodes,
+                                                                 const TravNode* __restrict__ nodes, ParentRef par,
                                                                  const int32_t* __restrict__ d_counts,
                                                                  int64_t q_start, const int2* __restrict__ queue,
                                                                  const int32_t* __restrict__ queue_count,
@@ -1230,13 +1233,13 @@ __global__ void __launch_bounds__(kKnnBlock) knn_overflow_kernel(const float4* _
         const int2 cloud = cm.at(qi, total);
         TopK<K> top;
         top.init();
-        knn_search<K>(pts, nodes, cloud.x, cloud.y, q.x, q.y, q.z, static_cast<uint32_t>(e.y), bucket, top);
+        knn_search<K>(pts, nodes, par, cloud.x, cloud.y, q.x, q.y, q.z, static_cast<uint32_t>(e.y), bucket, top);
         write_row<K>(out + static_cast<int64_t>(qi - q_start) * K, top, cloud.x);
     }
 }
 
 template <int K>
-int knn_tiles_k(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
+int knn_tiles_k(const float4* pts, const TravNode* nodes, ParentRef par, const int32_t* d_counts, int64_t q_start, int64_t n_q,
                 const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
                 int32_t* deferred_total, hipStream_t st, CloudMap cm, int queue_blocks, int32_t* cand_total) {
     const int64_t tiles = (n_q + kWave - 1) / kWave;
@@ -1255,7 +1258,7 @@ int knn_tiles_k(const float4* pts, const TravNode* nodes, const int32_t* d_count
     knn_group_kernel<K><<<static_cast<unsigned>(gq), kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, queue,
                                                                        queue_count, bucket, out, deferred_total, cm);
     SX_LAUNCH_CHECK("knn_group_kernel");
-    knn_overflow_kernel<K><<<64, kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, queue, queue_count, bucket, out,
+    knn_overflow_kernel<K><<<64, kKnnBlock, 0, st>>>(pts, nodes, par, d_counts, q_start, queue, queue_count, bucket, out,
                                                      cm);
     SX_LAUNCH_CHECK("knn_overflow_kernel");
     return SX_OK;
@@ -1264,18 +1267,18 @@ int knn_tiles_k(const float4* pts, const TravNode* nodes, const int32_t* d_count
 // Index queries through the tiles.  queue: n_q int2; queue_count: one int32,
 // cleared here; deferred_total accumulates the number of queued queries;
 // cand_total (nullable) the candidates the tiles broadcast.
-int knn_tiles_launch(const float4* pts, const TravNode* nodes, const int32_t* d_counts, int64_t q_start, int64_t n_q,
-                     int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
+int knn_tiles_launch(const float4* pts, const TravNode* nodes, ParentRef par, const int32_t* d_counts, int64_t q_start,
+                     int64_t n_q, int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
                      int32_t* deferred_total, hipStream_t st, CloudMap cm = CloudMap(), int queue_blocks = 0,
                      int32_t* cand_total = nullptr) {
     if (n_q <= 0) return SX_OK;
     if (reinterpret_cast<uintptr_t>(out) % 16 != 0) return fail(SX_EINVAL, "knn: output must be 16-byte aligned");
     SX_TRY(fill_i32(queue_count, 0, 1, st));
     switch (k) {
-        case 8: return knn_tiles_k<8>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks, cand_total);
-        case 16: return knn_tiles_k<16>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks, cand_total);
-        case 24: return knn_tiles_k<24>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks, cand_total);
-        case 32: return knn_tiles_k<32>(pts, nodes, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks, cand_total);
+        case 8: return knn_tiles_k<8>(pts, nodes, par, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks, cand_total);
+        case 16: return knn_tiles_k<16>(pts, nodes, par, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks, cand_total);
+        case 24: return knn_tiles_k<24>(pts, nodes, par, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks, cand_total);
+        case 32: return knn_tiles_k<32>(pts, nodes, par, d_counts, q_start, n_q, radius2, bucket, out, queue, queue_count, deferred_total, st, cm, queue_blocks, cand_total);
         default: return fail(SX_EINVAL, "knn: k must be 8, 16, 24 or 32");
     }
 }

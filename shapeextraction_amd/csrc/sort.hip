@@ -57,16 +57,24 @@ __global__ void __launch_bounds__(kSortBlock) radix_hist_kernel(const uint64_t* 
     hist[static_cast<int64_t>(threadIdx.x) * n_tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Lanes of the wave whose digit equals this lane's digit.
-__device__ __forceinline__ uint64_t match_digit(uint32_t d) {
-    uint64_t m = ~0ull;
+// Rank of this lane's digit among the wave's: *below = lanes under this one
+// with this lane's digit, *total = lanes with it.  Built from the complement
+// of the match set (one ballot per digit bit): with
+// s = -1 where bit b of this lane's digit is set (0 otherwise), s ^ ballot(bit b)
+// marks the lanes whose bit b differs, and OR over the bits marks every lane
+// whose digit differs - two 32-bit XORs and ORs per bit instead of a 64-bit
+// select and AND.
+__device__ __forceinline__ void rank_digit(uint32_t d, uint32_t& below, uint32_t& total) {
+    uint32_t nlo = 0u, nhi = 0u;
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-        const uint32_t bit = (d >> b) & 1u;
-        const uint64_t bal = __ballot(bit);
-        m &= bit ? bal : ~bal;
+        const int32_t sb = static_cast<int32_t>(d << (31 - b)) >> 31;
+        const uint64_t bal = __ballot(sb != 0);
+        nlo |= static_cast<uint32_t>(sb) ^ static_cast<uint32_t>(bal);
+        nhi |= static_cast<uint32_t>(sb) ^ static_cast<uint32_t>(bal >> 32);
     }
-    return m;
+    below = __builtin_amdgcn_mbcnt_hi(~nhi, __builtin_amdgcn_mbcnt_lo(~nlo, 0u));
+    total = static_cast<uint32_t>(__popc(~nlo) + __popc(~nhi));
 }
 
 __global__ void __launch_bounds__(kSortBlock) radix_scatter_kernel(const uint64_t* __restrict__ keys_in,
@@ -83,7 +91,6 @@ __global__ void __launch_bounds__(kSortBlock) radix_scatter_kernel(const uint64_
     for (int d = static_cast<int>(lane); d < kRadix; d += kWave) cnt[w][d] = 0;
     __syncthreads();
 
-    const uint64_t lt = (1ull << lane) - 1ull;
     const int64_t span = static_cast<int64_t>(blockIdx.x) * kSortTile + static_cast<int64_t>(w) * kWaveSpan;
     uint64_t k[kSortItems];
     uint32_t v[kSortItems];
@@ -95,10 +102,11 @@ __global__ void __launch_bounds__(kSortBlock) radix_scatter_kernel(const uint64_
         k[r] = valid ? keys_in[i] : ~0ull;
         v[r] = valid ? (vals_in ? vals_in[i] : static_cast<uint32_t>(i)) : 0u;
         const uint32_t d = static_cast<uint32_t>(k[r] >> shift) & (kRadix - 1);
-        const uint64_t m = match_digit(d);
+        uint32_t below, total;
+        rank_digit(d, below, total);
         const uint32_t c = cnt[w][d];
-        rank[r] = c + static_cast<uint32_t>(__popcll(m & lt));
-        if (((m >> lane) >> 1) == 0) cnt[w][d] = c + static_cast<uint32_t>(__popcll(m));
+        rank[r] = c + below;
+        if (below + 1u == total) cnt[w][d] = c + total;
     }
     __syncthreads();
     {
@@ -213,7 +221,6 @@ __global__ void __launch_bounds__(kPassBlock) __attribute__((amdgpu_waves_per_eu
     __syncthreads();
     const uint32_t tile = s_tile;
     const int64_t tile_base = static_cast<int64_t>(tile) * kSortTile;
-    const uint64_t lt = (1ull << lane) - 1ull;
     const int64_t span = tile_base + static_cast<int64_t>(w) * kPassSpan;
     uint64_t k[kPassItems];
     uint32_t v[kPassItems];
@@ -228,10 +235,11 @@ __global__ void __launch_bounds__(kPassBlock) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
     for (int r = 0; r < kPassItems; ++r) {
         const uint32_t d = static_cast<uint32_t>(k[r] >> shift) & (kRadix - 1);
-        const uint64_t m = match_digit(d);
+        uint32_t below, total;
+        rank_digit(d, below, total);
         const uint32_t c = cnt[w][d];
-        rank[r] = c + static_cast<uint32_t>(__popcll(m & lt));
-        if (((m >> lane) >> 1) == 0) cnt[w][d] = c + static_cast<uint32_t>(__popcll(m));
+        rank[r] = c + below;
+        if (below + 1u == total) cnt[w][d] = c + total;
     }
     __syncthreads();
     // per digit (thread tid < 256 = digit): wave offsets and the tile's count,

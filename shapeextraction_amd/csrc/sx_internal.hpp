@@ -102,16 +102,27 @@ struct Carver {
 // Traversal record of internal radix-tree node i (a BVH node over the sorted
 // unique points): the boxes of its two children plus, in the .w lanes, the
 // leaf range [a, b] it covers, its split gamma (left child covers [a, gamma],
-// right child [gamma+1, b]) and its parent (-1 at the root).  64 bytes.
+// right child [gamma+1, b]).  64 bytes, written whole by the node's own
+// thread; parents live in a separate array (ParentRef).
 struct __align__(16) TravNode {
     float4 lo_l;  // w: a
     float4 hi_l;  // w: gamma
     float4 lo_r;  // w: b
-    float4 hi_r;  // w: parent
+    float4 hi_r;  // w: 0
 };
 static_assert(sizeof(TravNode) == 64, "TravNode must be 64 bytes");
 
 __device__ __forceinline__ int w_int(float w) { return __float_as_int(w); }
+
+// Parent of each internal node of the traversal tree (-1 at a cloud's root),
+// read by the per-lane search when it climbs back up: int32 at
+// p[node * stride] (the radix tree's parent array, or the .x of (parent,
+// prefix) pairs).
+struct ParentRef {
+    const int32_t* p = nullptr;
+    int stride = 1;
+    __device__ __forceinline__ int at(int node) const { return p[static_cast<int64_t>(node) * stride]; }
+};
 
 // ---------------------------------------------------------------------------
 // Device-wide scan of int32 values (reduce, then scan of tile sums, then scan

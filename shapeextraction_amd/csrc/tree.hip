@@ -273,8 +273,7 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
     }
     if (!active) return;
     // the record of node g: child boxes with (first leaf, split, last leaf)
-    // in .w, written whole; .w of hi_r (the parent) is written by the
-    // parent's thread, so hi_r's box goes out component-wise
+    // in .w, written whole by this thread (parents are in their own array)
     const bool far_l = !hl && !(A >= blk0 && G < blk1), far_r = !hr && !(G + 1 >= blk0 && B < blk1);
     float mn = 0.0f, range = 0.0f;
     if (far_l || far_r) {
@@ -313,15 +312,7 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
     t->lo_l = make_float4(bl.lx, bl.ly, bl.lz, __int_as_float(static_cast<int>(A)));
     t->hi_l = make_float4(bl.hx, bl.hy, bl.hz, __int_as_float(static_cast<int>(G)));
     t->lo_r = make_float4(br.lx, br.ly, br.lz, __int_as_float(static_cast<int>(B)));
-    if (i == 0) {
-        t->hi_r = make_float4(br.hx, br.hy, br.hz, __int_as_float(-1));
-    } else {
-        t->hi_r.x = br.hx;
-        t->hi_r.y = br.hy;
-        t->hi_r.z = br.hz;
-    }
-    if (!hl) trav[G].hi_r.w = __int_as_float(static_cast<int>(g));
-    if (!hr) trav[G + 1].hi_r.w = __int_as_float(static_cast<int>(g));
+    t->hi_r = make_float4(br.hx, br.hy, br.hz, 0.0f);
 }
 
 // With trav, pts (the decoded points, slot numbering) and bounds (the cube:
