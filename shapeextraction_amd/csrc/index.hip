@@ -42,7 +42,11 @@ size_t octree_ws_bytes(int64_t n_max);
 int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, const float* bounds,
                   const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
                   const int32_t* parent, sx_otnode* nodes, int64_t cap, int32_t* d_oct_count, int32_t* leaf_oct,
-                  void* ws, hipStream_t st, const int32_t* node_end = nullptr, int2* aux = nullptr);
+                  void* ws, hipStream_t st, const int32_t* node_end = nullptr, int2* aux = nullptr,
+                  int32_t* keep_offsets = nullptr, int32_t* keep_counts = nullptr);
+int feature_links_launch(const int32_t* d_count, int64_t n_max, const int32_t* parent, const int32_t* leaf_parent,
+                         const int32_t* node_end, const int32_t* offsets, const int32_t* counts, int64_t cap,
+                         int32_t* leaf_oct, int2* aux, hipStream_t st);
 int knn_launch(const float4* pts, const TravNode* nodes, ParentRef par, const int32_t* d_counts, int64_t q_start,
                int64_t n_q, const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
                hipStream_t st, CloudMap cm = CloudMap());
@@ -95,6 +99,9 @@ struct sx_index {
     int32_t* leaf_oct = nullptr;
     int2* oct_aux = nullptr;  // (parent octree node, points in the cell) per octree node
     int32_t* node_end = nullptr;  // the far end of each radix node's leaf range
+    int32_t* oct_off = nullptr;   // first octree node of each radix node's chain
+    int32_t* oct_cnt = nullptr;   // its chain length
+    bool feat_valid = false;      // leaf_oct / oct_aux hold the last build's links
     int2* knn_queue = nullptr;  // deferred queries: (index, k-th distance bound bits)
     sx::TravNode* trav = nullptr;
     sx_otnode* oct = nullptr;
@@ -143,6 +150,8 @@ size_t index_layout(sx_index* ix, void* base) {
     ix->oct = c.take<sx_otnode>(static_cast<size_t>(ix->oct_cap));
     ix->oct_aux = c.take<int2>(static_cast<size_t>(ix->oct_cap));
     ix->node_end = c.take<int32_t>(n);
+    ix->oct_off = c.take<int32_t>(n);
+    ix->oct_cnt = c.take<int32_t>(n);
     ix->ws_bbox = c.take<char>(sx::bbox_ws_bytes());
     const size_t ws = std::max({sx::sort_ws_bytes(ix->cap), sx::scan_ws_bytes(ix->cap), sx::octree_ws_bytes(ix->cap)});
     ix->ws = c.take<char>(ws);
@@ -211,6 +220,12 @@ int index_feature_arrays(sx_index* ix, const TravNode** trav, const int2** up, c
     // textbook root: the octree links give the counts and the API's prefix
     // array the depth; otherwise the traversal tree's (parent, prefix) pairs
     const bool textbook = ix->root_mode == SX_ROOT_TEXTBOOK;
+    if (!ix->feat_valid) {
+        SX_TRY(feature_links_launch(ix->counts + kCntUnique, ix->n_input, ix->parent, ix->leaf_parent, ix->node_end,
+                                    ix->oct_off, ix->oct_cnt, ix->oct_cap, ix->leaf_oct, ix->oct_aux,
+                                    as_stream(stream)));
+        ix->feat_valid = true;
+    }
     *trav = ix->trav;
     *up = textbook ? nullptr : ix->up;
     *oct_aux = textbook ? ix->oct_aux : nullptr;
@@ -322,12 +337,16 @@ int build_trees(sx_index* ix, int64_t n, hipStream_t st) {
                                  nullptr, nullptr, nullptr, nullptr, ix->leaf_parent, ix->up, ix->trav, ix->pts16,
                                  ix->bounds, st));
     }
-    // with the textbook root the octree and the traversal tree are one tree:
-    // record each octree node's parent and point count for the features
+    // With the textbook root the octree and the traversal tree are one tree:
+    // the feature links (leaf -> octree node, each octree node's parent and
+    // point count) are derived from the kept chain offsets when features are
+    // first asked for.  With the reference root the gather places each leaf.
     const bool textbook = ix->root_mode == SX_ROOT_TEXTBOOK;
     SX_TRY(sx::octree_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, ix->bounds, ix->pn, ix->lc, ix->hll, ix->hlr,
-                             ix->parent, ix->oct, ix->oct_cap, ix->counts + kCntOct, ix->leaf_oct, ix->ws, st,
-                             textbook ? ix->node_end : nullptr, textbook ? ix->oct_aux : nullptr));
+                             ix->parent, ix->oct, ix->oct_cap, ix->counts + kCntOct,
+                             textbook ? nullptr : ix->leaf_oct, ix->ws, st, nullptr, nullptr, ix->oct_off,
+                             ix->oct_cnt));
+    ix->feat_valid = !textbook;
     return SX_OK;
 }
 

@@ -258,22 +258,94 @@ size_t octree_ws_bytes(int64_t n_max) {
     return c.off;
 }
 
-// offsets/counts are carved from ws unless given (the index keeps them).
+// offsets/counts (chain start and length of each radix node) are carved
+// from ws unless the caller keeps them (keep_offsets, keep_counts).
 int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, const float* bounds,
                   const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
                   const int32_t* parent, sx_otnode* nodes, int64_t cap, int32_t* d_oct_count, int32_t* leaf_oct,
-                  void* ws, hipStream_t st, const int32_t* node_end = nullptr, int2* aux = nullptr) {
+                  void* ws, hipStream_t st, const int32_t* node_end = nullptr, int2* aux = nullptr,
+                  int32_t* keep_offsets = nullptr, int32_t* keep_counts = nullptr) {
     if (n_max <= 0) return SX_OK;
     Carver c(ws);
     int32_t* offsets = c.take<int32_t>(static_cast<size_t>(n_max));
     int32_t* counts = c.take<int32_t>(static_cast<size_t>(n_max));
     void* scan_ws = c.take<char>(scan_ws_bytes(n_max));
+    if (keep_offsets) offsets = keep_offsets;
+    if (keep_counts) counts = keep_counts;
     SX_TRY(device_scan(OctCountIn{pn, parent, d_count}, OctCountOut{offsets, counts}, d_count, n_max, d_oct_count,
                        scan_ws, st));
     oct_gather_kernel<<<grid_for(n_max, kGatherBlock), kGatherBlock, 0, st>>>(codes, d_count, n_max, code_bits, bounds, pn, left_child,
                                                             hll, hlr, offsets, counts, nodes, cap, leaf_oct, node_end,
                                                             node_end ? aux : nullptr);
     SX_LAUNCH_CHECK("oct_gather_kernel");
+    return SX_OK;
+}
+
+// The octree links the learned-radius features read, for an octree built on
+// the traversal tree itself (textbook root), from the arrays the build keeps:
+//   leaf_oct[u]: the octree node holding point u - the bottom chain node
+//     (offsets[q]) of the nearest radix ancestor q of u with a non-zero
+//     chain, as the gather places leaves;
+//   aux[o] = (parent octree node, points in the cell) for every chain node:
+//     a chain node's parent is the next one up, the top's parent is the
+//     bottom node of the nearest ancestor with a chain (-1 at the root), and
+//     every node of radix node i's chain holds exactly i's leaf range.
+// The walks are short: a radix node without a chain lies at its parent's
+// octree depth, and prefix lengths grow by at least one bit per level.
+__device__ __forceinline__ int chain_holder(const int32_t* __restrict__ parent, const int32_t* __restrict__ counts,
+                                            int p, int count_p) {
+#pragma unroll 1
+    for (int s = 0; s < 64 && p > 0 && count_p == 0; ++s) {  // the root's count is 1
+        p = parent[p];
+        count_p = p >= 0 ? counts[p] : 1;
+    }
+    return p < 0 ? 0 : p;
+}
+
+// One thread per point t (leaf link) and radix node t (its chain's links);
+// the loads of each round are independent, so a thread waits for three
+// memory round trips when no walk is needed (most threads).
+__global__ void __launch_bounds__(256) feature_links_kernel(const int32_t* d_count, int64_t n_max,
+                                                            const int32_t* __restrict__ parent,
+                                                            const int32_t* __restrict__ leaf_parent,
+                                                            const int32_t* __restrict__ node_end,
+                                                            const int32_t* __restrict__ offsets,
+                                                            const int32_t* __restrict__ counts, int64_t cap,
+                                                            int32_t* __restrict__ leaf_oct, int2* __restrict__ aux) {
+    const int64_t U = scan_count(d_count, n_max);
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (U < 2 || t >= U) return;
+    const int i = static_cast<int>(t);
+    const bool node = t < U - 1;
+    // round 1
+    const int lp = leaf_parent[i] >> 1;
+    const int c = node ? counts[i] : 0;
+    const int pa = node && i > 0 ? parent[i] : 0;
+    const int base = node ? offsets[i] : 0;
+    const int e = node ? node_end[i] : i;
+    // round 2
+    const int c_lp = counts[lp];
+    const int c_pa = counts[pa];
+    const int hl = chain_holder(parent, counts, lp, c_lp);
+    const int hp = chain_holder(parent, counts, pa, c_pa);
+    // round 3
+    leaf_oct[i] = offsets[hl];
+    if (c <= 0) return;
+    const int top_parent = i == 0 ? -1 : offsets[hp];
+    const int sz = (e > i ? e - i : i - e) + 1;
+    for (int j = 0; j < c; ++j) {
+        if (base + j >= cap) break;
+        aux[base + j] = make_int2(j + 1 < c ? base + j + 1 : top_parent, sz);
+    }
+}
+
+int feature_links_launch(const int32_t* d_count, int64_t n_max, const int32_t* parent, const int32_t* leaf_parent,
+                         const int32_t* node_end, const int32_t* offsets, const int32_t* counts, int64_t cap,
+                         int32_t* leaf_oct, int2* aux, hipStream_t st) {
+    if (n_max <= 0) return SX_OK;
+    feature_links_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(d_count, n_max, parent, leaf_parent, node_end, offsets,
+                                                               counts, cap, leaf_oct, aux);
+    SX_LAUNCH_CHECK("feature_links_kernel");
     return SX_OK;
 }
 
