@@ -37,13 +37,13 @@ int unique_launch(const uint64_t* sorted, const uint32_t* vals, int64_t n, uint6
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
                   int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav, const float4* pts,
-                  const float* bounds, hipStream_t st, CloudMap cm = CloudMap());
+                  const float* bounds, hipStream_t st, CloudMap cm = CloudMap(), int32_t* oct_count = nullptr);
 size_t octree_ws_bytes(int64_t n_max);
 int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, const float* bounds,
                   const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
                   const int32_t* parent, sx_otnode* nodes, int64_t cap, int32_t* d_oct_count, int32_t* leaf_oct,
-                  void* ws, hipStream_t st, const int32_t* node_end = nullptr, int2* aux = nullptr,
-                  int32_t* keep_offsets = nullptr, int32_t* keep_counts = nullptr);
+                  void* ws, hipStream_t st, int32_t* keep_offsets = nullptr, int32_t* keep_counts = nullptr,
+                  bool counts_given = false);
 int feature_links_launch(const int32_t* d_count, int64_t n_max, const int32_t* parent, const int32_t* leaf_parent,
                          const int32_t* node_end, const int32_t* offsets, const int32_t* counts, int64_t cap,
                          int32_t* leaf_oct, int2* aux, hipStream_t st);
@@ -328,7 +328,7 @@ int build_trees(sx_index* ix, int64_t n, hipStream_t st) {
     if (ix->root_mode == SX_ROOT_TEXTBOOK) {
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, ix->pn, ix->lc, ix->hll,
                                  ix->hlr, ix->parent, ix->node_end, ix->leaf_parent, nullptr, ix->trav, ix->pts16,
-                                 ix->bounds, st));
+                                 ix->bounds, st, sx::CloudMap(), ix->oct_cnt));
     } else {
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_REFERENCE, ix->pn, ix->lc,
                                  ix->hll, ix->hlr, ix->parent, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
@@ -344,8 +344,8 @@ int build_trees(sx_index* ix, int64_t n, hipStream_t st) {
     const bool textbook = ix->root_mode == SX_ROOT_TEXTBOOK;
     SX_TRY(sx::octree_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, ix->bounds, ix->pn, ix->lc, ix->hll, ix->hlr,
                              ix->parent, ix->oct, ix->oct_cap, ix->counts + kCntOct,
-                             textbook ? nullptr : ix->leaf_oct, ix->ws, st, nullptr, nullptr, ix->oct_off,
-                             ix->oct_cnt));
+                             textbook ? nullptr : ix->leaf_oct, ix->ws, st, ix->oct_off, ix->oct_cnt,
+                             textbook));
     ix->feat_valid = !textbook;
     return SX_OK;
 }

@@ -32,6 +32,18 @@ struct OctCountIn {
     }
 };
 
+// Edge counts already written by the radix-tree kernel (textbook root).
+struct OctCountGiven {
+    const int32_t* counts;
+    const int32_t* d_count;
+    __device__ int operator()(int64_t i) const { return i < static_cast<int64_t>(*d_count) - 1 ? counts[i] : 0; }
+};
+
+struct OctOffsetOut {
+    int32_t* offsets;
+    __device__ void operator()(int64_t i, int ex, int) const { offsets[i] = ex; }
+};
+
 struct OctCountOut {
     int32_t* offsets;
     int32_t* counts;
@@ -102,6 +114,27 @@ __device__ __forceinline__ void store_node(sx_otnode* nodes, int64_t cap, int o,
     w[6] = make_int2(node_mask, leaf_mask);
 }
 
+// The nodes of a block's radix nodes form one contiguous run of the node
+// array (the exclusive scan of the chain lengths); when the run fits in the
+// block's LDS (reused once the gather is done) it is staged there and written
+// out as contiguous 8-byte words, so a wave's stores cover whole cache lines
+// instead of seven scattered 8-byte pieces per node.  Larger runs are written
+// node by node.
+constexpr int kGatherLds = kGatherWin * (8 + 4 + 4 + 4 + 1);
+constexpr int kStageNodes = kGatherLds / static_cast<int>(sizeof(sx_otnode));
+static_assert(sizeof(sx_otnode) == 56, "octree nodes are seven 8-byte words");
+
+__device__ __forceinline__ void stage_node(int2* w, const int32_t ch[8], float3 corner, float cell, int node_mask,
+                                           int leaf_mask) {
+    w[0] = make_int2(ch[0], ch[1]);
+    w[1] = make_int2(ch[2], ch[3]);
+    w[2] = make_int2(ch[4], ch[5]);
+    w[3] = make_int2(ch[6], ch[7]);
+    w[4] = make_int2(__float_as_int(corner.x), __float_as_int(corner.y));
+    w[5] = make_int2(__float_as_int(corner.z), __float_as_int(cell));
+    w[6] = make_int2(node_mask, leaf_mask);
+}
+
 __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count,
                                                          int64_t n_max, int code_bits, const float* __restrict__ bounds,
                                                          const uint8_t* __restrict__ pn,
@@ -110,17 +143,20 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
                                                          const uint8_t* __restrict__ hlr,
                                                          const int32_t* __restrict__ offsets,
                                                          const int32_t* __restrict__ counts, sx_otnode* nodes,
-                                                         int64_t cap, int32_t* leaf_oct,
-                                                         const int32_t* __restrict__ node_end, int2* aux) {
-    __shared__ uint64_t s_code[kGatherWin];
-    __shared__ int32_t s_lc[kGatherWin], s_cnt[kGatherWin], s_off[kGatherWin];
-    __shared__ uint8_t s_leaf[kGatherWin];  // bit 0: left child a leaf, bit 1: right child a leaf
+                                                         int64_t cap, int32_t* leaf_oct) {
+    // the radix arrays of the block's window, then (same memory) the staged nodes
+    __shared__ __align__(16) unsigned char s_raw[kGatherLds];
+    uint64_t* s_code = reinterpret_cast<uint64_t*>(s_raw);
+    int32_t* s_lc = reinterpret_cast<int32_t*>(s_code + kGatherWin);
+    int32_t* s_cnt = s_lc + kGatherWin;
+    int32_t* s_off = s_cnt + kGatherWin;
+    uint8_t* s_leaf = reinterpret_cast<uint8_t*>(s_off + kGatherWin);  // bit 0: left child a leaf, bit 1: right
     const int64_t U = scan_count(d_count, n_max);
     const int64_t M = U - 1;
     // XCD-contiguous block order: a block's halo is its neighbours' nodes, read
     // from the same L2
     const int64_t blk0 = static_cast<int64_t>(xcd_block()) * kGatherBlock;
-    if (blk0 >= M) return;  // whole block idle (before the barrier)
+    if (blk0 >= M) return;  // whole block idle (before the barriers)
     const int64_t w0 = blk0 - kGatherHalo > 0 ? blk0 - kGatherHalo : 0;
     const int64_t w1 = blk0 + kGatherBlock + kGatherHalo < U ? blk0 + kGatherBlock + kGatherHalo : U;
     for (int64_t x = w0 + threadIdx.x; x < w1; x += kGatherBlock) {
@@ -162,92 +198,121 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
         if (!in_win(x)) v = offsets[x];
         return v;
     };
+    // the block's run of nodes [o_begin, o_begin + n_run)
+    const int64_t last = (blk0 + kGatherBlock < M ? blk0 + kGatherBlock : M) - 1;
+    const int o_begin = s_off[blk0 - w0];
+    const int n_run = s_off[last - w0] + s_cnt[last - w0] - o_begin;
+    const bool staged = n_run <= kStageNodes;  // uniform
+
     const int64_t i = blk0 + threadIdx.x;
-    if (i >= M) return;
-    const int c = cnt_at(i);
-    if (c <= 0) return;
+    const int c = i < M ? cnt_at(i) : 0;
     const float mn = bounds[0], range = bounds[2];
     const float fscale = static_cast<float>(axis_scale(code_bits));
     const int root_level = pn[0] / 3;
-    const int D = pn[i] / 3;
-    const int base = off_at(i);
-    const uint64_t ci = code_at(i);
-    if (aux) {
-        // (parent octree node, points in the cell) of radix node i's chain:
-        // each chain node's parent is the next one up; the top's parent is
-        // written by the ancestor that gathers it (below), the root's is -1
-        const int e = node_end[i];
-        const int sz = (e > i ? e - static_cast<int>(i) : static_cast<int>(i) - e) + 1;
-        for (int j = 0; j < c; ++j) {
-            if (base + j >= cap) break;
-            if (j + 1 < c) aux[base + j] = make_int2(base + j + 1, sz);
-            else if (i == 0) aux[base + j] = make_int2(-1, sz);
-            else aux[base + j].y = sz;
-        }
-    }
+    int D = 0, base = 0;
+    uint64_t ci = 0;
     int32_t ch[8];
-    const RawXYZ ri = raw_xyz(ci);
-    // chain nodes above the bottom
-    for (int j = 1; j < c; ++j) {
-        const int L = D - j;
-#pragma unroll
-        for (int s = 0; s < 8; ++s) ch[s] = 0;
-        const int slot = static_cast<int>(code_prefix(ci, L + 1, code_bits) & 7);
-        ch[slot] = base + j - 1;
-        store_node(nodes, cap, base + j, ch, cell_corner_raw(ri, L, code_bits, mn, range, fscale),
-                   ldexpf(range, root_level - L), 1 << slot, 0);
-    }
-    // bottom node: gather children through count-0 radix descendants
 #pragma unroll
     for (int s = 0; s < 8; ++s) ch[s] = 0;
     int node_mask = 0, leaf_mask = 0;
-    int stack[8];
-    int sp = 0;
-    int cur = static_cast<int>(i);
-    while (true) {
-        int g;
-        uint32_t leafs;
-        g = s_lc[clampw(cur)];
-        leafs = s_leaf[clampw(cur)];
-        asm volatile("" : "+v"(g), "+v"(leafs));
-        if (!in_win(cur)) {  // internal nodes are below M, so inside the window they are staged
-            g = left_child[cur];
-            leafs = (hll[cur] != 0 ? 1u : 0u) | (hlr[cur] != 0 ? 2u : 0u);
-        }
+    if (c > 0) {
+        D = pn[i] / 3;
+        base = off_at(i);
+        ci = code_at(i);
+        // bottom node: gather children through count-0 radix descendants
+        int stack[8];
+        int sp = 0;
+        int cur = static_cast<int>(i);
+        while (true) {
+            int g;
+            uint32_t leafs;
+            g = s_lc[clampw(cur)];
+            leafs = s_leaf[clampw(cur)];
+            asm volatile("" : "+v"(g), "+v"(leafs));
+            if (!in_win(cur)) {  // internal nodes are below M, so inside the window they are staged
+                g = left_child[cur];
+                leafs = (hll[cur] != 0 ? 1u : 0u) | (hlr[cur] != 0 ? 2u : 0u);
+            }
 #pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            const int child = g + side;
-            if ((leafs >> side) & 1u) {
-                const int slot = static_cast<int>(code_prefix(code_at(child), D + 1, code_bits) & 7);
-                ch[slot] = child;
-                leaf_mask |= 1 << slot;
-                if (leaf_oct) leaf_oct[child] = base;
-                continue;
+            for (int side = 0; side < 2; ++side) {
+                const int child = g + side;
+                if ((leafs >> side) & 1u) {
+                    const int slot = static_cast<int>(code_prefix(code_at(child), D + 1, code_bits) & 7);
+                    ch[slot] = child;
+                    leaf_mask |= 1 << slot;
+                    if (leaf_oct) leaf_oct[child] = base;
+                    continue;
+                }
+                const int cc = cnt_at(child);
+                if (cc > 0) {
+                    const int slot = static_cast<int>(code_prefix(code_at(child), D + 1, code_bits) & 7);
+                    ch[slot] = off_at(child) + cc - 1;
+                    node_mask |= 1 << slot;
+                } else if (sp < 8) {
+                    stack[sp++] = child;
+                }
             }
-            const int cc = cnt_at(child);
-            if (cc > 0) {
-                const int slot = static_cast<int>(code_prefix(code_at(child), D + 1, code_bits) & 7);
-                const int top = off_at(child) + cc - 1;
-                ch[slot] = top;
-                node_mask |= 1 << slot;
-                if (aux && top < cap) aux[top].x = base;
-            } else if (sp < 8) {
-                stack[sp++] = child;
-            }
+            if (sp == 0) break;
+            cur = stack[--sp];
         }
-        if (sp == 0) break;
-        cur = stack[--sp];
     }
-    float3 corner;
-    float cell;
-    if (i == 0) {
-        corner = cell_corner(code_prefix(codes[0], root_level, code_bits), root_level, code_bits, mn, range, fscale);
-        cell = range;
-    } else {
-        corner = cell_corner_raw(ri, D, code_bits, mn, range, fscale);
-        cell = ldexpf(range, root_level - D);
+    float3 corner = make_float3(0.f, 0.f, 0.f);
+    float cell = 0.0f;
+    const RawXYZ ri = raw_xyz(ci);
+    if (c > 0) {
+        if (i == 0) {
+            corner = cell_corner(code_prefix(codes[0], root_level, code_bits), root_level, code_bits, mn, range, fscale);
+            cell = range;
+        } else {
+            corner = cell_corner_raw(ri, D, code_bits, mn, range, fscale);
+            cell = ldexpf(range, root_level - D);
+        }
     }
-    store_node(nodes, cap, base, ch, corner, cell, node_mask, leaf_mask);
+    // Chain nodes above the bottom hold one child each, the chain node below:
+    // node base + j sits at depth D - j.
+    auto chain_node = [&](int j, int32_t (&cj)[8], float3& cc, float& cs, int& cm) {
+        const int L = D - j;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) cj[s] = 0;
+        const int slot = static_cast<int>(code_prefix(ci, L + 1, code_bits) & 7);
+        cj[slot] = base + j - 1;
+        cc = cell_corner_raw(ri, L, code_bits, mn, range, fscale);
+        cs = ldexpf(range, root_level - L);
+        cm = 1 << slot;
+    };
+    if (!staged) {
+        if (c > 0) {
+            int32_t cj[8];
+            float3 cc;
+            float cs;
+            int cm;
+            for (int j = 1; j < c; ++j) {
+                chain_node(j, cj, cc, cs, cm);
+                store_node(nodes, cap, base + j, cj, cc, cs, cm, 0);
+            }
+            store_node(nodes, cap, base, ch, corner, cell, node_mask, leaf_mask);
+        }
+        return;  // uniform: the whole block takes this path
+    }
+    __syncthreads();  // every read of the staged radix arrays is done
+    int2* stage = reinterpret_cast<int2*>(s_raw);
+    if (c > 0) {
+        int32_t cj[8];
+        float3 cc;
+        float cs;
+        int cm;
+        for (int j = 1; j < c; ++j) {
+            chain_node(j, cj, cc, cs, cm);
+            stage_node(stage + 7 * (base + j - o_begin), cj, cc, cs, cm, 0);
+        }
+        stage_node(stage + 7 * (base - o_begin), ch, corner, cell, node_mask, leaf_mask);
+    }
+    __syncthreads();
+    int2* out = reinterpret_cast<int2*>(nodes) + 7 * static_cast<int64_t>(o_begin);
+    const int64_t words = 7 * static_cast<int64_t>(n_run);
+    const int64_t cap_words = 7 * (cap - o_begin);
+    for (int e = static_cast<int>(threadIdx.x); e < words; e += kGatherBlock)
+        if (e < cap_words) out[e] = stage[e];
 }
 
 size_t octree_ws_bytes(int64_t n_max) {
@@ -259,12 +324,14 @@ size_t octree_ws_bytes(int64_t n_max) {
 }
 
 // offsets/counts (chain start and length of each radix node) are carved
-// from ws unless the caller keeps them (keep_offsets, keep_counts).
+// from ws unless the caller keeps them (keep_offsets, keep_counts); with
+// counts_given the radix-tree kernel has already written the counts into
+// keep_counts, and only their scan remains.
 int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, const float* bounds,
                   const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
                   const int32_t* parent, sx_otnode* nodes, int64_t cap, int32_t* d_oct_count, int32_t* leaf_oct,
-                  void* ws, hipStream_t st, const int32_t* node_end = nullptr, int2* aux = nullptr,
-                  int32_t* keep_offsets = nullptr, int32_t* keep_counts = nullptr) {
+                  void* ws, hipStream_t st, int32_t* keep_offsets = nullptr, int32_t* keep_counts = nullptr,
+                  bool counts_given = false) {
     if (n_max <= 0) return SX_OK;
     Carver c(ws);
     int32_t* offsets = c.take<int32_t>(static_cast<size_t>(n_max));
@@ -272,11 +339,15 @@ int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
     void* scan_ws = c.take<char>(scan_ws_bytes(n_max));
     if (keep_offsets) offsets = keep_offsets;
     if (keep_counts) counts = keep_counts;
-    SX_TRY(device_scan(OctCountIn{pn, parent, d_count}, OctCountOut{offsets, counts}, d_count, n_max, d_oct_count,
-                       scan_ws, st));
-    oct_gather_kernel<<<grid_for(n_max, kGatherBlock), kGatherBlock, 0, st>>>(codes, d_count, n_max, code_bits, bounds, pn, left_child,
-                                                            hll, hlr, offsets, counts, nodes, cap, leaf_oct, node_end,
-                                                            node_end ? aux : nullptr);
+    if (counts_given && keep_counts)
+        SX_TRY(device_scan(OctCountGiven{keep_counts, d_count}, OctOffsetOut{offsets}, d_count, n_max, d_oct_count,
+                           scan_ws, st));
+    else
+        SX_TRY(device_scan(OctCountIn{pn, parent, d_count}, OctCountOut{offsets, counts}, d_count, n_max,
+                           d_oct_count, scan_ws, st));
+    oct_gather_kernel<<<grid_for(n_max, kGatherBlock), kGatherBlock, 0, st>>>(codes, d_count, n_max, code_bits, bounds, pn,
+                                                                               left_child, hll, hlr, offsets, counts,
+                                                                               nodes, cap, leaf_oct);
     SX_LAUNCH_CHECK("oct_gather_kernel");
     return SX_OK;
 }

@@ -162,15 +162,30 @@ constexpr uint32_t kFlagAgg = 1u << 30, kFlagPrefix = 2u << 30, kCountMask = (1u
 constexpr int kMaxPasses = 8;
 constexpr uint32_t kDefaultSpinLimit = 1u << 22;
 
+// A block takes chunks of kHistItems keys per thread: the chunk's loads are
+// issued together, then counted, so a wave waits for one memory round trip
+// per chunk rather than one per key.
+constexpr int kHistItems = 16;
+
 __global__ void __launch_bounds__(kSortBlock) onesweep_hist_kernel(const uint64_t* __restrict__ keys, int64_t n,
                                                                    int passes, uint32_t* __restrict__ ghist) {
     __shared__ uint32_t h[kMaxPasses][kRadix];
     for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kSortBlock) (&h[0][0])[i] = 0;
     __syncthreads();
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * kSortBlock;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kSortBlock + threadIdx.x; i < n; i += stride) {
-        const uint64_t k = keys[i];
-        for (int p = 0; p < passes; ++p) atomicAdd(&h[p][static_cast<uint32_t>(k >> (8 * p)) & (kRadix - 1)], 1u);
+    constexpr int64_t kChunk = static_cast<int64_t>(kSortBlock) * kHistItems;
+    for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * kChunk; c0 < n; c0 += static_cast<int64_t>(gridDim.x) * kChunk) {
+        uint64_t k[kHistItems];
+#pragma unroll
+        for (int j = 0; j < kHistItems; ++j) {
+            const int64_t i = c0 + j * kSortBlock + threadIdx.x;
+            k[j] = i < n ? keys[i] : 0ull;
+        }
+#pragma unroll
+        for (int j = 0; j < kHistItems; ++j) {
+            if (c0 + j * kSortBlock + threadIdx.x >= n) continue;
+            for (int p = 0; p < passes; ++p)
+                atomicAdd(&h[p][static_cast<uint32_t>(k[j] >> (8 * p)) & (kRadix - 1)], 1u);
+        }
     }
     __syncthreads();
     for (int p = 0; p < passes; ++p) {
@@ -383,8 +398,11 @@ int onesweep_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t n,
     const size_t clear = reinterpret_cast<char*>(status + static_cast<size_t>(passes) * kRadix * tiles) -
                          reinterpret_cast<char*>(ghist);
     SX_TRY(fill_u8(ghist, 0, clear, st));
-    unsigned hb = grid_for(n, kSortBlock * 16);
-    if (hb > 2048) hb = 2048;
+    // 512 blocks at most: each block's 2048 flush atomics land on the same
+    // 2048 device-wide counters, so fewer, longer blocks contend less
+    // (8M keys: 51 us at 2048 blocks, 29 us at 512, 33 us at 256)
+    unsigned hb = grid_for(n, kSortBlock * kHistItems);
+    if (hb > 512) hb = 512;
     onesweep_hist_kernel<<<hb, kSortBlock, 0, st>>>(keys_in, n, passes, ghist);
     SX_LAUNCH_CHECK("onesweep_hist_kernel");
     onesweep_base_kernel<<<passes, kRadix, 0, st>>>(ghist);

@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(kKarrasBlock)
 karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
               uint8_t* prefix_n, int32_t* left_child, uint8_t* has_leaf_left, uint8_t* has_leaf_right,
               int32_t* parent, int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav,
-              const float4* pts, const float* __restrict__ bounds, CloudMap cm) {
+              const float4* pts, const float* __restrict__ bounds, CloudMap cm, int32_t* oct_count) {
     __shared__ uint64_t s_codes[kKarrasBlock + 2 * kKarrasHalo];
     __shared__ float4 s_pts[kKarrasBlock];  // points of the block's own slots
     __shared__ float4 s_lo[kKarrasBlock], s_hi[kKarrasBlock];  // box of each in-block node
@@ -224,6 +224,19 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
             if (!hr) parent[G + 1] = static_cast<int32_t>(g);
         }
         if (node_end) node_end[g] = static_cast<int32_t>(base + j);
+        if (oct_count) {
+            // octree edge count depth(node) - depth(parent), depth = prefix / 3
+            // (calcEdgeCounts, src/Octree.cu:38-50): the parent's prefix is the
+            // longer of the two prefixes across the node's range ends, since
+            // the parent adds the sibling on that side; the root counts 1
+            int cnt = 1;
+            if (i > 0) {
+                const int dl = a > 0 ? prefix_len(code(a - 1), code(a), code_bits) : -1;
+                const int dr = b < N ? prefix_len(code(b), code(b + 1), code_bits) : -1;
+                cnt = dnode / 3 - (dl > dr ? dl : dr) / 3;
+            }
+            oct_count[g] = cnt;
+        }
         if (leaf_parent) {
             if (hl) leaf_parent[G] = static_cast<int32_t>(2 * g);
             if (hr) leaf_parent[G + 1] = static_cast<int32_t>(2 * g + 1);
@@ -321,7 +334,7 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
                   int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav, const float4* pts,
-                  const float* bounds, hipStream_t st, CloudMap cm = CloudMap()) {
+                  const float* bounds, hipStream_t st, CloudMap cm = CloudMap(), int32_t* oct_count = nullptr) {
     if (n_max <= 0) return SX_OK;
     if (trav && (!pts || !bounds)) return fail(SX_EINVAL, "karras_launch: traversal records need points and bounds");
     // Nodes that no other node claims (only possible under the reference root
@@ -329,7 +342,7 @@ int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
     if (parent) SX_TRY(fill_i32(parent, -1, n_max, st));
     karras_kernel<<<grid_for(n_max, kKarrasBlock), kKarrasBlock, 0, st>>>(
         codes, d_count, n_max, code_bits, root_mode, prefix_n, left_child, hll, hlr, parent, node_end, leaf_parent, up,
-        trav, pts, bounds, cm);
+        trav, pts, bounds, cm, oct_count);
     SX_LAUNCH_CHECK("karras_kernel");
     return SX_OK;
 }
