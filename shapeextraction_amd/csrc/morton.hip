@@ -18,7 +18,7 @@
 namespace sx {
 
 constexpr int kBBoxBlock = 256;
-constexpr int kBBoxMaxBlocks = 1024;
+constexpr int kBBoxMaxBlocks = 4096;
 
 __device__ __forceinline__ void bbox_acc(float v, float& mn, float& mx, int& bad) {
     const bool ok = isfinite(v);
@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(kBBoxBlock) bbox_kernel(const float* __restric
         const float4* x4 = reinterpret_cast<const float4*>(x);
         const float4* y4 = reinterpret_cast<const float4*>(y);
         const float4* z4 = reinterpret_cast<const float4*>(z);
-#pragma unroll 2
+#pragma unroll 4
         for (int64_t i = tid; i < n4; i += stride) {
             const float4 a = x4[i], b = y4[i], c = z4[i];
             bbox_acc(a.x, mn, mx, bad); bbox_acc(a.y, mn, mx, bad); bbox_acc(a.z, mn, mx, bad); bbox_acc(a.w, mn, mx, bad);

@@ -197,17 +197,30 @@ __global__ void __launch_bounds__(kScanBlock) scan_reduce_kernel(In in, const in
 }
 
 // One block: exclusive scan of the tile sums in place, total to *d_total
-// (0 instead when *d_zero_if is nonzero: a failed earlier stage).
+// (0 instead when *d_zero_if is nonzero: a failed earlier stage).  Each
+// thread takes kTilesPer consecutive sums per round (their loads issued
+// together), so 8M elements (4096 tiles) take one block scan, not sixteen.
+constexpr int kTilesPer = 16;
 template <int kBlock = kScanBlock>
 __global__ void __launch_bounds__(kBlock) scan_tiles_kernel(int* tile_sums, int n_tiles, int32_t* d_total,
                                                             const int32_t* d_zero_if) {
     int carry = 0;
-    for (int base = 0; base < n_tiles; base += kBlock) {
-        const int i = base + static_cast<int>(threadIdx.x);
-        const int v = (i < n_tiles) ? tile_sums[i] : 0;
+    for (int base = 0; base < n_tiles; base += kBlock * kTilesPer) {
+        const int first = base + static_cast<int>(threadIdx.x) * kTilesPer;
+        int v[kTilesPer];
+        int s = 0;
+#pragma unroll
+        for (int j = 0; j < kTilesPer; ++j) {
+            v[j] = (first + j < n_tiles) ? tile_sums[first + j] : 0;
+            s += v[j];
+        }
         int total;
-        const int ex = block_excl_sum(v, &total);
-        if (i < n_tiles) tile_sums[i] = carry + ex;
+        int run = carry + block_excl_sum(s, &total);
+#pragma unroll
+        for (int j = 0; j < kTilesPer; ++j) {
+            if (first + j < n_tiles) tile_sums[first + j] = run;
+            run += v[j];
+        }
         carry += total;
     }
     if (threadIdx.x == 0 && d_total) *d_total = (d_zero_if && *d_zero_if) ? 0 : carry;

@@ -173,11 +173,14 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
     if (active) {
         const uint64_t ci = code(i);
         int d;
+        int d_out = -1;  // prefix with the neighbour outside the range on i's side
         if (i == 0) {
             d = 1;
         } else {
-            const int diff = prefix_len(ci, code(i + 1), code_bits) - prefix_len(ci, code(i - 1), code_bits);
+            const int p_next = prefix_len(ci, code(i + 1), code_bits), p_prev = prefix_len(ci, code(i - 1), code_bits);
+            const int diff = p_next - p_prev;
             d = (diff > 0) - (diff < 0);
+            d_out = d > 0 ? p_prev : p_next;
         }
         int64_t l = 0;
         if (i == 0) {
@@ -198,7 +201,8 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
                 if (l + t <= cutoff && (ci ^ code(step(i, l + t, d))) < tmin) l += t;
         }
         const int64_t j = step(i, l, d);
-        const int dnode = prefix_len(ci, code(j), code_bits);
+        const uint64_t cj = code(j);
+        const int dnode = prefix_len(ci, cj, code_bits);
         int64_t s = 0;
         const int64_t scut = (d == -1) ? i - 1 : N - i - 1;
         const int lg = ceil_log2_i64(l);
@@ -231,9 +235,9 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
             // the parent adds the sibling on that side; the root counts 1
             int cnt = 1;
             if (i > 0) {
-                const int dl = a > 0 ? prefix_len(code(a - 1), code(a), code_bits) : -1;
-                const int dr = b < N ? prefix_len(code(b), code(b + 1), code_bits) : -1;
-                cnt = dnode / 3 - (dl > dr ? dl : dr) / 3;
+                const int64_t jo = j + d;  // the neighbour outside the range on j's side
+                const int d_far = (jo >= 0 && jo <= N) ? prefix_len(cj, code(jo), code_bits) : -1;
+                cnt = dnode / 3 - (d_out > d_far ? d_out : d_far) / 3;
             }
             oct_count[g] = cnt;
         }
