@@ -316,7 +316,7 @@ int knn_launch(const float4* pts, const TravNode* nodes, ParentRef par, const in
 // ---------------------------------------------------------------------------
 constexpr int kTileWaves = kKnnBlock / kWave;
 constexpr int kTileStack = 256;                          // subtrees waiting to be tested against E
-constexpr int kRangeFlush = 64;                          // stream the ranges once this many are pending
+constexpr int kRangeFlush = 192;                         // stream the ranges once this many are pending
 constexpr int kTileRanges = kRangeFlush - 1 + 2 * kWave;  // one expansion adds at most 2 per lane
 // Plain window: the tile's points +-kWindow<K> in Morton order.  Measured:
 // K = 16 is flat from 16 to 64 (8M scene); K = 32 gains from 96 (16M
@@ -500,6 +500,34 @@ __device__ __forceinline__ int range_point(const int2* rng, const int* pre, int 
     return rng[lo].y - (pre[lo] - 1 - f);
 }
 
+// In place: range i = (first, last) becomes (inclusive size prefix, last),
+// so the prefix needs no array of its own; returns the total (uniform).
+__device__ __forceinline__ int range_prefix_inplace(int2* rng, int nr) {
+    const int lane = static_cast<int>(lane_id());
+    int total = 0;
+    for (int base = 0; base < nr; base += kWave) {
+        const int idx = base + lane;
+        const int2 rg = idx < nr ? rng[idx] : make_int2(0, -1);
+        const int incl = total + wave_incl_sum(rg.y - rg.x + 1);
+        if (idx < nr) rng[idx].x = incl;
+        total = __builtin_amdgcn_readlane(incl, kWave - 1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    return total;
+}
+
+// Flattened position f (< total) of ranges prepared by range_prefix_inplace
+// -> point index.
+__device__ __forceinline__ int range_point_inplace(const int2* rng, int nr, int f) {
+    int lo = 0, hi = nr - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (rng[mid].x > f) hi = mid; else lo = mid + 1;
+    }
+    const int2 r = rng[lo];
+    return r.y - (r.x - 1 - f);
+}
+
 // Inclusive size prefix of nr ranges into pre; returns the total (uniform).
 __device__ __forceinline__ int range_prefix(const int2* rng, int* pre, int nr) {
     const int lane = static_cast<int>(lane_id());
@@ -669,17 +697,16 @@ __device__ __forceinline__ int stage_candidates(float4* cand, float4 p, int j, b
 // coalesced); those inside E are staged and broadcast.  Indices in
 // [skip_lo, skip_hi] (the window) were seen by every lane already and are
 // not staged.
+// The ranges come prepared by range_prefix_inplace (total points: total).
 template <int K, bool kTau>
-__device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, const int2* rng, int* pre, int nr,
+__device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, const int2* rng, int nr, int total,
                                               int skip_lo, int skip_hi, float ex0, float ey0, float ez0, float ex1,
                                               float ey1, float ez1, float4 q, bool on, uint32_t tau, TopK<K>& top,
-                                              int& seen, int& streamed, float4* cand) {
+                                              int& streamed, float4* cand) {
     const int lane = static_cast<int>(lane_id());
-    const int total = range_prefix(rng, pre, nr);
-    seen += total;
     for (int f0 = 0; f0 < total; f0 += kWave) {
         const int f = f0 + lane;
-        int j = f < total ? range_point(rng, pre, nr, f) : -1;
+        int j = f < total ? range_point_inplace(rng, nr, f) : -1;
         if (j >= skip_lo && j <= skip_hi) j = -1;
         float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
         if (j >= 0) p = pts[j];
@@ -711,8 +738,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
     union TileScratch {
         struct {
             int stack[kTileStack];
-            int2 range[kTileRanges];
-            int pre[kTileRanges];
+            int2 range[kTileRanges];  // (first, last), then (size prefix, last)
         } q;
         float4 win[kWinSlots<K>];
     };
@@ -737,7 +763,6 @@ knn_tile_kernel(const float4* __restrict__ pts,
     const float4 q = valid ? pts[qi] : make_float4(0.f, 0.f, 0.f, 0.f);
     int* stk = s_scr[w].q.stack;
     int2* rng = s_scr[w].q.range;
-    int* pre = s_scr[w].q.pre;
     TopK<K> top;
     top.init();
 
@@ -861,7 +886,8 @@ knn_tile_kernel(const float4* __restrict__ pts,
             lim = tau;
         }
         // ---- depth-first range query for E, 64 subtrees per step; the
-        // ranges are streamed whenever kRangeFlush of them are pending
+        // ranges are streamed whenever kRangeFlush of them are pending, each
+        // batch only after its size is checked against the segment's budget
         int sp = 0, nr = 0, seen = 0;
         bool overflow = false;
         if (U <= bucket) {
@@ -913,12 +939,17 @@ knn_tile_kernel(const float4* __restrict__ pts,
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
             if (overflow || nr == 0) break;
-            stream_ranges<K, kLearned>(pts, rng, pre, nr, wlo, whi, ex0, ey0, ez0, ex1, ey1, ez1,
-                                       q, in_seg, tau, top, seen, streamed, s_cand[w]);
+            const int total = range_prefix_inplace(rng, nr);
+            if (seen + total > kSegBudget<K>) {
+                overflow = true;
+                break;
+            }
+            seen += total;
+            stream_ranges<K, kLearned>(pts, rng, nr, total, wlo, whi, ex0, ey0, ez0, ex1, ey1, ez1, q, in_seg, tau,
+                                       top, streamed, s_cand[w]);
             nr = 0;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            if (seen > kSegBudget<K>) overflow = true;
-            if (overflow || sp == 0) break;
+            if (sp == 0) break;
         }
         if (overflow) lost = lost || in_seg;  // the segment's lanes are deferred
     }
