@@ -41,7 +41,8 @@ int sort_pairs_launch(const uint64_t* keys_in, const uint32_t* vals_in, int64_t 
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
                   int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav, const float4* pts,
-                  const float* bounds, hipStream_t st, CloudMap cm, int32_t* oct_count = nullptr);
+                  const float* bounds, hipStream_t st, CloudMap cm, int32_t* oct_count = nullptr,
+                  bool claimed_parents = false);
 int knn_tiles_launch(const float4* pts, const TravNode* nodes, ParentRef par, const int32_t* d_counts, int64_t q_start,
                      int64_t n_q, int k, const float* radius2, int bucket, int32_t* out, int2* queue, int32_t* queue_count,
                      int32_t* deferred_total, hipStream_t st, CloudMap cm, int queue_blocks, int32_t* cand_total);

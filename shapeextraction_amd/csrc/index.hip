@@ -24,7 +24,8 @@ namespace sx {
 // launchers from the other translation units
 size_t bbox_ws_bytes();
 int bbox_launch(const float* x, const float* y, const float* z, int64_t n, float* bounds, int32_t* d_bad, void* ws,
-                hipStream_t st);
+                hipStream_t st, int32_t* clear = nullptr, int n_clear = 0);
+int pack_points12_launch(const float4* pts16, const int32_t* d_count, int64_t n_max, float* pts12, hipStream_t st);
 int encode_launch(const float* x, const float* y, const float* z, int64_t n, const float* bounds, int code_bits,
                   uint64_t* codes, hipStream_t st);
 int decode_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, const float* bounds, int code_bits,
@@ -37,7 +38,8 @@ int unique_launch(const uint64_t* sorted, const uint32_t* vals, int64_t n, uint6
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
                   int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav, const float4* pts,
-                  const float* bounds, hipStream_t st, CloudMap cm = CloudMap(), int32_t* oct_count = nullptr);
+                  const float* bounds, hipStream_t st, CloudMap cm = CloudMap(), int32_t* oct_count = nullptr,
+                  bool claimed_parents = false);
 size_t octree_ws_bytes(int64_t n_max);
 int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, const float* bounds,
                   const uint8_t* pn, const int32_t* left_child, const uint8_t* hll, const uint8_t* hlr,
@@ -114,6 +116,7 @@ struct sx_index {
     bool built = false;
     bool from_codes = false;  // the last build took sorted codes (sx_index_build_codes)
     bool uoi_valid = false;   // unique_of_input holds the last build's map
+    bool pts12_valid = false; // the 12-byte point copy holds the last build's points
     bool counts_valid = false;
     hipEvent_t built_event = nullptr;  // recorded after every build
     int64_t n_unique = 0;
@@ -198,6 +201,15 @@ int input_order_map(sx_index* ix, hipStream_t st) {
     if (ix->uoi_valid) return SX_OK;
     SX_TRY(sx::scatter_i32_launch(ix->uos, ix->perm, ix->n_input, ix->uoi, st));
     ix->uoi_valid = true;
+    return SX_OK;
+}
+
+// The 12-byte point copy of the last build (written the first time a view
+// needs it; the searches read the float4 copy).
+int points12(sx_index* ix, hipStream_t st) {
+    if (ix->pts12_valid) return SX_OK;
+    SX_TRY(sx::pack_points12_launch(ix->pts16, ix->counts + kCntUnique, ix->n_input, ix->pts12, st));
+    ix->pts12_valid = true;
     return SX_OK;
 }
 
@@ -324,11 +336,13 @@ namespace {
 // sized by counts[kCntUnique] (capacity n).
 int build_trees(sx_index* ix, int64_t n, hipStream_t st) {
     const int cb = ix->code_bits;
-    SX_TRY(sx::decode_launch(ix->ucodes, ix->counts + kCntUnique, n, ix->bounds, cb, ix->pts12, ix->pts16, st));
+    // the 12-byte point copy (API layout) is written when a view asks for it
+    SX_TRY(sx::decode_launch(ix->ucodes, ix->counts + kCntUnique, n, ix->bounds, cb, nullptr, ix->pts16, st));
+    ix->pts12_valid = false;
     if (ix->root_mode == SX_ROOT_TEXTBOOK) {
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_TEXTBOOK, ix->pn, ix->lc, ix->hll,
                                  ix->hlr, ix->parent, ix->node_end, ix->leaf_parent, nullptr, ix->trav, ix->pts16,
-                                 ix->bounds, st, sx::CloudMap(), ix->oct_cnt));
+                                 ix->bounds, st, sx::CloudMap(), ix->oct_cnt, true));
     } else {
         SX_TRY(sx::karras_launch(ix->ucodes, ix->counts + kCntUnique, n, cb, SX_ROOT_REFERENCE, ix->pn, ix->lc,
                                  ix->hll, ix->hlr, ix->parent, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
@@ -430,10 +444,11 @@ int sx_index_build(sx_index* ix, const float* d_x, const float* d_y, const float
     ix->from_codes = false;
     ix->uoi_valid = false;
     const int cb = ix->code_bits;
-    // unique / octree / non-finite counts; the deferred-query total accumulates
-    // across builds until sx_index_knn_reruns resets it
-    SX_TRY(sx::fill_i32(ix->counts, 0, kCntReruns, st));
-    SX_TRY(sx::bbox_launch(d_x, d_y, d_z, n, ix->bounds, ix->counts + kCntBad, ix->ws_bbox, st));
+    // unique / octree / non-finite counts and the sort flag are cleared by the
+    // bounds kernel; the deferred-query total accumulates across builds until
+    // sx_index_knn_reruns resets it
+    SX_TRY(sx::bbox_launch(d_x, d_y, d_z, n, ix->bounds, ix->counts + kCntBad, ix->ws_bbox, st, ix->counts,
+                           kCntReruns));
     SX_TRY(sx::encode_launch(d_x, d_y, d_z, n, ix->bounds, cb, ix->codes_in, st));
     SX_TRY(sx::sort_pairs_launch(ix->codes_in, nullptr, n, cb, ix->codes_sorted, ix->perm, ix->ws, ix->sort_algo, st,
                                  ix->counts + kCntSortErr));
@@ -477,6 +492,7 @@ int sx_index_get_view(sx_index* ix, sx_index_view* view, void* stream) {
     if (!view) return fail(SX_EINVAL, "sx_index_get_view: null view");
     SX_TRY(after_build(ix, stream));
     SX_TRY(input_order_map(ix, sx::as_stream(stream)));
+    SX_TRY(points12(ix, sx::as_stream(stream)));
     int32_t flags[2] = {0, 0};  // non-finite inputs (rejected codes), sort error
     SX_HIP(hipMemcpyAsync(flags, ix->counts + kCntBad, sizeof(flags), hipMemcpyDeviceToHost, sx::as_stream(stream)));
     SX_HIP(hipStreamSynchronize(sx::as_stream(stream)));

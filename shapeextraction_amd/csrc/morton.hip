@@ -54,10 +54,14 @@ __device__ __forceinline__ void block_minmax(float& mn, float& mx, int& bad) {
 // need an agent-scope release per block, i.e. an L2 write-back on a GPU whose
 // XCDs have separate L2s; the kernel boundary orders the partials instead.
 // kVec: x, y, z are 16-byte aligned and read as float4.
+// clear (nullable): n_clear int32 zeroed by block 0 for the caller, so that
+// its counts need no launch of their own (stream order makes the zeros
+// visible to every later kernel).
 template <bool kVec>
 __global__ void __launch_bounds__(kBBoxBlock) bbox_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                                           const float* __restrict__ z, int64_t n, float2* partial,
-                                                          int* bad_partial) {
+                                                          int* bad_partial, int32_t* clear, int n_clear) {
+    if (clear && blockIdx.x == 0 && static_cast<int>(threadIdx.x) < n_clear) clear[threadIdx.x] = 0;
     float mn = INFINITY, mx = -INFINITY;
     int bad = 0;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
@@ -124,7 +128,8 @@ size_t bbox_ws_bytes() {
 }
 
 int bbox_launch(const float* x, const float* y, const float* z, int64_t n, float* bounds, int32_t* d_bad, void* ws,
-                hipStream_t st) {
+                hipStream_t st, int32_t* clear = nullptr, int n_clear = 0) {
+    if (n_clear > kBBoxBlock) return fail(SX_EINVAL, "bbox_launch: too many words to clear");
     Carver c(ws);
     float2* partial = c.take<float2>(kBBoxMaxBlocks);
     int* badp = c.take<int>(kBBoxMaxBlocks);
@@ -132,9 +137,9 @@ int bbox_launch(const float* x, const float* y, const float* z, int64_t n, float
     const bool vec = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y) |
                        reinterpret_cast<uintptr_t>(z)) & 15u) == 0;
     if (vec)
-        bbox_kernel<true><<<nb, kBBoxBlock, 0, st>>>(x, y, z, n, partial, badp);
+        bbox_kernel<true><<<nb, kBBoxBlock, 0, st>>>(x, y, z, n, partial, badp, clear, n_clear);
     else
-        bbox_kernel<false><<<nb, kBBoxBlock, 0, st>>>(x, y, z, n, partial, badp);
+        bbox_kernel<false><<<nb, kBBoxBlock, 0, st>>>(x, y, z, n, partial, badp, clear, n_clear);
     SX_LAUNCH_CHECK("bbox_kernel");
     bbox_final_kernel<<<1, kBBoxBlock, 0, st>>>(partial, badp, nb, bounds, d_bad);
     SX_LAUNCH_CHECK("bbox_final_kernel");
@@ -217,6 +222,25 @@ __global__ void __launch_bounds__(256) decode_kernel(const uint64_t* __restrict_
         pts12[3 * i + 2] = pz;
     }
     if (pts16) pts16[i] = make_float4(px, py, pz, 0.0f);
+}
+
+// The 12-byte (API layout) copy of the float4 points, [0, *d_count).
+__global__ void __launch_bounds__(256) pack_points12_kernel(const float4* __restrict__ pts16, const int32_t* d_count,
+                                                            int64_t n_max, float* __restrict__ pts12) {
+    const int64_t n = scan_count(d_count, n_max);
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pts16[i];
+    pts12[3 * i] = p.x;
+    pts12[3 * i + 1] = p.y;
+    pts12[3 * i + 2] = p.z;
+}
+
+int pack_points12_launch(const float4* pts16, const int32_t* d_count, int64_t n_max, float* pts12, hipStream_t st) {
+    if (n_max <= 0) return SX_OK;
+    pack_points12_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(pts16, d_count, n_max, pts12);
+    SX_LAUNCH_CHECK("pack_points12_kernel");
+    return SX_OK;
 }
 
 int decode_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, const float* bounds, int code_bits,

@@ -149,7 +149,7 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
     const int64_t base = cloud.x, U = cloud.y;
     const int64_t N = U - 1;  // internal nodes; leaves are 0..N
     const int64_t i = g - base;
-    if (in_range && i == 0 && parent) parent[g] = -1;
+    if (in_range && (i == 0 || i == N) && parent) parent[g] = -1;
     if (in_range && i == 0 && up) up[g].x = -1;
     const bool active = in_range && i < N;
     // code of the cloud's leaf x (local index): from LDS inside [w0, w1), else
@@ -338,12 +338,17 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
 int karras_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, int code_bits, int root_mode,
                   uint8_t* prefix_n, int32_t* left_child, uint8_t* hll, uint8_t* hlr, int32_t* parent,
                   int32_t* node_end, int32_t* leaf_parent, int2* up, TravNode* trav, const float4* pts,
-                  const float* bounds, hipStream_t st, CloudMap cm = CloudMap(), int32_t* oct_count = nullptr) {
+                  const float* bounds, hipStream_t st, CloudMap cm = CloudMap(), int32_t* oct_count = nullptr,
+                  bool claimed_parents = false) {
     if (n_max <= 0) return SX_OK;
     if (trav && (!pts || !bounds)) return fail(SX_EINVAL, "karras_launch: traversal records need points and bounds");
     // Nodes that no other node claims (only possible under the reference root
-    // rule) keep parent -1.
-    if (parent) SX_TRY(fill_i32(parent, -1, n_max, st));
+    // rule) keep parent -1, as do a batch's padding slots and the slots past
+    // the unique count.  claimed_parents: the caller reads only the slots below
+    // the unique count of a textbook tree, where every internal node is
+    // claimed and the kernel writes -1 at the root and past the last node.
+    const bool need_fill = !(claimed_parents && root_mode == SX_ROOT_TEXTBOOK && !cm.tile_cloud);
+    if (parent && need_fill) SX_TRY(fill_i32(parent, -1, n_max, st));
     karras_kernel<<<grid_for(n_max, kKarrasBlock), kKarrasBlock, 0, st>>>(
         codes, d_count, n_max, code_bits, root_mode, prefix_n, left_child, hll, hlr, parent, node_end, leaf_parent, up,
         trav, pts, bounds, cm, oct_count);
