@@ -684,11 +684,17 @@ __device__ __forceinline__ bool exact_window(const float4* win, int wlo, float4 
 
 // Stage the lanes' candidates (p, j) with ok set into cand[0, n), in lane
 // order, followed by one point at +inf; returns n (uniform).
+template <bool kFresh = false>
 __device__ __forceinline__ int stage_candidates(float4* cand, float4 p, int j, bool ok) {
     const uint64_t m = __ballot(ok);
     const int n = __popcll(m);
     if (ok) cand[rank_below(m)] = make_float4(p.x, p.y, p.z, __int_as_float(j));
-    if (static_cast<int>(lane_id()) == 0) cand[n] = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+    if (static_cast<int>(lane_id()) == 0) {
+        // kFresh: made here (not a loop-invariant vector the compiler would
+        // keep live, and spill, across the caller's loops)
+        const float inf = kFresh ? opaque(INFINITY) : INFINITY;
+        cand[n] = make_float4(inf, inf, inf, 0.0f);
+    }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     return n;
 }
@@ -711,7 +717,7 @@ __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, co
         float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
         if (j >= 0) p = pts[j];
         const bool in_e = j >= 0 && p.x >= ex0 && p.x <= ex1 && p.y >= ey0 && p.y <= ey1 && p.z >= ez0 && p.z <= ez1;
-        const int n = stage_candidates(cand, p, j, in_e);
+        const int n = stage_candidates<K <= 16>(cand, p, j, in_e);
         streamed += n;
         broadcast_insert<K, kTau>(cand, n, q, on, tau, top);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -744,7 +750,14 @@ knn_tile_kernel(const float4* __restrict__ pts,
     };
     __shared__ TileScratch s_scr[kTileWaves];
     __shared__ float4 s_cand[kTileWaves][kWave + 1];
-    const int w = static_cast<int>(threadIdx.x) / kWave;
+    // K <= 16: wave-uniform values, the query index and the query point are
+    // kept out of the long-lived vector registers (scalar registers, or
+    // recomputed / reloaded where used): those kernels run at 72 VGPRs and
+    // would spill them; the larger lists have the registers and measured
+    // slower with these changes
+    constexpr bool kLean = K <= 16;
+    const int w = kLean ? __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave)
+                        : static_cast<int>(threadIdx.x) / kWave;
     const int lane = static_cast<int>(lane_id());
     const int total = d_counts[0];
     const int64_t q_req = q_start + n_q;
@@ -757,7 +770,9 @@ knn_tile_kernel(const float4* __restrict__ pts,
     if (tile0 >= q_end) return;  // whole wave idle (no block barriers below)
     // the tile's cloud: points and radix nodes [cbase, cbase + U)
     const int2 cloud = cm.at(tile0, total);
-    const int cbase = cloud.x, U = cloud.y, cend = cloud.x + cloud.y;
+    const int cbase = kLean ? __builtin_amdgcn_readfirstlane(cloud.x) : cloud.x;
+    const int U = kLean ? __builtin_amdgcn_readfirstlane(cloud.y) : cloud.y;
+    const int cend = cbase + U;
     const int64_t qi = tile0 + lane;
     const bool valid = qi < q_end && qi < cend;
     const float4 q = valid ? pts[qi] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -772,6 +787,10 @@ knn_tile_kernel(const float4* __restrict__ pts,
         constexpr int hw = kWindow<K>;
         wlo = static_cast<int>(tile0 - hw > cbase ? tile0 - hw : cbase);
         whi = static_cast<int>(tile0 + kWave - 1 + hw < cend - 1 ? tile0 + kWave - 1 + hw : cend - 1);
+        if (kLean) {  // wave-uniform: kept in scalar registers
+            wlo = __builtin_amdgcn_readfirstlane(wlo);
+            whi = __builtin_amdgcn_readfirstlane(whi);
+        }
         if constexpr (kNetWindow<K>) {
             // the whole window in LDS; slots past it hold points at +inf,
             // whose keys follow every finite one, so they reach the list only
@@ -862,12 +881,14 @@ knn_tile_kernel(const float4* __restrict__ pts,
         const bool in_seg = (seg >> lane) & 1ull;
         const float rps = opaque(rp);  // recomputed per segment rather than held live
         const int ls = opaque(lane);
-        const float ex0 = uniform_f(wave_min_at(in_seg ? nextafterf(q.x - rps, -INFINITY) : INFINITY, ls));
-        const float ey0 = uniform_f(wave_min_at(in_seg ? nextafterf(q.y - rps, -INFINITY) : INFINITY, ls));
-        const float ez0 = uniform_f(wave_min_at(in_seg ? nextafterf(q.z - rps, -INFINITY) : INFINITY, ls));
-        const float ex1 = uniform_f(wave_max_at(in_seg ? nextafterf(q.x + rps, INFINITY) : -INFINITY, ls));
-        const float ey1 = uniform_f(wave_max_at(in_seg ? nextafterf(q.y + rps, INFINITY) : -INFINITY, ls));
-        const float ez1 = uniform_f(wave_max_at(in_seg ? nextafterf(q.z + rps, INFINITY) : -INFINITY, ls));
+        // K <= 16: the query point loaded again (a cached read)
+        const float4 qs = kLean ? (valid ? pts[tile0 + ls] : make_float4(0.f, 0.f, 0.f, 0.f)) : q;
+        const float ex0 = uniform_f(wave_min_at(in_seg ? nextafterf(qs.x - rps, -INFINITY) : INFINITY, ls));
+        const float ey0 = uniform_f(wave_min_at(in_seg ? nextafterf(qs.y - rps, -INFINITY) : INFINITY, ls));
+        const float ez0 = uniform_f(wave_min_at(in_seg ? nextafterf(qs.z - rps, -INFINITY) : INFINITY, ls));
+        const float ex1 = uniform_f(wave_max_at(in_seg ? nextafterf(qs.x + rps, INFINITY) : -INFINITY, ls));
+        const float ey1 = uniform_f(wave_max_at(in_seg ? nextafterf(qs.y + rps, INFINITY) : -INFINITY, ls));
+        const float ez1 = uniform_f(wave_max_at(in_seg ? nextafterf(qs.z + rps, INFINITY) : -INFINITY, ls));
         uint32_t tau = 0u;
         if (in_seg) {
             tau = r2bits;
@@ -876,8 +897,8 @@ knn_tile_kernel(const float4* __restrict__ pts,
                 // nearest face is shrunk by 2^-20 twice, far more than the
                 // rounding of the gap, the square and dist2 itself.  A lane may
                 // so accept beyond its predicted radius (up to kRadiusStretch).
-                float g = fminf(fminf(q.x - ex0, ex1 - q.x),
-                                fminf(fminf(q.y - ey0, ey1 - q.y), fminf(q.z - ez0, ez1 - q.z)));
+                float g = fminf(fminf(qs.x - ex0, ex1 - qs.x),
+                                fminf(fminf(qs.y - ey0, ey1 - qs.y), fminf(qs.z - ez0, ez1 - qs.z)));
                 g = fmaxf(g, 0.0f) * (1.0f - 0x1p-20f);
                 const float rho2 = (g * g) * (1.0f - 0x1p-20f);
                 const uint32_t t = __float_as_uint(fminf(rho2, __uint_as_float(r2bits) * kRadiusStretch));
@@ -945,7 +966,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
                 break;
             }
             seen += total;
-            stream_ranges<K, kLearned>(pts, rng, nr, total, wlo, whi, ex0, ey0, ez0, ex1, ey1, ez1, q, in_seg, tau,
+            stream_ranges<K, kLearned>(pts, rng, nr, total, wlo, whi, ex0, ey0, ez0, ex1, ey1, ez1, qs, in_seg, tau,
                                        top, streamed, s_cand[w]);
             nr = 0;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -956,7 +977,8 @@ knn_tile_kernel(const float4* __restrict__ pts,
     // uncapped: every point within the window radius was a candidate;
     // capped: the k-th distance must lie within the radius the box proves
     const bool exact = join && !lost && (capped ? top.kth_bits() <= lim : true);
-    if (valid && exact) write_row<K>(out + (qi - q_start) * K, top, cbase);
+    const int64_t qo = kLean ? tile0 + opaque(lane) : qi;  // the query index (K <= 16: again)
+    if (valid && exact) write_row<K>(out + (qo - q_start) * K, top, cbase);
     const bool defer = valid && !exact;
     const uint64_t md = __ballot(defer);
     if (md) {
@@ -965,7 +987,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
         slot0 = __shfl(slot0, __ffsll(static_cast<unsigned long long>(md)) - 1, kWave);
         // with the k-th key of its list, an upper bound on the k-th distance
         // when the list holds K real points (+inf bits otherwise)
-        if (defer) queue[slot0 + rank_below(md)] = make_int2(static_cast<int32_t>(qi), static_cast<int>(top.kth_bits()));
+        if (defer) queue[slot0 + rank_below(md)] = make_int2(static_cast<int32_t>(qo), static_cast<int>(top.kth_bits()));
     }
     if (cand_total && lane == 0 && streamed) atomicAdd(cand_total, streamed);
 }
