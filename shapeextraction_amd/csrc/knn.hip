@@ -20,8 +20,8 @@
 // Traversal: nearest child first, depth first, stackless.  Returning from a
 // child to its parent re-derives which child was nearer (the same
 // comparison as on the way down), so the far child is visited after the near
-// one without any stack in scratch or LDS; the parent index sits in the
-// record.  A subtree is skipped when its box distance exceeds the current
+// one without any stack in scratch or LDS; parents come from the radix
+// tree's parent array.  A subtree is skipped when its box distance exceeds the current
 // bound, min(R^2, k-th best key); ties on distance survive because pruning is
 // strict and box distances are exact lower bounds (see box_dist2).
 //
@@ -29,6 +29,14 @@
 // (float bits << 32 | index; squared distances are non-negative so the bit
 // order is the numeric order), kept sorted in registers with a branch-free
 // shift insert.  Ties are therefore broken by point index.
+//
+// Read as an IEEE double, such a key is a non-negative number and never a
+// NaN: its high word is the bits of a non-negative float distance, below
+// 0x7FF00000 even for +inf.  The numeric order of those doubles is the
+// unsigned order of the keys (denormal doubles included: FP64 denormals are
+// not flushed), so a 64-bit min or max is one v_min_f64 / v_max_f64 instead
+// of a 64-bit compare and four selects.  The empty key is kNoKey (+inf as a
+// double), above every key of a real point.
 //
 // Radius seeding (SX_KNN_RADIUS): the bound starts at R^2.  If the final
 // k-th distance is <= R^2, every pruned point was strictly farther than it,
@@ -39,22 +47,37 @@
 
 namespace sx {
 
+constexpr uint64_t kNoKey = 0x7FF0000000000000ull;
+
+// Written as asm: the compiler's double min/max first quiets both operands
+// (one more v_max_f64 each) unless it can prove they are not signalling NaNs,
+// which it cannot for list entries carried across loop iterations.
+__device__ __forceinline__ uint64_t key_min(uint64_t a, uint64_t b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(__builtin_bit_cast(double, a)), "v"(__builtin_bit_cast(double, b)));
+    return __builtin_bit_cast(uint64_t, r);
+}
+__device__ __forceinline__ uint64_t key_max(uint64_t a, uint64_t b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(__builtin_bit_cast(double, a)), "v"(__builtin_bit_cast(double, b)));
+    return __builtin_bit_cast(uint64_t, r);
+}
+
 template <int K>
 struct TopK {
     uint64_t e[K];
     __device__ __forceinline__ void init() {
 #pragma unroll
-        for (int s = 0; s < K; ++s) e[s] = ~0ull;
+        for (int s = 0; s < K; ++s) e[s] = kNoKey;
     }
     __device__ __forceinline__ uint32_t kth_bits() const { return static_cast<uint32_t>(e[K - 1] >> 32); }
+    // Slot s of the sorted list with key added is max(e[s-1], min(key, e[s]))
+    // (e[s-1] where the key ranks below s, else the smaller of key and e[s]);
+    // a key not below e[K-1] leaves the list as it is.
     __device__ __forceinline__ void insert(uint64_t key) {
 #pragma unroll
-        for (int s = K - 1; s > 0; --s) {
-            const uint64_t prev = e[s - 1];
-            const uint64_t cur = e[s];
-            e[s] = key < prev ? prev : (key < cur ? key : cur);
-        }
-        e[0] = key < e[0] ? key : e[0];
+        for (int s = K - 1; s > 0; --s) e[s] = key_max(e[s - 1], key_min(key, e[s]));
+        e[0] = key_min(key, e[0]);
     }
     // insert() for a wave whose inserting lanes (the active ones) mostly add
     // keys that rank in the upper half of their lists: the lower half is
@@ -65,10 +88,7 @@ struct TopK {
     template <int lo, int hi>
     __device__ __forceinline__ void shift_slots(uint64_t key) {
 #pragma unroll
-        for (int s = hi - 1; s >= lo; --s) {
-            const uint64_t cur = e[s];
-            e[s] = s > 0 && key < e[s > 0 ? s - 1 : 0] ? e[s > 0 ? s - 1 : 0] : (key < cur ? key : cur);
-        }
+        for (int s = hi - 1; s >= lo; --s) e[s] = s > 0 ? key_max(e[s > 0 ? s - 1 : 0], key_min(key, e[s])) : key_min(key, e[s]);
     }
     // insert() for the active lanes of a wave, half the list at a time: the
     // lower half is shifted only when some active lane's key ranks there.
@@ -112,14 +132,14 @@ __device__ __forceinline__ void scan_range(const float4* __restrict__ pts, int l
 // missing entries (fewer than K points) repeat the nearest.
 template <int K>
 __device__ __forceinline__ void write_row(int32_t* row, const TopK<K>& top, int base) {
-    const int32_t first = static_cast<int32_t>(top.e[0] & 0xFFFFFFFFull) - base;
+    const int32_t first = top.e[0] == kNoKey ? -1 : static_cast<int32_t>(top.e[0] & 0xFFFFFFFFull) - base;
 #pragma unroll
     for (int s = 0; s < K; s += 4) {
         int4 v;
-        v.x = top.e[s] == ~0ull ? first : static_cast<int32_t>(top.e[s] & 0xFFFFFFFFull) - base;
-        v.y = top.e[s + 1] == ~0ull ? first : static_cast<int32_t>(top.e[s + 1] & 0xFFFFFFFFull) - base;
-        v.z = top.e[s + 2] == ~0ull ? first : static_cast<int32_t>(top.e[s + 2] & 0xFFFFFFFFull) - base;
-        v.w = top.e[s + 3] == ~0ull ? first : static_cast<int32_t>(top.e[s + 3] & 0xFFFFFFFFull) - base;
+        v.x = top.e[s] == kNoKey ? first : static_cast<int32_t>(top.e[s] & 0xFFFFFFFFull) - base;
+        v.y = top.e[s + 1] == kNoKey ? first : static_cast<int32_t>(top.e[s + 1] & 0xFFFFFFFFull) - base;
+        v.z = top.e[s + 2] == kNoKey ? first : static_cast<int32_t>(top.e[s + 2] & 0xFFFFFFFFull) - base;
+        v.w = top.e[s + 3] == kNoKey ? first : static_cast<int32_t>(top.e[s + 3] & 0xFFFFFFFFull) - base;
         *reinterpret_cast<int4*>(row + s) = v;
     }
 }
@@ -456,15 +476,10 @@ struct OddEvenNet {
     }
 };
 
-// One compare and four selects.  The empty asm hides that the selects pick
-// the compared values; otherwise they become a 64-bit min and max, each with
-// a compare of its own.
 __device__ __forceinline__ void cswap(uint64_t& x, uint64_t& y) {
-    const bool s = y < x;
-    uint64_t xo = x, yo = y;
-    asm("" : "+v"(xo), "+v"(yo));
-    x = s ? yo : xo;
-    y = s ? xo : yo;
+    const uint64_t lo = key_min(x, y);
+    y = key_max(x, y);
+    x = lo;
 }
 
 template <int N>
@@ -528,21 +543,6 @@ __device__ __forceinline__ int range_point_inplace(const int2* rng, int nr, int 
     return r.y - (r.x - 1 - f);
 }
 
-// Inclusive size prefix of nr ranges into pre; returns the total (uniform).
-__device__ __forceinline__ int range_prefix(const int2* rng, int* pre, int nr) {
-    const int lane = static_cast<int>(lane_id());
-    int total = 0;
-    for (int base = 0; base < nr; base += kWave) {
-        const int idx = base + lane;
-        const int2 rg = idx < nr ? rng[idx] : make_int2(0, -1);
-        const int incl = total + wave_incl_sum(rg.y - rg.x + 1);
-        if (idx < nr) pre[idx] = incl;
-        total = __builtin_amdgcn_readlane(incl, kWave - 1);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    return total;
-}
-
 // Broadcast the n candidates staged in cand[0, n) (x, y, z, index bits; two
 // per step, uniform LDS reads) to every lane; a lane inserts a candidate when
 // on, it beats the lane's k-th key and (kTau) its squared distance is <= tau.
@@ -563,10 +563,12 @@ __device__ __forceinline__ void broadcast_insert(const float4* cand, int n, floa
         // (the seeded search with K <= 16 at seven waves keeps more values
         // live: there the half-wise insert spills less than the quarter-wise)
         constexpr bool kHalf = kTau && K <= 16;
-        if (on && (!kTau || d0 <= tau) && k0 < top.e[K - 1]) {
+        // the distance alone decides whether to insert: a key that ties the
+        // k-th distance with a larger index leaves the list as it is
+        if (on && (!kTau || d0 <= tau) && d0 <= top.kth_bits()) {
             if constexpr (kHalf) top.insert_wave_half(k0); else top.insert_wave(k0);
         }
-        if (on && (!kTau || d1 <= tau) && k1 < top.e[K - 1]) {
+        if (on && (!kTau || d1 <= tau) && d1 <= top.kth_bits()) {
             if constexpr (kHalf) top.insert_wave_half(k1); else top.insert_wave(k1);
         }
     }
