@@ -79,23 +79,12 @@ struct TopK {
         for (int s = K - 1; s > 0; --s) e[s] = key_max(e[s - 1], key_min(key, e[s]));
         e[0] = key_min(key, e[0]);
     }
-    // insert() for a wave whose inserting lanes (the active ones) mostly add
-    // keys that rank in the upper half of their lists: the lower half is
-    // shifted only when some active lane's key belongs there (a uniform
-    // branch), which halves the typical cost of a wave-wide insert.
     // Shift slots [lo, hi): the insert's formula per slot, highest first (a
     // slot reads its lower neighbour before that neighbour is shifted).
     template <int lo, int hi>
     __device__ __forceinline__ void shift_slots(uint64_t key) {
 #pragma unroll
         for (int s = hi - 1; s >= lo; --s) e[s] = s > 0 ? key_max(e[s > 0 ? s - 1 : 0], key_min(key, e[s])) : key_min(key, e[s]);
-    }
-    // insert() for the active lanes of a wave, half the list at a time: the
-    // lower half is shifted only when some active lane's key ranks there.
-    __device__ __forceinline__ void insert_wave_half(uint64_t key) {
-        const bool lower = __ballot(key < e[K / 2 - 1]) != 0;
-        shift_slots<K / 2, K>(key);
-        if (lower) shift_slots<0, K / 2>(key);
     }
     // insert() for the active lanes of a wave, one quarter of the list at a
     // time from the top: a lower quarter is shifted only when some active
@@ -560,17 +549,10 @@ __device__ __forceinline__ void broadcast_insert(const float4* cand, int n, floa
         const int j0 = __float_as_int(c0.w), j1 = __float_as_int(c1.w);
         const uint64_t k0 = (static_cast<uint64_t>(d0) << 32) | static_cast<uint32_t>(j0);
         const uint64_t k1 = (static_cast<uint64_t>(d1) << 32) | static_cast<uint32_t>(j1);
-        // (the seeded search with K <= 16 at seven waves keeps more values
-        // live: there the half-wise insert spills less than the quarter-wise)
-        constexpr bool kHalf = kTau && K <= 16;
         // the distance alone decides whether to insert: a key that ties the
         // k-th distance with a larger index leaves the list as it is
-        if (on && (!kTau || d0 <= tau) && d0 <= top.kth_bits()) {
-            if constexpr (kHalf) top.insert_wave_half(k0); else top.insert_wave(k0);
-        }
-        if (on && (!kTau || d1 <= tau) && d1 <= top.kth_bits()) {
-            if constexpr (kHalf) top.insert_wave_half(k1); else top.insert_wave(k1);
-        }
+        if (on && (!kTau || d0 <= tau) && d0 <= top.kth_bits()) top.insert_wave(k0);
+        if (on && (!kTau || d1 <= tau) && d1 <= top.kth_bits()) top.insert_wave(k1);
     }
 }
 
@@ -726,14 +708,16 @@ __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, co
     }
 }
 
-// Seven waves per SIMD for K <= 16: 72 VGPRs, a few values spilled to scratch
-// (measured on the 8M scene, plain kNN: 4.11-4.12 ms at seven waves, 4.15-4.19
-// at six with 80 VGPRs, 4.17 at eight with 64).  Five for K = 24 (4M
-// degenerate scene: 4.44 ms against 4.81 at the natural 107 VGPRs, four
-// waves).  K = 32 keeps its natural allocation (123 VGPRs; forced to five
-// waves it spills heavily: 35.4 ms against 22.1 on the 16M mixture).
+// Waves per SIMD, measured on the 8M scene (K = 16): plain eight (64 VGPRs,
+// 16 bytes per lane spilled; kNN 2.72 ms against 2.76 at seven with 72 VGPRs
+// and 2.89 at six with 77), seeded seven (eight spill 44 bytes per lane).
+// Six for K = 24 (80 VGPRs, 64 bytes per lane spilled; 4M degenerate scene:
+// kNN 2.61 ms against 2.65 at five waves and 2.97 at the natural 102 VGPRs,
+// four waves).
+// K = 32 keeps its natural allocation (about 120 VGPRs; forced to five waves
+// it spilled heavily: 35.4 ms against 22.1 on the 16M mixture).
 template <int K, bool kLearned>
-__global__ void __launch_bounds__(kKnnBlock) __attribute__((amdgpu_waves_per_eu(K <= 16 ? 7 : (K <= 24 ? 5 : 1), 8)))
+__global__ void __launch_bounds__(kKnnBlock) __attribute__((amdgpu_waves_per_eu(K <= 16 ? (kLearned ? 7 : 8) : (K <= 24 ? 6 : 1), 8)))
 knn_tile_kernel(const float4* __restrict__ pts,
                                                              const TravNode* __restrict__ nodes,
                                                              const int32_t* __restrict__ d_counts, int64_t q_start,
@@ -754,8 +738,8 @@ knn_tile_kernel(const float4* __restrict__ pts,
     __shared__ float4 s_cand[kTileWaves][kWave + 1];
     // K <= 16: wave-uniform values, the query index and the query point are
     // kept out of the long-lived vector registers (scalar registers, or
-    // recomputed / reloaded where used): those kernels run at 72 VGPRs and
-    // would spill them; the larger lists have the registers and measured
+    // recomputed / reloaded where used): those kernels run at 64-72 VGPRs
+    // and would spill them; the larger lists have the registers and measured
     // slower with these changes
     constexpr bool kLean = K <= 16;
     const int w = kLean ? __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave)
