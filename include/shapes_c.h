@@ -59,8 +59,10 @@ extern "C" {
 /* Index algorithm choices (results are identical; speed differs). */
 #define SX_KNN_ALGO_TILES 0     /* wave-cooperative tiles + per-lane queue (default) */
 #define SX_KNN_ALGO_LANE 1      /* one lane per query, stackless descent            */
-#define SX_SORT_ONESWEEP 0      /* single-pass-per-digit radix sort (default)       */
+#define SX_SORT_ONESWEEP 0      /* single-pass-per-digit radix sort                 */
 #define SX_SORT_THREE_KERNEL 1  /* histogram, scan, scatter per digit               */
+#define SX_SORT_HYBRID 2        /* top digits by onesweep, then small buckets sorted
+                                   in LDS, large ones by onesweep (default)         */
 
 /* Normal precision modes. */
 #define SX_PREC_FP32 0    /* compensated FP32 covariance + closed-form FP32 eigen,

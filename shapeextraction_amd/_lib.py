@@ -17,7 +17,7 @@ SX_ROOT_TEXTBOOK, SX_ROOT_REFERENCE = 0, 1
 SX_KNN_PLAIN, SX_KNN_RADIUS = 0, 1
 SX_PREC_FP32, SX_PREC_FP64 = 0, 1
 SX_KNN_ALGO_TILES, SX_KNN_ALGO_LANE = 0, 1
-SX_SORT_ONESWEEP, SX_SORT_THREE_KERNEL = 0, 1
+SX_SORT_ONESWEEP, SX_SORT_THREE_KERNEL, SX_SORT_HYBRID = 0, 1, 2
 
 STATUS_NAMES = {
     SX_OK: "SX_OK",

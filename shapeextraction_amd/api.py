@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (SX_KNN_ALGO_LANE, SX_KNN_ALGO_TILES, SX_KNN_PLAIN, SX_KNN_RADIUS, SX_PREC_FP32, SX_PREC_FP64,
-                   SX_ROOT_REFERENCE, SX_ROOT_TEXTBOOK, SX_SORT_ONESWEEP, SX_SORT_THREE_KERNEL, call)
+                   SX_ROOT_REFERENCE, SX_ROOT_TEXTBOOK, SX_SORT_ONESWEEP, SX_SORT_THREE_KERNEL, SX_SORT_HYBRID, call)
 from .device import DeviceArray, fetch
 
 OTNODE_DTYPE = np.dtype(
@@ -253,7 +253,7 @@ class Index:
     def set_bucket(self, bucket: int):
         call("sx_index_set_bucket", self.handle, bucket)
 
-    def set_algorithms(self, knn: int = _lib.SX_KNN_ALGO_TILES, sort: int = _lib.SX_SORT_ONESWEEP):
+    def set_algorithms(self, knn: int = _lib.SX_KNN_ALGO_TILES, sort: int = _lib.SX_SORT_HYBRID):
         call("sx_index_set_algorithms", self.handle, knn, sort)
 
     def build(self, x, y, z, stream=None) -> "Index":
@@ -402,7 +402,7 @@ class Batch:
         self._view = None
         return self
 
-    def set_tuning(self, bucket: int = 16, sort: int = _lib.SX_SORT_ONESWEEP, queue_blocks: int = 0):
+    def set_tuning(self, bucket: int = 16, sort: int = _lib.SX_SORT_HYBRID, queue_blocks: int = 0):
         call("sx_batch_set_tuning", self.handle, bucket, sort, queue_blocks)
 
     def build(self, x, y, z, stream=None) -> "Batch":
@@ -515,6 +515,6 @@ __all__ = [
     "Index", "Batch", "BatchGraph", "Workload", "WORKLOADS", "OTNODE_DTYPE", "bbox", "morton_encode", "morton_decode", "sort_pairs",
     "unique", "radix_tree", "octree", "check_counts", "set_sort_spin_limit", "DEFAULT_SORT_SPIN_LIMIT", "gen_sphere", "gen_scene", "gen_gmm_scene",
     "gen_batch_sizes", "gen_degenerate", "fnv64", "SX_KNN_PLAIN", "SX_KNN_RADIUS", "SX_PREC_FP32", "SX_PREC_FP64",
-    "SX_ROOT_TEXTBOOK", "SX_ROOT_REFERENCE", "SX_KNN_ALGO_TILES", "SX_KNN_ALGO_LANE", "SX_SORT_ONESWEEP",
+    "SX_ROOT_TEXTBOOK", "SX_ROOT_REFERENCE", "SX_KNN_ALGO_TILES", "SX_KNN_ALGO_LANE", "SX_SORT_ONESWEEP", "SX_SORT_HYBRID",
     "SX_SORT_THREE_KERNEL",
 ]

@@ -170,7 +170,7 @@ struct sx_batch {
     int64_t slot_cap = 0;   // cap + (kCloudAlign - 1) * max_clouds
     int code_bits = 63;
     int bucket = 16;
-    int sort_algo = SX_SORT_ONESWEEP;
+    int sort_algo = SX_SORT_HYBRID;
     int queue_blocks = 0;   // deferred-query workers; 0 = default
     void* mem = nullptr;
     hipEvent_t built_event = nullptr;  // recorded after every eager build
@@ -371,7 +371,7 @@ int sx_batch_set_clouds(sx_batch* b, const int64_t* h_offsets, int n_clouds, voi
 int sx_batch_set_tuning(sx_batch* b, int bucket, int sort_algo, int queue_blocks) {
     if (!b) return fail(SX_EINVAL, "sx_batch_set_tuning: null batch");
     if (bucket < 1 || bucket > 256) return fail(SX_EINVAL, "sx_batch_set_tuning: bucket must be in [1, 256]");
-    if (sort_algo != SX_SORT_ONESWEEP && sort_algo != SX_SORT_THREE_KERNEL)
+    if (sort_algo != SX_SORT_ONESWEEP && sort_algo != SX_SORT_THREE_KERNEL && sort_algo != SX_SORT_HYBRID)
         return fail(SX_EINVAL, "sx_batch_set_tuning: bad sort algorithm");
     if (queue_blocks < 0 || queue_blocks > (1 << 16)) return fail(SX_EINVAL, "sx_batch_set_tuning: bad queue_blocks");
     b->bucket = bucket;

@@ -77,7 +77,7 @@ struct sx_index {
     int root_mode = SX_ROOT_TEXTBOOK;
     int bucket = 16;
     int knn_algo = SX_KNN_ALGO_TILES;
-    int sort_algo = SX_SORT_ONESWEEP;
+    int sort_algo = SX_SORT_HYBRID;
     bool knn_stats = false;  // count the candidates the kNN tiles broadcast
     void* mem = nullptr;
     float* bounds = nullptr;
@@ -427,7 +427,7 @@ int sx_index_set_algorithms(sx_index* ix, int knn_algo, int sort_algo) {
     if (!ix) return fail(SX_EINVAL, "sx_index_set_algorithms: null index");
     if (knn_algo != SX_KNN_ALGO_TILES && knn_algo != SX_KNN_ALGO_LANE)
         return fail(SX_EINVAL, "sx_index_set_algorithms: bad kNN algorithm");
-    if (sort_algo != SX_SORT_ONESWEEP && sort_algo != SX_SORT_THREE_KERNEL)
+    if (sort_algo != SX_SORT_ONESWEEP && sort_algo != SX_SORT_THREE_KERNEL && sort_algo != SX_SORT_HYBRID)
         return fail(SX_EINVAL, "sx_index_set_algorithms: bad sort algorithm");
     ix->knn_algo = knn_algo;
     ix->sort_algo = sort_algo;

@@ -149,6 +149,35 @@ def test_sort_pairs_arbitrary_keys(gpu):
         assert np.array_equal(vo.numpy(), ref) and np.array_equal(ko.numpy(), keys[ref])
 
 
+@pytest.mark.parametrize("bits", [63, 64, 52])
+def test_sort_pairs_bucket_edges(gpu, bits):
+    """Keys whose top bits (the four digits the sort places first) form
+    buckets of every size class and at every class edge: ranked in the local
+    pass (up to 64 keys), sorted by one wave (65-1024), sorted by the full
+    passes (larger); buckets across 2048-key tile edges, large ones next to
+    small ones, duplicate keys inside buckets of every class; explicit
+    values; a shuffled input."""
+    g = rng(11 + bits)
+    low_bits = 8 * ((bits + 7) // 8 - 4)  # bits below the four top digits
+    sizes = [1, 2, 3, 63, 64, 65, 255, 256, 257, 511, 1023, 1024, 1025, 2047, 2048, 2049, 5000, 70_000, 17, 64, 1]
+    sizes += list(g.integers(1, 300, size=3000)) + list(g.integers(900, 1200, size=40))
+    top = np.repeat(np.arange(len(sizes), dtype=np.uint64) * np.uint64(7919) + np.uint64(3), sizes)
+    top_bits = bits - low_bits
+    top = top % np.uint64(1 << top_bits)
+    if bits == 64:
+        top |= np.uint64(1 << (top_bits - 1))  # keys with bit 63 set
+    low = g.integers(0, 1 << low_bits, size=len(top), dtype=np.uint64)
+    low[g.random(len(low)) < 0.3] = np.uint64(5)  # duplicate keys within buckets
+    keys = (top << np.uint64(low_bits)) | low
+    perm = g.permutation(len(keys))
+    keys = keys[perm]
+    vals = g.integers(0, 2**32, size=len(keys), dtype=np.uint64).astype(np.uint32)
+    ko, vo = gpu.sort_pairs(gpu.DeviceArray.from_numpy(keys), bits, gpu.DeviceArray.from_numpy(vals), check=True)
+    ref = np.argsort(keys, kind="stable")
+    assert np.array_equal(ko.numpy(), keys[ref])
+    assert np.array_equal(vo.numpy(), vals[ref])
+
+
 def test_index_object_matches_oracle(gpu, O):
     x, y, z = gpu.gen_scene(2, 1_000_000)
     ix = gpu.Index(len(x), 63).build(x, y, z)
@@ -321,11 +350,12 @@ def test_knn_algorithms_agree(gpu, kind, seed, n, k):
     print(f"[knn] {kind}: plain tiles deferred {deferred / len(lane):.3%} to one lane per query")
 
 
-def test_sort_algorithms_agree(gpu):
+@pytest.mark.parametrize("algo", ["SX_SORT_THREE_KERNEL", "SX_SORT_ONESWEEP"])
+def test_sort_algorithms_agree(gpu, algo):
     x, y, z = gpu.gen_scene(2, 2_000_000)
-    a = gpu.Index(len(x), 63).build(x, y, z).fetch()
+    a = gpu.Index(len(x), 63).build(x, y, z).fetch()  # hybrid sort (default)
     ixb = gpu.Index(len(x), 63)
-    ixb.set_algorithms(sort=gpu.SX_SORT_THREE_KERNEL)
+    ixb.set_algorithms(sort=getattr(gpu, algo))
     b = ixb.build(x, y, z).fetch()
     for key in ("codes", "sorted_perm", "rep", "points"):
         assert np.array_equal(a[key], b[key]), key
