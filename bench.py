@@ -39,6 +39,11 @@ def knn_bytes_per_query(k: int) -> int:
     return 16 + 64 + 4 * k
 
 
+# Passes over the (key, value) pairs of the 63-bit sort: four onesweep passes
+# on the top digits and one pass that sorts every bucket of equal top bits.
+SORT_PAIR_PASSES = 5
+
+
 def pipeline_bytes_per_point(key_bytes: int, passes: int, k: int, r: float) -> float:
     """Byte model of the whole path per input point (137 + 8K + 2P(K+4) + 112r + 8k)."""
     return 137 + 8 * key_bytes + 2 * passes * (key_bytes + 4) + 112 * r + 8 * k
@@ -282,7 +287,7 @@ def main() -> int:
     if tf.exists() and n == 8_388_608 and k == 16:
         traffic = json.loads(tf.read_text()).get("hbm_bytes_per_call", {}).get(head)
     r = O_nodes / U
-    pipe_bytes = pipeline_bytes_per_point(8, 8, k, r) * n
+    pipe_bytes = pipeline_bytes_per_point(8, SORT_PAIR_PASSES, k, r) * n
 
     def stage(res):
         return {
@@ -336,7 +341,7 @@ def main() -> int:
         },
         "valu_roofline": valu_roofline(head, knn_avg, n, k),
         "pipeline_roofline": {
-            "bytes_per_point": pipeline_bytes_per_point(8, 8, k, r),
+            "bytes_per_point": pipeline_bytes_per_point(8, SORT_PAIR_PASSES, k, r),
             "achieved_GBs": pipe_bytes / (ms_per_step / 1000.0) / 1e9,
             "frac": pipe_bytes / (ms_per_step / 1000.0) / 1e9 / HBM_PEAK_GBS,
         },
