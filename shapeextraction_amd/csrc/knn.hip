@@ -710,14 +710,12 @@ __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, co
 
 // Waves per SIMD, measured on the 8M scene (K = 16): plain eight (64 VGPRs,
 // 16 bytes per lane spilled; kNN 2.72 ms against 2.76 at seven with 72 VGPRs
-// and 2.89 at six with 77), seeded seven (eight spill 44 bytes per lane).
-// Six for K = 24 (80 VGPRs, 64 bytes per lane spilled; 4M degenerate scene:
-// kNN 2.61 ms against 2.65 at five waves and 2.97 at the natural 102 VGPRs,
-// four waves).
-// K = 32 keeps its natural allocation (about 120 VGPRs; forced to five waves
-// it spilled heavily: 35.4 ms against 22.1 on the 16M mixture).
+// and 2.89 at six with 77), seeded seven (eight: no gain).  K = 24: seven
+// (4M degenerate scene: kNN 2.41 ms against 2.46 at six, 5.6 at eight);
+// K = 32: five (16M mixture: 11.02 ms against 12.16 at its natural ~120
+// VGPRs, four waves, and 11.49 at six).
 template <int K, bool kLearned>
-__global__ void __launch_bounds__(kKnnBlock) __attribute__((amdgpu_waves_per_eu(K <= 16 ? (kLearned ? 7 : 8) : (K <= 24 ? 6 : 1), 8)))
+__global__ void __launch_bounds__(kKnnBlock) __attribute__((amdgpu_waves_per_eu(K <= 16 ? (kLearned ? 7 : 8) : (K <= 24 ? 7 : 5), 8)))
 knn_tile_kernel(const float4* __restrict__ pts,
                                                              const TravNode* __restrict__ nodes,
                                                              const int32_t* __restrict__ d_counts, int64_t q_start,
@@ -736,14 +734,12 @@ knn_tile_kernel(const float4* __restrict__ pts,
     };
     __shared__ TileScratch s_scr[kTileWaves];
     __shared__ float4 s_cand[kTileWaves][kWave + 1];
-    // K <= 16: wave-uniform values, the query index and the query point are
-    // kept out of the long-lived vector registers (scalar registers, or
-    // recomputed / reloaded where used): those kernels run at 64-72 VGPRs
-    // and would spill them; the larger lists have the registers and measured
-    // slower with these changes
-    constexpr bool kLean = K <= 16;
-    const int w = kLean ? __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave)
-                        : static_cast<int>(threadIdx.x) / kWave;
+    // Wave-uniform values, the query index and the query point are kept out
+    // of the long-lived vector registers (scalar registers, or recomputed /
+    // reloaded where used): every list length runs at an occupancy that
+    // would otherwise spill them (K = 24 and 32: 2.61 -> 2.41 ms and 12.07 ->
+    // 11.02 ms with the occupancies above).
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
     const int lane = static_cast<int>(lane_id());
     const int total = d_counts[0];
     const int64_t q_req = q_start + n_q;
@@ -756,8 +752,8 @@ knn_tile_kernel(const float4* __restrict__ pts,
     if (tile0 >= q_end) return;  // whole wave idle (no block barriers below)
     // the tile's cloud: points and radix nodes [cbase, cbase + U)
     const int2 cloud = cm.at(tile0, total);
-    const int cbase = kLean ? __builtin_amdgcn_readfirstlane(cloud.x) : cloud.x;
-    const int U = kLean ? __builtin_amdgcn_readfirstlane(cloud.y) : cloud.y;
+    const int cbase = __builtin_amdgcn_readfirstlane(cloud.x);
+    const int U = __builtin_amdgcn_readfirstlane(cloud.y);
     const int cend = cbase + U;
     const int64_t qi = tile0 + lane;
     const bool valid = qi < q_end && qi < cend;
@@ -773,10 +769,9 @@ knn_tile_kernel(const float4* __restrict__ pts,
         constexpr int hw = kWindow<K>;
         wlo = static_cast<int>(tile0 - hw > cbase ? tile0 - hw : cbase);
         whi = static_cast<int>(tile0 + kWave - 1 + hw < cend - 1 ? tile0 + kWave - 1 + hw : cend - 1);
-        if (kLean) {  // wave-uniform: kept in scalar registers
-            wlo = __builtin_amdgcn_readfirstlane(wlo);
-            whi = __builtin_amdgcn_readfirstlane(whi);
-        }
+        // wave-uniform: kept in scalar registers
+        wlo = __builtin_amdgcn_readfirstlane(wlo);
+        whi = __builtin_amdgcn_readfirstlane(whi);
         if constexpr (kNetWindow<K>) {
             // the whole window in LDS; slots past it hold points at +inf,
             // whose keys follow every finite one, so they reach the list only
@@ -867,8 +862,8 @@ knn_tile_kernel(const float4* __restrict__ pts,
         const bool in_seg = (seg >> lane) & 1ull;
         const float rps = opaque(rp);  // recomputed per segment rather than held live
         const int ls = opaque(lane);
-        // K <= 16: the query point loaded again (a cached read)
-        const float4 qs = kLean ? (valid ? pts[tile0 + ls] : make_float4(0.f, 0.f, 0.f, 0.f)) : q;
+        // the query point loaded again (a cached read)
+        const float4 qs = valid ? pts[tile0 + ls] : make_float4(0.f, 0.f, 0.f, 0.f);
         const float ex0 = uniform_f(wave_min_at(in_seg ? nextafterf(qs.x - rps, -INFINITY) : INFINITY, ls));
         const float ey0 = uniform_f(wave_min_at(in_seg ? nextafterf(qs.y - rps, -INFINITY) : INFINITY, ls));
         const float ez0 = uniform_f(wave_min_at(in_seg ? nextafterf(qs.z - rps, -INFINITY) : INFINITY, ls));
@@ -963,7 +958,7 @@ knn_tile_kernel(const float4* __restrict__ pts,
     // uncapped: every point within the window radius was a candidate;
     // capped: the k-th distance must lie within the radius the box proves
     const bool exact = join && !lost && (capped ? top.kth_bits() <= lim : true);
-    const int64_t qo = kLean ? tile0 + opaque(lane) : qi;  // the query index (K <= 16: again)
+    const int64_t qo = tile0 + opaque(lane);  // the query index, recomputed
     if (valid && exact) write_row<K>(out + (qo - q_start) * K, top, cbase);
     const bool defer = valid && !exact;
     const uint64_t md = __ballot(defer);
