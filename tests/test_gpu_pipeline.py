@@ -178,6 +178,22 @@ def test_sort_pairs_bucket_edges(gpu, bits):
     assert np.array_equal(vo.numpy(), vals[ref])
 
 
+@pytest.mark.parametrize("n", [2048, 2049, 2111, 2177, 4097, 6000])
+def test_sort_pairs_hybrid_small_inputs(gpu, n):
+    """The hybrid schedule from its smallest input (one 2,048-key local tile)
+    up: clustered 63-bit keys (buckets of every class inside a few tiles),
+    duplicates, and the last tile partly filled."""
+    g = rng(n)
+    top = g.integers(0, 40, size=n, dtype=np.uint64) * np.uint64(7)  # few buckets: large ones
+    low = g.integers(0, 1 << 32, size=n, dtype=np.uint64)
+    low[g.random(n) < 0.25] = np.uint64(9)
+    keys = (top << np.uint64(32)) | low
+    keys[: n // 8] = g.integers(0, 2**63, size=n // 8, dtype=np.uint64)  # and some singletons
+    ko, vo = gpu.sort_pairs(gpu.DeviceArray.from_numpy(keys), 63, check=True)
+    ref = np.argsort(keys, kind="stable")
+    assert np.array_equal(vo.numpy(), ref) and np.array_equal(ko.numpy(), keys[ref])
+
+
 def test_index_object_matches_oracle(gpu, O):
     x, y, z = gpu.gen_scene(2, 1_000_000)
     ix = gpu.Index(len(x), 63).build(x, y, z)
@@ -350,9 +366,10 @@ def test_knn_algorithms_agree(gpu, kind, seed, n, k):
     print(f"[knn] {kind}: plain tiles deferred {deferred / len(lane):.3%} to one lane per query")
 
 
-@pytest.mark.parametrize("algo", ["SX_SORT_THREE_KERNEL", "SX_SORT_ONESWEEP"])
-def test_sort_algorithms_agree(gpu, algo):
-    x, y, z = gpu.gen_scene(2, 2_000_000)
+@pytest.mark.parametrize("algo,n", [("SX_SORT_THREE_KERNEL", 2_000_000), ("SX_SORT_ONESWEEP", 2_000_000),
+                                    ("SX_SORT_ONESWEEP", 2_100)])
+def test_sort_algorithms_agree(gpu, algo, n):
+    x, y, z = gpu.gen_scene(2, n)
     a = gpu.Index(len(x), 63).build(x, y, z).fetch()  # hybrid sort (default)
     ixb = gpu.Index(len(x), 63)
     ixb.set_algorithms(sort=getattr(gpu, algo))
