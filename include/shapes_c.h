@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SX_ABI_VERSION 1
+#define SX_ABI_VERSION 2
 
 /* Status codes. */
 #define SX_OK 0
@@ -368,8 +368,12 @@ int sx_radius_predict(sx_index* index, sx_mlp* mlp, int64_t q_start, int64_t n_q
  * unique points occupy slots [cloud_base[c], cloud_base[c] + cloud_size[c])
  * of the per-slot arrays (cloud_base[c] a multiple of 64); neighbour rows and
  * normals are per slot, with indices relative to the cloud's first slot, and
- * rep[] is relative to the cloud's first input point.  kNN is plain (the
- * learned radius needs octree features, which the batch does not build).
+ * rep[] is relative to the cloud's first input point.  With
+ * sx_batch_set_octree, every build also makes each cloud's octree (the
+ * reference's OT::Octree constructor, src/Octree.cu:213-319, once per cloud):
+ * cloud c's nodes are oct_nodes[cloud_oct_base[c] ...] (cloud_oct_size[c]
+ * nodes), byte-identical to a standalone index's octree of that cloud (child
+ * links relative to the cloud), and the learned radius runs on the batch.
  * ------------------------------------------------------------------------- */
 typedef struct sx_batch sx_batch;
 
@@ -388,6 +392,13 @@ typedef struct sx_batch_view {
     const uint8_t* prefix_n;       /* [n_slots] radix tree, slot numbering     */
     const int32_t* left_child;
     const int32_t* parent;
+    /* per-cloud octrees (NULL / 0 unless sx_batch_set_octree enabled them) */
+    int64_t n_oct;                 /* octree nodes over all clouds            */
+    const sx_otnode* oct_nodes;    /* [n_oct]                                 */
+    const int32_t* cloud_oct_base; /* [n_clouds] first node of each cloud     */
+    const int32_t* cloud_oct_size; /* [n_clouds] nodes of each cloud          */
+    const uint8_t* has_leaf_left;  /* [n_slots] radix tree, slot numbering    */
+    const uint8_t* has_leaf_right;
 } sx_batch_view;
 
 int sx_batch_create(sx_batch** out, int64_t max_points, int max_clouds, int code_bits);
@@ -396,6 +407,11 @@ int sx_batch_destroy(sx_batch* batch);
 int sx_batch_set_clouds(sx_batch* batch, const int64_t* h_offsets, int n_clouds, void* stream);
 /* bucket: leaf-range size of the kNN; queue_blocks: deferred-query workgroups (0 = default). */
 int sx_batch_set_tuning(sx_batch* batch, int bucket, int sort_algo, int queue_blocks);
+/* oct_factor > 0: later builds (and graphs created later) also build every
+ * cloud's octree, in storage for oct_factor x slot-capacity nodes allocated
+ * here (3 is ample for scans; the view reports SX_ERANGE when a build needs
+ * more); 0 frees it.  Synchronises the device; the batch must be rebuilt. */
+int sx_batch_set_octree(sx_batch* batch, float oct_factor);
 /* Asynchronous, no host synchronisation, capturable. */
 int sx_batch_build(sx_batch* batch, const float* d_x, const float* d_y, const float* d_z, void* stream);
 /* Waits for the last sx_batch_build (whatever its stream); SX_EDATA when an
@@ -405,6 +421,16 @@ int sx_batch_get_view(sx_batch* batch, sx_batch_view* view, void* stream);
  * < min(n_rows, n_slots) (rows of padding slots are not written); n_rows is
  * the row capacity of the caller's buffers (n_slots from the view). */
 int sx_batch_knn(sx_batch* batch, int k, int32_t* d_out, int64_t n_rows, void* stream);
+/* kNN seeded by a per-slot squared search radius (the batch form of sx_knn in
+ * SX_KNN_RADIUS mode): the same rows as sx_batch_knn, for any radii. */
+int sx_batch_knn_radius(sx_batch* batch, int k, const float* d_radius2, int32_t* d_out, int64_t n_rows,
+                        void* stream);
+/* Learned-radius features (n_rows x 8 floats, 16-byte aligned) and fused
+ * features + MLP radii of slots [0, n_rows); per cloud equal to
+ * sx_radius_features / sx_radius_predict of a standalone index (rows of
+ * padding slots are zero features).  Need the per-cloud octrees (SX_ESTATE). */
+int sx_batch_radius_features(sx_batch* batch, float* d_feats, int64_t n_rows, void* stream);
+int sx_batch_radius_predict(sx_batch* batch, sx_mlp* mlp, float* d_radius2, int64_t n_rows, void* stream);
 int sx_batch_normals(sx_batch* batch, const int32_t* d_nn, int k, int precision, float vx, float vy, float vz,
                      float* d_normals, uint8_t* d_flags, int64_t n_rows, void* stream);
 /* Queries the kNN tiles handed to the per-query search since the last reset. */

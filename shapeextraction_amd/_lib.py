@@ -89,6 +89,12 @@ class BatchView(C.Structure):
         ("prefix_n", C.c_void_p),
         ("left_child", C.c_void_p),
         ("parent", C.c_void_p),
+        ("n_oct", C.c_int64),
+        ("oct_nodes", C.c_void_p),
+        ("cloud_oct_base", C.c_void_p),
+        ("cloud_oct_size", C.c_void_p),
+        ("has_leaf_left", C.c_void_p),
+        ("has_leaf_right", C.c_void_p),
     ]
 
 
@@ -173,9 +179,13 @@ SIGNATURES = {
     "sx_batch_destroy": (I, [P]),
     "sx_batch_set_clouds": (I, [P, C.POINTER(C.c_int64), I, P]),
     "sx_batch_set_tuning": (I, [P, I, I, I]),
+    "sx_batch_set_octree": (I, [P, F]),
     "sx_batch_build": (I, [P, P, P, P, P]),
     "sx_batch_get_view": (I, [P, C.POINTER(BatchView), P]),
     "sx_batch_knn": (I, [P, I, P, I64, P]),
+    "sx_batch_knn_radius": (I, [P, I, P, P, I64, P]),
+    "sx_batch_radius_features": (I, [P, P, I64, P]),
+    "sx_batch_radius_predict": (I, [P, P, P, I64, P]),
     "sx_batch_normals": (I, [P, P, I, I, F, F, F, P, P, I64, P]),
     "sx_batch_deferred": (I, [P, C.POINTER(C.c_int64), I, P]),
     "sx_batch_graph_create": (I, [P, P, P, P, I, P, P, I64, C.POINTER(P), P]),

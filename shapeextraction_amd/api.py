@@ -405,6 +405,12 @@ class Batch:
     def set_tuning(self, bucket: int = 16, sort: int = _lib.SX_SORT_HYBRID, queue_blocks: int = 0):
         call("sx_batch_set_tuning", self.handle, bucket, sort, queue_blocks)
 
+    def set_octree(self, oct_factor: float = 3.0) -> "Batch":
+        """Build every cloud's octree from the next build on (0: stop, free it)."""
+        call("sx_batch_set_octree", self.handle, float(oct_factor))
+        self._view = None
+        return self
+
     def build(self, x, y, z, stream=None) -> "Batch":
         dx, dy, dz = _dev(x, np.float32), _dev(y, np.float32), _dev(z, np.float32)
         self._inputs = (dx, dy, dz)
@@ -443,6 +449,16 @@ class Batch:
             "left_child": fetch(v.left_child, S, np.int32),
             "parent": fetch(v.parent, S, np.int32),
         })
+        if v.oct_nodes:
+            n, O = int(v.n_clouds), int(v.n_oct)
+            out.update({
+                "n_oct": O,
+                "oct_nodes": fetch(v.oct_nodes, O, OTNODE_DTYPE),
+                "oct_base": fetch(v.cloud_oct_base, n, np.int32),
+                "oct_size": fetch(v.cloud_oct_size, n, np.int32),
+                "has_leaf_left": fetch(v.has_leaf_left, S, np.uint8),
+                "has_leaf_right": fetch(v.has_leaf_right, S, np.uint8),
+            })
         return out
 
     def knn(self, k: int, out: DeviceArray | None = None, stream=None) -> DeviceArray:
@@ -450,6 +466,31 @@ class Batch:
         if out is None:
             out = DeviceArray((int(v.n_slots), k), np.int32)
         call("sx_batch_knn", self.handle, k, out.ptr, out.shape[0], stream)
+        return out
+
+    def knn_radius(self, k: int, radius2: DeviceArray, out: DeviceArray | None = None, stream=None) -> DeviceArray:
+        """kNN seeded by per-slot squared radii (same rows as knn)."""
+        v = self._view or self.view(stream)
+        if out is None:
+            out = DeviceArray((int(v.n_slots), k), np.int32)
+        rows = min(out.shape[0], radius2.shape[0])
+        call("sx_batch_knn_radius", self.handle, k, radius2.ptr, out.ptr, rows, stream)
+        return out
+
+    def radius_features(self, out: DeviceArray | None = None, stream=None) -> DeviceArray:
+        """Learned-radius features of every slot (needs set_octree before the build)."""
+        v = self._view or self.view(stream)
+        if out is None:
+            out = DeviceArray((int(v.n_slots), 8), np.float32)
+        call("sx_batch_radius_features", self.handle, out.ptr, out.shape[0], stream)
+        return out
+
+    def radius_predict(self, model, out: DeviceArray | None = None, stream=None) -> DeviceArray:
+        """Squared search radius of every slot from a RadiusModel (features + MLP fused)."""
+        v = self._view or self.view(stream)
+        if out is None:
+            out = DeviceArray((int(v.n_slots),), np.float32)
+        call("sx_batch_radius_predict", self.handle, model.handle, out.ptr, out.shape[0], stream)
         return out
 
     def normals(self, nn: DeviceArray, k: int, precision: int = SX_PREC_FP32, viewpoint=(0.0, 0.0, 0.0),

@@ -18,8 +18,12 @@
 //                        standalone tree shifted by base[c]
 //   boxes, kNN, normals  the single-cloud kernels with the cloud map
 //
-// Not built per cloud: the octree (the batched path stops at neighbours and
-// normals; the learned radius needs octree features, so batched kNN is plain).
+//   octrees (optional)   one scan of the radix tree's edge counts over all
+//                        slots, then oct_gather_kernel with the cloud map:
+//                        cloud c's nodes are one run from oct_base[c], its
+//                        links relative to the cloud, so the run is the
+//                        standalone octree byte for byte; the feature links
+//                        follow, so the learned radius runs on batches too
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,10 +53,18 @@ int knn_tiles_launch(const float4* pts, const TravNode* nodes, ParentRef par, co
 int normals_launch(const float4* pts, const int32_t* d_counts, const int32_t* nn, int64_t q_start, int64_t n_q, int k,
                    int precision, float vx, float vy, float vz, float* normals, uint8_t* flags, hipStream_t st,
                    CloudMap cm);
+int batch_octree_launch(const uint64_t* codes, const int32_t* d_slots, int64_t slot_cap, int code_bits,
+                        const float* bounds, const uint8_t* pn, const int32_t* left_child, const uint8_t* hll,
+                        const uint8_t* hlr, const int32_t* counts, int32_t* offsets, CloudMap cm, int n_clouds,
+                        int32_t* oct_base, int32_t* oct_size, sx_otnode* nodes, int64_t cap, int32_t* d_oct_count,
+                        void* scan_ws, hipStream_t st);
+int feature_links_launch(const int32_t* d_count, int64_t n_max, const int32_t* parent, const int32_t* leaf_parent,
+                         const int32_t* node_end, const int32_t* offsets, const int32_t* counts, int64_t cap,
+                         int32_t* leaf_oct, int2* aux, hipStream_t st, CloudMap cm);
 
 // counts[] slots (the first four are cleared by every build; kBSortErr is
 // the sorts' look-back error flag)
-enum { kBSlots = 0, kBUnique = 1, kBBad = 2, kBSortErr = 3, kBDeferred = 4, kBQueue = 5 };
+enum { kBSlots = 0, kBUnique = 1, kBBad = 2, kBSortErr = 3, kBDeferred = 4, kBQueue = 5, kBOct = 6 };
 
 // Cloud of each element in code order: a binary search of the cloud offsets,
 // staged in LDS when they fit (each block handles kCloudKeysItems elements
@@ -194,17 +206,44 @@ struct sx_batch {
     sx::TravNode* trav = nullptr;
     int32_t* counts = nullptr;
     void* ws = nullptr;
+    // per-cloud octrees (own allocation, present when enabled)
+    void* oct_mem = nullptr;
+    int64_t oct_cap = 0;
+    uint8_t *hll = nullptr, *hlr = nullptr;                // per slot
+    int32_t *oct_cnt = nullptr, *oct_off = nullptr, *leaf_parent = nullptr, *node_end = nullptr,
+            *leaf_oct = nullptr;                           // per slot
+    int32_t *oct_base = nullptr, *oct_size = nullptr;      // per cloud
+    sx_otnode* oct = nullptr;                              // per octree node
+    int2* oct_aux = nullptr;                               // per octree node: (parent, points)
     // host state
     int n_clouds = 0;
     int64_t n_input = 0;
     bool built = false;
+    bool octree() const { return oct_mem != nullptr; }
     sx::CloudMap map() const { return sx::CloudMap{tile_cloud, base, size}; }
 };
 
 namespace {
 size_t batch_ws_bytes(int64_t cap, int64_t slot_cap) {
-    size_t ws = std::max(sx::sort_ws_bytes(cap), sx::scan_ws_bytes(cap));
-    return ws;
+    // the octree's count scan runs over every slot
+    return std::max({sx::sort_ws_bytes(cap), sx::scan_ws_bytes(cap), sx::scan_ws_bytes(slot_cap)});
+}
+
+size_t octree_layout(sx_batch* b, void* base) {
+    sx::Carver c(base);
+    const int64_t s = b->slot_cap, mc = b->max_clouds, o = b->oct_cap;
+    b->hll = c.take<uint8_t>(s);
+    b->hlr = c.take<uint8_t>(s);
+    b->oct_cnt = c.take<int32_t>(s);
+    b->oct_off = c.take<int32_t>(s);
+    b->leaf_parent = c.take<int32_t>(s);
+    b->node_end = c.take<int32_t>(s);
+    b->leaf_oct = c.take<int32_t>(s);
+    b->oct_base = c.take<int32_t>(mc);
+    b->oct_size = c.take<int32_t>(mc);
+    b->oct = c.take<sx_otnode>(o);
+    b->oct_aux = c.take<int2>(o);
+    return c.off;
 }
 
 size_t batch_layout(sx_batch* b, void* base) {
@@ -280,17 +319,31 @@ int batch_build(sx_batch* b, const float* x, const float* y, const float* z, hip
     SX_LAUNCH_CHECK("compact_kernel");
     const int32_t* slots = b->counts + sx::kBSlots;
     SX_TRY(sx::seg_decode_launch(b->ucodes, slots, b->slot_cap, b->map(), b->bounds, cb, b->pts16, st));
-    SX_TRY(sx::karras_launch(b->ucodes, slots, b->slot_cap, cb, SX_ROOT_TEXTBOOK, b->pn, b->lc, nullptr, nullptr,
-                             b->parent, nullptr, nullptr, nullptr, b->trav, b->pts16, b->bounds, st, b->map()));
+    if (!b->octree()) {
+        SX_TRY(sx::karras_launch(b->ucodes, slots, b->slot_cap, cb, SX_ROOT_TEXTBOOK, b->pn, b->lc, nullptr, nullptr,
+                                 b->parent, nullptr, nullptr, nullptr, b->trav, b->pts16, b->bounds, st, b->map()));
+        return SX_OK;
+    }
+    // with octrees: the radix-tree kernel also writes the leaf flags, range
+    // ends, leaf parents and octree edge counts
+    SX_TRY(sx::karras_launch(b->ucodes, slots, b->slot_cap, cb, SX_ROOT_TEXTBOOK, b->pn, b->lc, b->hll, b->hlr,
+                             b->parent, b->node_end, b->leaf_parent, nullptr, b->trav, b->pts16, b->bounds, st,
+                             b->map(), b->oct_cnt));
+    SX_TRY(sx::batch_octree_launch(b->ucodes, slots, b->slot_cap, cb, b->bounds, b->pn, b->lc, b->hll, b->hlr,
+                                   b->oct_cnt, b->oct_off, b->map(), nc, b->oct_base, b->oct_size, b->oct, b->oct_cap,
+                                   b->counts + sx::kBOct, b->ws, st));
+    SX_TRY(sx::feature_links_launch(slots, b->slot_cap, b->parent, b->leaf_parent, b->node_end, b->oct_off,
+                                    b->oct_cnt, b->oct_cap, b->leaf_oct, b->oct_aux, st, b->map()));
     return SX_OK;
 }
 
 // Rows are written for slots < min(n_rows, slot count): n_rows is the row
 // capacity of the caller's buffers.
-int batch_knn(sx_batch* b, int k, int32_t* out, int64_t n_rows, hipStream_t st) {
+// radius2: per-slot squared search radius (learned seeds), or null.
+int batch_knn(sx_batch* b, int k, int32_t* out, int64_t n_rows, hipStream_t st, const float* radius2 = nullptr) {
     sx::ParentRef par;
     par.p = b->parent;
-    return sx::knn_tiles_launch(b->pts16, b->trav, par, b->counts + sx::kBSlots, 0, n_rows, k, nullptr, b->bucket,
+    return sx::knn_tiles_launch(b->pts16, b->trav, par, b->counts + sx::kBSlots, 0, n_rows, k, radius2, b->bucket,
                                 out, b->queue, b->counts + sx::kBQueue, b->counts + sx::kBDeferred, st, b->map(),
                                 b->queue_blocks, nullptr);
 }
@@ -302,6 +355,31 @@ int batch_normals(sx_batch* b, const int32_t* nn, int k, int precision, float vx
 }
 
 bool valid_k(int k) { return k == 8 || k == 16 || k == 24 || k == 32; }
+}  // namespace
+
+namespace sx {
+int batch_feature_arrays(sx_batch* b, BatchFeatures* out, void* stream) {
+    SX_TRY(check_batch(b));
+    if (!b->octree())
+        return fail(SX_ESTATE, "batch features need the per-cloud octrees: call sx_batch_set_octree, then build");
+    // ordered after the last build, whichever stream it ran on
+    SX_HIP(hipStreamWaitEvent(as_stream(stream), b->built_event, 0));
+    out->trav = b->trav;
+    out->aux = b->oct_aux;
+    out->pn = b->pn;
+    out->leaf_parent = b->leaf_parent;
+    out->leaf_oct = b->leaf_oct;
+    out->oct = b->oct;
+    out->oct_cap = b->oct_cap;
+    out->bounds = b->bounds;
+    out->slots = b->counts + kBSlots;
+    out->cm = b->map();
+    out->limit = b->slot_cap;
+    return SX_OK;
+}
+}  // namespace sx
+
+namespace {
 
 struct BatchGraph {
     hipGraph_t graph = nullptr;
@@ -346,8 +424,35 @@ int sx_batch_create(sx_batch** out, int64_t max_points, int max_clouds, int code
 int sx_batch_destroy(sx_batch* b) {
     if (!b) return SX_OK;
     if (b->built_event) SX_HIP(hipEventDestroy(b->built_event));
+    if (b->oct_mem) SX_HIP(hipFree(b->oct_mem));
     if (b->mem) SX_HIP(hipFree(b->mem));
     delete b;
+    return SX_OK;
+}
+
+int sx_batch_set_octree(sx_batch* b, float oct_factor) {
+    if (!b) return fail(SX_EINVAL, "sx_batch_set_octree: null batch");
+    if (!(oct_factor >= 0.0f) || oct_factor > 64.0f)
+        return fail(SX_EINVAL, "sx_batch_set_octree: oct_factor must be in [0, 64]");
+    SX_HIP(hipDeviceSynchronize());  // no build or query may still use the old storage
+    if (b->oct_mem) {
+        SX_HIP(hipFree(b->oct_mem));
+        b->oct_mem = nullptr;
+        b->oct_cap = 0;
+    }
+    b->built = false;
+    if (oct_factor == 0.0f) return SX_OK;
+    b->oct_cap = std::max<int64_t>(16, static_cast<int64_t>(static_cast<double>(oct_factor) *
+                                                            static_cast<double>(b->slot_cap)));
+    const size_t bytes = octree_layout(b, nullptr);
+    void* mem = nullptr;
+    if (hipMalloc(&mem, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        b->oct_cap = 0;
+        return fail(SX_ENOMEM, "sx_batch_set_octree: device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    b->oct_mem = mem;
+    octree_layout(b, mem);
     return SX_OK;
 }
 
@@ -402,6 +507,9 @@ int sx_batch_get_view(sx_batch* b, sx_batch_view* view, void* stream) {
         return fail(SX_EINTERNAL, "sx_batch_get_view: a sort of the build gave up waiting in a look-back; "
                                   "the batch is empty and must be rebuilt");
     if (h[sx::kBBad] != 0) return fail(SX_EDATA, "sx_batch_get_view: non-finite input coordinates");
+    if (b->octree() && (h[sx::kBOct] < 0 || h[sx::kBOct] > b->oct_cap))
+        return fail(SX_ERANGE, "sx_batch_get_view: the octrees need " + std::to_string(h[sx::kBOct]) +
+                                   " nodes, capacity is " + std::to_string(b->oct_cap) + " (raise oct_factor)");
     view->n_clouds = b->n_clouds;
     view->n_input = b->n_input;
     view->n_slots = h[sx::kBSlots];
@@ -416,6 +524,13 @@ int sx_batch_get_view(sx_batch* b, sx_batch_view* view, void* stream) {
     view->prefix_n = b->pn;
     view->left_child = b->lc;
     view->parent = b->parent;
+    const bool oct = b->octree();
+    view->n_oct = oct ? h[sx::kBOct] : 0;
+    view->oct_nodes = oct ? b->oct : nullptr;
+    view->cloud_oct_base = oct ? b->oct_base : nullptr;
+    view->cloud_oct_size = oct ? b->oct_size : nullptr;
+    view->has_leaf_left = oct ? b->hll : nullptr;
+    view->has_leaf_right = oct ? b->hlr : nullptr;
     return SX_OK;
 }
 
@@ -426,6 +541,17 @@ int sx_batch_knn(sx_batch* b, int k, int32_t* d_out, int64_t n_rows, void* strea
         return fail(SX_EINVAL, "sx_batch_knn: output must be a 16-byte aligned device buffer");
     if (n_rows < 0 || n_rows > b->slot_cap) return fail(SX_EINVAL, "sx_batch_knn: n_rows outside [0, slot capacity]");
     return batch_knn(b, k, d_out, n_rows, sx::as_stream(stream));
+}
+
+int sx_batch_knn_radius(sx_batch* b, int k, const float* d_radius2, int32_t* d_out, int64_t n_rows, void* stream) {
+    SX_TRY(check_batch(b));
+    if (!valid_k(k)) return fail(SX_EINVAL, "sx_batch_knn_radius: k must be 8, 16, 24 or 32");
+    if (!d_radius2) return fail(SX_EINVAL, "sx_batch_knn_radius: null radius2");
+    if (!d_out || reinterpret_cast<uintptr_t>(d_out) % 16 != 0)
+        return fail(SX_EINVAL, "sx_batch_knn_radius: output must be a 16-byte aligned device buffer");
+    if (n_rows < 0 || n_rows > b->slot_cap)
+        return fail(SX_EINVAL, "sx_batch_knn_radius: n_rows outside [0, slot capacity]");
+    return batch_knn(b, k, d_out, n_rows, sx::as_stream(stream), d_radius2);
 }
 
 int sx_batch_normals(sx_batch* b, const int32_t* d_nn, int k, int precision, float vx, float vy, float vz,

@@ -48,7 +48,7 @@ int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
                   bool counts_given = false);
 int feature_links_launch(const int32_t* d_count, int64_t n_max, const int32_t* parent, const int32_t* leaf_parent,
                          const int32_t* node_end, const int32_t* offsets, const int32_t* counts, int64_t cap,
-                         int32_t* leaf_oct, int2* aux, hipStream_t st);
+                         int32_t* leaf_oct, int2* aux, hipStream_t st, CloudMap cm = CloudMap());
 int knn_launch(const float4* pts, const TravNode* nodes, ParentRef par, const int32_t* d_counts, int64_t q_start,
                int64_t n_q, const float* queries, int k, const float* radius2, int bucket, int32_t* out, int32_t* reruns,
                hipStream_t st, CloudMap cm = CloudMap());

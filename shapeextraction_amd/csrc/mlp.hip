@@ -562,18 +562,43 @@ __device__ __forceinline__ void query_features(const FeatTree& tr, const int32_t
     }
 }
 
+// The cloud of point u: its first slot s0, unique points U, log2 of its cube
+// range.  One cloud (kBatch false): slot 0, the point count, bounds[2]; a
+// batch: cloud c = the cloud of u's 64-slot tile, cube at bounds[4c].
+template <bool kBatch>
+__device__ __forceinline__ void feature_cloud(const CloudMap& cm, const float* __restrict__ bounds, int total,
+                                              int64_t u, int64_t& s0, int& U, float& log2_range) {
+    if (kBatch) {
+        const int cid = cm.tile_cloud[u / kCloudAlign];
+        s0 = cm.base[cid];
+        U = cm.size[cid];
+        log2_range = __log2f(bounds[4 * cid + 2]);
+    } else {
+        s0 = 0;
+        U = total;
+        log2_range = __log2f(bounds[2]);
+    }
+}
+
+template <bool kBatch>
 __global__ void __launch_bounds__(256) features_kernel(FeatTree tr, const int32_t* __restrict__ leaf_parent,
                                                        const int32_t* __restrict__ leaf_oct,
                                                        const sx_otnode* __restrict__ oct,
                                                        const float* __restrict__ bounds,
                                                        const int32_t* __restrict__ d_counts, int64_t q_start,
-                                                       int64_t n_q, float* __restrict__ feats) {
+                                                       int64_t n_q, float* __restrict__ feats, CloudMap cm) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n_q) return;
-    const int U = d_counts[0];
+    const int total = d_counts[0];
     const int64_t u = q_start + t;
     float f[kIn] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (u < U && U >= 2) query_features(tr, leaf_parent, leaf_oct, oct, __log2f(bounds[2]), tr.prefix(0) / 3, u, f);
+    if (u < total) {
+        int64_t s0;
+        int U;
+        float log2_range;
+        feature_cloud<kBatch>(cm, bounds, total, u, s0, U, log2_range);
+        if (u - s0 < U && U >= 2) query_features(tr, leaf_parent, leaf_oct, oct, log2_range, tr.prefix(s0) / 3, u, f);
+    }
     float4* out = reinterpret_cast<float4*>(feats + t * kIn);
     out[0] = make_float4(f[0], f[1], f[2], f[3]);
     out[1] = make_float4(f[4], f[5], f[6], f[7]);
@@ -583,21 +608,23 @@ __global__ void __launch_bounds__(256) features_kernel(FeatTree tr, const int32_
 // 64 queries (one per lane), stages them in LDS and runs mlp_forward64 on
 // them, r2 = exp(y + margin): the same arithmetic as features_kernel followed
 // by mlp_forward_kernel (mode 1), without the 32-byte feature rows in HBM.
+template <bool kBatch>
 __global__ void __launch_bounds__(kMlpBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) radius_kernel(FeatTree tr, const int32_t* __restrict__ leaf_parent,
                                                            const int32_t* __restrict__ leaf_oct,
                                                            const sx_otnode* __restrict__ oct,
                                                            const float* __restrict__ bounds,
                                                            const int32_t* __restrict__ d_counts, int64_t q_start,
-                                                           int64_t n_q, MlpDev m, float* __restrict__ r2) {
+                                                           int64_t n_q, MlpDev m, float* __restrict__ r2,
+                                                           CloudMap cm) {
     __shared__ MlpLds s_w;
     __shared__ float4 s_f[kWavesPerBlock][2][kWave];  // features 0-3 and 4-7 of the wave's queries
     stage_weights(m, s_w);
     const int w = static_cast<int>(threadIdx.x) / kWave;
     const int l = static_cast<int>(lane_id());
     const int r = l % 16, g = l / 16;
-    const int U = d_counts[0];
-    const float log2_range = __log2f(bounds[2]);
-    const int root_level = U >= 2 ? tr.prefix(0) / 3 : 0;
+    const int U = d_counts[0];  // a batch: the slot count
+    const float log2_range = kBatch ? 0.0f : __log2f(bounds[2]);
+    const int root_level = !kBatch && U >= 2 ? tr.prefix(0) / 3 : 0;
     // persistent: each wave walks groups of 64 queries (no block barriers below)
     const int64_t groups = (n_q + kWave - 1) / kWave;
     for (int64_t grp = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w; grp < groups;
@@ -606,8 +633,18 @@ __global__ void __launch_bounds__(kMlpBlock) __attribute__((amdgpu_waves_per_eu(
         {
             const int64_t u = q_start + q0 + l;
             float f[kIn] = {0, 0, 0, 0, 0, 0, 0, 0};
-            if (q0 + l < n_q && u < U && U >= 2)
+            if (kBatch) {
+                if (q0 + l < n_q && u < U) {
+                    int64_t s0;
+                    int Uc;
+                    float lr;
+                    feature_cloud<true>(cm, bounds, U, u, s0, Uc, lr);
+                    if (u - s0 < Uc && Uc >= 2)
+                        query_features(tr, leaf_parent, leaf_oct, oct, lr, tr.prefix(s0) / 3, u, f);
+                }
+            } else if (q0 + l < n_q && u < U && U >= 2) {
                 query_features(tr, leaf_parent, leaf_oct, oct, log2_range, root_level, u, f);
+            }
             s_f[w][0][l] = make_float4(f[0], f[1], f[2], f[3]);
             s_f[w][1][l] = make_float4(f[4], f[5], f[6], f[7]);
         }
@@ -688,7 +725,7 @@ int pack_image(sx_mlp* m, hipStream_t st) {
 // blocks the queries need, at most the blocks the device holds at once (a
 // block that starts after the first ones finish would double the tail).  The
 // occupancy is queried once per device and kernel.
-std::atomic<int64_t> g_resident_forward[64], g_resident_radius[64];
+std::atomic<int64_t> g_resident_forward[64], g_resident_radius[64], g_resident_radius_batch[64];
 
 template <typename Kern>
 int persistent_grid(Kern kernel, std::atomic<int64_t>* cache, int64_t n, unsigned* grid) {
@@ -713,13 +750,16 @@ int check_mlp(const sx_mlp* m) { return m ? SX_OK : sx::fail(SX_EINVAL, "null ml
 
 }  // namespace
 
-// sx_index internals needed by the feature kernel
+// sx_index / sx_batch internals needed by the feature kernels
 struct sx_index;
+struct sx_batch;
 namespace sx {
 int index_feature_arrays(sx_index* ix, const TravNode** trav, const int2** up, const int2** oct_aux,
                          const uint8_t** pn, const int32_t** leaf_parent, const int32_t** leaf_oct,
                          const sx_otnode** oct, int64_t* oct_cap, const float** bounds, const int32_t** counts,
                          int64_t* limit, void* stream);
+// the same for a batch built with per-cloud octrees (SX_ESTATE without them)
+int batch_feature_arrays(sx_batch* b, BatchFeatures* out, void* stream);
 }
 
 using sx::fail;
@@ -895,8 +935,8 @@ int sx_radius_features(sx_index* index, int64_t q_start, int64_t n_q, float* d_f
     if (n_q == 0) return SX_OK;
     if (!d_feats || reinterpret_cast<uintptr_t>(d_feats) % 16 != 0)
         return fail(SX_EINVAL, "sx_radius_features: output must be 16-byte aligned");
-    sx::features_kernel<<<sx::grid_for(n_q, 256), 256, 0, sx::as_stream(stream)>>>(
-        sx::FeatTree{trav, up, aux, pn, oct_cap}, lp, lo, oct, bounds, counts, q_start, n_q, d_feats);
+    sx::features_kernel<false><<<sx::grid_for(n_q, 256), 256, 0, sx::as_stream(stream)>>>(
+        sx::FeatTree{trav, up, aux, pn, oct_cap}, lp, lo, oct, bounds, counts, q_start, n_q, d_feats, sx::CloudMap());
     SX_LAUNCH_CHECK("features_kernel");
     return SX_OK;
 }
@@ -919,9 +959,45 @@ int sx_radius_predict(sx_index* index, sx_mlp* mlp, int64_t q_start, int64_t n_q
     if (n_q == 0) return SX_OK;
     if (!d_radius2) return fail(SX_EINVAL, "sx_radius_predict: null output");
     unsigned grid = 1;
-    SX_TRY(persistent_grid(sx::radius_kernel, g_resident_radius, n_q, &grid));
-    sx::radius_kernel<<<grid, sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
-        sx::FeatTree{trav, up, aux, pn, oct_cap}, lp, lo, oct, bounds, counts, q_start, n_q, dev_view(mlp), d_radius2);
+    SX_TRY(persistent_grid(sx::radius_kernel<false>, g_resident_radius, n_q, &grid));
+    sx::radius_kernel<false><<<grid, sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
+        sx::FeatTree{trav, up, aux, pn, oct_cap}, lp, lo, oct, bounds, counts, q_start, n_q, dev_view(mlp), d_radius2,
+        sx::CloudMap());
+    SX_LAUNCH_CHECK("radius_kernel");
+    return SX_OK;
+}
+
+// Features of a batch's slots [0, n_rows) (n_rows x 8 floats; rows of padding
+// slots are zero); each cloud's rows equal sx_radius_features of a standalone
+// index of that cloud.
+int sx_batch_radius_features(sx_batch* batch, float* d_feats, int64_t n_rows, void* stream) {
+    sx::BatchFeatures bf;
+    SX_TRY(sx::batch_feature_arrays(batch, &bf, stream));
+    if (n_rows < 0 || n_rows > bf.limit) return fail(SX_EINVAL, "sx_batch_radius_features: n_rows outside [0, slot capacity]");
+    if (n_rows == 0) return SX_OK;
+    if (!d_feats || reinterpret_cast<uintptr_t>(d_feats) % 16 != 0)
+        return fail(SX_EINVAL, "sx_batch_radius_features: output must be 16-byte aligned");
+    sx::features_kernel<true><<<sx::grid_for(n_rows, 256), 256, 0, sx::as_stream(stream)>>>(
+        sx::FeatTree{bf.trav, nullptr, bf.aux, bf.pn, bf.oct_cap}, bf.leaf_parent, bf.leaf_oct, bf.oct, bf.bounds,
+        bf.slots, 0, n_rows, d_feats, bf.cm);
+    SX_LAUNCH_CHECK("features_kernel");
+    return SX_OK;
+}
+
+// Squared search radius of a batch's slots [0, n_rows): features and MLP in one
+// kernel, equal per cloud to sx_radius_predict of a standalone index.
+int sx_batch_radius_predict(sx_batch* batch, sx_mlp* mlp, float* d_radius2, int64_t n_rows, void* stream) {
+    SX_TRY(check_mlp(mlp));
+    sx::BatchFeatures bf;
+    SX_TRY(sx::batch_feature_arrays(batch, &bf, stream));
+    if (n_rows < 0 || n_rows > bf.limit) return fail(SX_EINVAL, "sx_batch_radius_predict: n_rows outside [0, slot capacity]");
+    if (n_rows == 0) return SX_OK;
+    if (!d_radius2) return fail(SX_EINVAL, "sx_batch_radius_predict: null output");
+    unsigned grid = 1;
+    SX_TRY(persistent_grid(sx::radius_kernel<true>, g_resident_radius_batch, n_rows, &grid));
+    sx::radius_kernel<true><<<grid, sx::kMlpBlock, 0, sx::as_stream(stream)>>>(
+        sx::FeatTree{bf.trav, nullptr, bf.aux, bf.pn, bf.oct_cap}, bf.leaf_parent, bf.leaf_oct, bf.oct, bf.bounds,
+        bf.slots, 0, n_rows, dev_view(mlp), d_radius2, bf.cm);
     SX_LAUNCH_CHECK("radius_kernel");
     return SX_OK;
 }

@@ -135,6 +135,13 @@ __device__ __forceinline__ void stage_node(int2* w, const int32_t ch[8], float3 
     w[6] = make_int2(node_mask, leaf_mask);
 }
 
+// kBatch: the slots of many clouds (cloud map cm; cloud c's radix node i at
+// slot base[c] + i, counts 0 on every slot that is not an internal node).
+// The scan of the counts runs over all slots, so cloud c's nodes are one run
+// starting at oct_base[c]; child links are written relative to the cloud
+// (octree children minus oct_base[c], leaves minus base[c]), so each cloud's
+// run is byte for byte the octree of a standalone build of that cloud.
+template <bool kBatch>
 __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count,
                                                          int64_t n_max, int code_bits, const float* __restrict__ bounds,
                                                          const uint8_t* __restrict__ pn,
@@ -143,7 +150,8 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
                                                          const uint8_t* __restrict__ hlr,
                                                          const int32_t* __restrict__ offsets,
                                                          const int32_t* __restrict__ counts, sx_otnode* nodes,
-                                                         int64_t cap, int32_t* leaf_oct) {
+                                                         int64_t cap, int32_t* leaf_oct, CloudMap cm,
+                                                         const int32_t* __restrict__ oct_base) {
     // the radix arrays of the block's window, then (same memory) the staged nodes
     __shared__ __align__(16) unsigned char s_raw[kGatherLds];
     uint64_t* s_code = reinterpret_cast<uint64_t*>(s_raw);
@@ -152,7 +160,7 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
     int32_t* s_off = s_cnt + kGatherWin;
     uint8_t* s_leaf = reinterpret_cast<uint8_t*>(s_off + kGatherWin);  // bit 0: left child a leaf, bit 1: right
     const int64_t U = scan_count(d_count, n_max);
-    const int64_t M = U - 1;
+    const int64_t M = kBatch ? U : U - 1;  // slots that may hold a radix node
     // XCD-contiguous block order: a block's halo is its neighbours' nodes, read
     // from the same L2
     const int64_t blk0 = static_cast<int64_t>(xcd_block()) * kGatherBlock;
@@ -206,9 +214,19 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
 
     const int64_t i = blk0 + threadIdx.x;
     const int c = i < M ? cnt_at(i) : 0;
-    const float mn = bounds[0], range = bounds[2];
+    // the thread's cloud: first slot, first octree node, cube
+    int64_t s0 = 0;
+    int o0 = 0;
+    const float* bd = bounds;
+    if (kBatch && c > 0) {
+        const int cid = cm.tile_cloud[i / kCloudAlign];
+        s0 = cm.base[cid];
+        o0 = oct_base[cid];
+        bd = bounds + 4 * cid;
+    }
+    const float mn = bd[0], range = bd[2];
     const float fscale = static_cast<float>(axis_scale(code_bits));
-    const int root_level = pn[0] / 3;
+    const int root_level = c > 0 ? pn[s0] / 3 : 0;
     int D = 0, base = 0;
     uint64_t ci = 0;
     int32_t ch[8];
@@ -238,7 +256,7 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
                 const int child = g + side;
                 if ((leafs >> side) & 1u) {
                     const int slot = static_cast<int>(code_prefix(code_at(child), D + 1, code_bits) & 7);
-                    ch[slot] = child;
+                    ch[slot] = static_cast<int32_t>(child - s0);
                     leaf_mask |= 1 << slot;
                     if (leaf_oct) leaf_oct[child] = base;
                     continue;
@@ -246,7 +264,7 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
                 const int cc = cnt_at(child);
                 if (cc > 0) {
                     const int slot = static_cast<int>(code_prefix(code_at(child), D + 1, code_bits) & 7);
-                    ch[slot] = off_at(child) + cc - 1;
+                    ch[slot] = off_at(child) + cc - 1 - o0;
                     node_mask |= 1 << slot;
                 } else if (sp < 8) {
                     stack[sp++] = child;
@@ -260,8 +278,8 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
     float cell = 0.0f;
     const RawXYZ ri = raw_xyz(ci);
     if (c > 0) {
-        if (i == 0) {
-            corner = cell_corner(code_prefix(codes[0], root_level, code_bits), root_level, code_bits, mn, range, fscale);
+        if (i == s0) {
+            corner = cell_corner(code_prefix(codes[s0], root_level, code_bits), root_level, code_bits, mn, range, fscale);
             cell = range;
         } else {
             corner = cell_corner_raw(ri, D, code_bits, mn, range, fscale);
@@ -275,7 +293,7 @@ __global__ void __launch_bounds__(kGatherBlock) oct_gather_kernel(const uint64_t
 #pragma unroll
         for (int s = 0; s < 8; ++s) cj[s] = 0;
         const int slot = static_cast<int>(code_prefix(ci, L + 1, code_bits) & 7);
-        cj[slot] = base + j - 1;
+        cj[slot] = base + j - 1 - o0;
         cc = cell_corner_raw(ri, L, code_bits, mn, range, fscale);
         cs = ldexpf(range, root_level - L);
         cm = 1 << slot;
@@ -345,9 +363,57 @@ int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
     else
         SX_TRY(device_scan(OctCountIn{pn, parent, d_count}, OctCountOut{offsets, counts}, d_count, n_max,
                            d_oct_count, scan_ws, st));
-    oct_gather_kernel<<<grid_for(n_max, kGatherBlock), kGatherBlock, 0, st>>>(codes, d_count, n_max, code_bits, bounds, pn,
-                                                                               left_child, hll, hlr, offsets, counts,
-                                                                               nodes, cap, leaf_oct);
+    oct_gather_kernel<false><<<grid_for(n_max, kGatherBlock), kGatherBlock, 0, st>>>(
+        codes, d_count, n_max, code_bits, bounds, pn, left_child, hll, hlr, offsets, counts, nodes, cap, leaf_oct,
+        CloudMap(), nullptr);
+    SX_LAUNCH_CHECK("oct_gather_kernel");
+    return SX_OK;
+}
+
+// Edge counts of a batch's slots, written by the radix-tree kernel (0 on every
+// slot that is not an internal node, padding included).
+struct OctCountSlots {
+    const int32_t* counts;
+    const int32_t* d_slots;
+    __device__ int operator()(int64_t i) const { return i < static_cast<int64_t>(*d_slots) ? counts[i] : 0; }
+};
+
+// First octree node and node count of every cloud: the scanned offset at its
+// first slot, up to the offset at its first slot past its unique points
+// (padding slots count 0), or the batch total past the last slot.
+__global__ void __launch_bounds__(256) cloud_oct_kernel(const int32_t* __restrict__ offsets,
+                                                        const int32_t* __restrict__ d_slots,
+                                                        const int32_t* __restrict__ d_total, CloudMap cm,
+                                                        int n_clouds, int32_t* __restrict__ oct_base,
+                                                        int32_t* __restrict__ oct_size) {
+    const int c = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= n_clouds) return;
+    const int64_t slots = *d_slots, total = *d_total;
+    const int64_t s0 = cm.base[c], s1 = s0 + cm.size[c];
+    const int64_t o0 = s0 < slots ? offsets[s0] : total;
+    const int64_t o1 = s1 < slots ? offsets[s1] : total;
+    oct_base[c] = static_cast<int32_t>(o0);
+    oct_size[c] = static_cast<int32_t>(o1 - o0);
+}
+
+// The octrees of a batch's clouds in one pass over the slots: the counts
+// (given by the radix-tree kernel) are scanned over every slot, then each
+// cloud's node run is located and gathered with cloud-relative links.
+// offsets: the chain start of each slot (kept for the feature links).
+int batch_octree_launch(const uint64_t* codes, const int32_t* d_slots, int64_t slot_cap, int code_bits,
+                        const float* bounds, const uint8_t* pn, const int32_t* left_child, const uint8_t* hll,
+                        const uint8_t* hlr, const int32_t* counts, int32_t* offsets, CloudMap cm, int n_clouds,
+                        int32_t* oct_base, int32_t* oct_size, sx_otnode* nodes, int64_t cap, int32_t* d_oct_count,
+                        void* scan_ws, hipStream_t st) {
+    if (slot_cap <= 0 || n_clouds <= 0) return SX_OK;
+    SX_TRY(device_scan(OctCountSlots{counts, d_slots}, OctOffsetOut{offsets}, d_slots, slot_cap, d_oct_count, scan_ws,
+                       st));
+    cloud_oct_kernel<<<grid_for(n_clouds, 256), 256, 0, st>>>(offsets, d_slots, d_oct_count, cm, n_clouds, oct_base,
+                                                              oct_size);
+    SX_LAUNCH_CHECK("cloud_oct_kernel");
+    oct_gather_kernel<true><<<grid_for(slot_cap, kGatherBlock), kGatherBlock, 0, st>>>(
+        codes, d_slots, slot_cap, code_bits, bounds, pn, left_child, hll, hlr, offsets, counts, nodes, cap, nullptr,
+        cm, oct_base);
     SX_LAUNCH_CHECK("oct_gather_kernel");
     return SX_OK;
 }
@@ -363,46 +429,53 @@ int octree_launch(const uint64_t* codes, const int32_t* d_count, int64_t n_max, 
 //     every node of radix node i's chain holds exactly i's leaf range.
 // The walks are short: a radix node without a chain lies at its parent's
 // octree depth, and prefix lengths grow by at least one bit per level.
+// root: the slot of the tree's root (its count is 1, so every walk stops there).
 __device__ __forceinline__ int chain_holder(const int32_t* __restrict__ parent, const int32_t* __restrict__ counts,
-                                            int p, int count_p) {
+                                            int p, int count_p, int root) {
 #pragma unroll 1
-    for (int s = 0; s < 64 && p > 0 && count_p == 0; ++s) {  // the root's count is 1
+    for (int s = 0; s < 64 && p > root && count_p == 0; ++s) {
         p = parent[p];
         count_p = p >= 0 ? counts[p] : 1;
     }
-    return p < 0 ? 0 : p;
+    return p < 0 ? root : p;
 }
 
 // One thread per point t (leaf link) and radix node t (its chain's links);
 // the loads of each round are independent, so a thread waits for three
-// memory round trips when no walk is needed (most threads).
+// memory round trips when no walk is needed (most threads).  With a cloud
+// map, t is a slot and each cloud's tree is walked from its own root (the
+// links stay in slot and batch-octree numbering).
 __global__ void __launch_bounds__(256) feature_links_kernel(const int32_t* d_count, int64_t n_max,
                                                             const int32_t* __restrict__ parent,
                                                             const int32_t* __restrict__ leaf_parent,
                                                             const int32_t* __restrict__ node_end,
                                                             const int32_t* __restrict__ offsets,
                                                             const int32_t* __restrict__ counts, int64_t cap,
-                                                            int32_t* __restrict__ leaf_oct, int2* __restrict__ aux) {
-    const int64_t U = scan_count(d_count, n_max);
+                                                            int32_t* __restrict__ leaf_oct, int2* __restrict__ aux,
+                                                            CloudMap cm) {
+    const int64_t total = scan_count(d_count, n_max);
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (U < 2 || t >= U) return;
-    const int i = static_cast<int>(t);
-    const bool node = t < U - 1;
+    if (t >= total) return;
+    const int2 cloud = cm.at(t, static_cast<int>(total));
+    const int64_t U = cloud.y, li = t - cloud.x;
+    if (U < 2 || li >= U) return;
+    const int i = static_cast<int>(t), root = cloud.x;
+    const bool node = li < U - 1;
     // round 1
     const int lp = leaf_parent[i] >> 1;
     const int c = node ? counts[i] : 0;
-    const int pa = node && i > 0 ? parent[i] : 0;
+    const int pa = node && li > 0 ? parent[i] : root;
     const int base = node ? offsets[i] : 0;
     const int e = node ? node_end[i] : i;
     // round 2
     const int c_lp = counts[lp];
     const int c_pa = counts[pa];
-    const int hl = chain_holder(parent, counts, lp, c_lp);
-    const int hp = chain_holder(parent, counts, pa, c_pa);
+    const int hl = chain_holder(parent, counts, lp, c_lp, root);
+    const int hp = chain_holder(parent, counts, pa, c_pa, root);
     // round 3
     leaf_oct[i] = offsets[hl];
     if (c <= 0) return;
-    const int top_parent = i == 0 ? -1 : offsets[hp];
+    const int top_parent = li == 0 ? -1 : offsets[hp];
     const int sz = (e > i ? e - i : i - e) + 1;
     for (int j = 0; j < c; ++j) {
         if (base + j >= cap) break;
@@ -412,10 +485,10 @@ __global__ void __launch_bounds__(256) feature_links_kernel(const int32_t* d_cou
 
 int feature_links_launch(const int32_t* d_count, int64_t n_max, const int32_t* parent, const int32_t* leaf_parent,
                          const int32_t* node_end, const int32_t* offsets, const int32_t* counts, int64_t cap,
-                         int32_t* leaf_oct, int2* aux, hipStream_t st) {
+                         int32_t* leaf_oct, int2* aux, hipStream_t st, CloudMap cm) {
     if (n_max <= 0) return SX_OK;
     feature_links_kernel<<<grid_for(n_max, 256), 256, 0, st>>>(d_count, n_max, parent, leaf_parent, node_end, offsets,
-                                                               counts, cap, leaf_oct, aux);
+                                                               counts, cap, leaf_oct, aux, cm);
     SX_LAUNCH_CHECK("feature_links_kernel");
     return SX_OK;
 }

@@ -124,6 +124,22 @@ struct ParentRef {
     __device__ __forceinline__ int at(int node) const { return p[static_cast<int64_t>(node) * stride]; }
 };
 
+// The arrays a batch's learned-radius features read (slot numbering; octree
+// links in batch-octree numbering) and the bound on its rows.
+struct BatchFeatures {
+    const TravNode* trav = nullptr;
+    const int2* aux = nullptr;        // (parent octree node, points in the cell)
+    const uint8_t* pn = nullptr;      // prefix lengths
+    const int32_t* leaf_parent = nullptr;
+    const int32_t* leaf_oct = nullptr;
+    const sx_otnode* oct = nullptr;
+    int64_t oct_cap = 0;
+    const float* bounds = nullptr;    // per cloud, stride 4
+    const int32_t* slots = nullptr;   // device slot count
+    CloudMap cm;
+    int64_t limit = 0;                // slot capacity
+};
+
 // ---------------------------------------------------------------------------
 // Device-wide scan of int32 values (reduce, then scan of tile sums, then scan
 // within tiles).  In: int operator()(int64_t i) const for i < n.  Out:

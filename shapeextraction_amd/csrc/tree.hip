@@ -152,6 +152,9 @@ karras_kernel(const uint64_t* __restrict__ codes, const int32_t* d_count, int64_
     if (in_range && (i == 0 || i == N) && parent) parent[g] = -1;
     if (in_range && i == 0 && up) up[g].x = -1;
     const bool active = in_range && i < N;
+    // slots that hold no internal node (a cloud's last leaf, a batch's
+    // padding) own no octree chain
+    if (oct_count && in_range && !active) oct_count[g] = 0;
     // code of the cloud's leaf x (local index): from LDS inside [w0, w1), else
     // from HBM.  The LDS read is unconditional (index clamped) and the empty
     // asm keeps it apart from the HBM read: merged they would become one

@@ -30,7 +30,7 @@ def test_python_signatures_cover_header():
 
 
 def test_abi_version():
-    assert _lib.load().sx_abi_version() == 1
+    assert _lib.load().sx_abi_version() == 2
 
 
 def test_otnode_layout():

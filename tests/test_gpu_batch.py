@@ -268,3 +268,129 @@ def test_batch_knn_respects_row_capacity(gpu):
     keep = valid.copy()
     keep[rows:] = False
     assert np.array_equal(got[keep], full[keep])
+
+
+# ---------------------------------------------------------------- per-cloud octrees
+
+def _batch_oct(sx, clouds, oct_factor=3.0):
+    off = np.zeros(len(clouds) + 1, np.int64)
+    off[1:] = np.cumsum([len(c) for c in clouds])
+    allp = np.concatenate(clouds, 0)
+    b = sx.Batch(int(off[-1]), len(clouds))
+    b.set_clouds(off)
+    b.set_octree(oct_factor)
+    b.build(allp[:, 0].copy(), allp[:, 1].copy(), allp[:, 2].copy())
+    return b, off
+
+
+def test_batch_octrees_equal_standalone(gpu):
+    """Every cloud's octree run (nodes at oct_base[c], cloud-relative links) is
+    byte for byte the octree of a standalone build of that cloud, and its leaf
+    flags equal the standalone radix tree's; turning the octrees on changes no
+    neighbour row."""
+    sx = gpu
+    clouds = _mixed_clouds(sx, 3)
+    b, off = _batch_oct(sx, clouds)
+    tab = b.table()
+    bf = b.fetch()
+    assert bf["oct_base"][0] == 0 or tab["size"][0] == 0
+    assert int(bf["oct_base"][-1]) + int(bf["oct_size"][-1]) == bf["n_oct"]
+    print(f"[batch octree] {len(clouds)} clouds, {tab['n_unique']} unique points, {bf['n_oct']} octree nodes")
+    for c, cl in enumerate(clouds):
+        ob, osz = int(bf["oct_base"][c]), int(bf["oct_size"][c])
+        if len(cl) == 0:
+            assert osz == 0
+            continue
+        s = sx.Index(len(cl), 63).build(cl[:, 0].copy(), cl[:, 1].copy(), cl[:, 2].copy()).fetch()
+        assert osz == s["n_oct"], c
+        assert bf["oct_nodes"][ob:ob + osz].tobytes() == s["oct_nodes"].tobytes(), c
+        base, U = int(tab["base"][c]), int(tab["size"][c])
+        if U >= 2:
+            inner = slice(base, base + U - 1)
+            assert np.array_equal(bf["has_leaf_left"][inner], s["has_leaf_left"][: U - 1]), c
+            assert np.array_equal(bf["has_leaf_right"][inner], s["has_leaf_right"][: U - 1]), c
+    plain, _ = _batch(sx, clouds)
+    valid = np.zeros(tab["n_slots"], bool)
+    for c in range(len(clouds)):
+        valid[tab["base"][c]:tab["base"][c] + tab["size"][c]] = True
+    assert np.array_equal(b.knn(K).numpy()[valid], plain.knn(K).numpy()[valid])
+
+
+def test_batch_octrees_vs_oracle(gpu, O):
+    sx = gpu
+    g = rng(12)
+    clouds = [g.uniform(-5, 5, size=(n, 3)).astype(np.float32) for n in (2, 3, 40, 333, 1500, 64, 65)]
+    clouds.append(g.integers(0, 4, size=(900, 3)).astype(np.float32))  # duplicates
+    b, off = _batch_oct(sx, clouds)
+    bf = b.fetch()
+    for c, cl in enumerate(clouds):
+        x, y, z = cl[:, 0], cl[:, 1], cl[:, 2]
+        bounds, _ = O.bbox(x, y, z)
+        codes = O.encode(x, y, z, bounds[0], bounds[2], 63)
+        ks, perm = O.sort_pairs(codes)
+        uq, _, _ = O.unique(ks, perm)
+        nodes = O.octree(uq, 63, bounds[0], bounds[2], O.karras(uq, 63, 0))
+        ob, osz = int(bf["oct_base"][c]), int(bf["oct_size"][c])
+        assert osz == len(nodes), c
+        assert bf["oct_nodes"][ob:ob + osz].tobytes() == nodes.tobytes(), c
+
+
+def test_batch_octree_capacity_overflow_reported(gpu):
+    sx = gpu
+    x, y, z = sx.gen_scene(3, 50_000)
+    b = sx.Batch(len(x), 1)
+    b.set_clouds(np.array([0, len(x)], np.int64))
+    b.set_octree(0.05)
+    b.build(x, y, z)
+    with pytest.raises(sx.SxError) as e:
+        b.view()
+    assert e.value.code == 4  # SX_ERANGE
+    b.set_octree(0.0)  # off again: the build has no octree and a valid view
+    b.build(x, y, z)
+    v = b.view()
+    assert v.n_oct == 0 and not v.oct_nodes
+
+
+def test_batch_learned_radius_equals_standalone(gpu):
+    """Features, fused radii and the radius-seeded kNN of a batch equal those
+    of standalone indexes cloud by cloud (bit for bit), and the seeded rows
+    equal the plain rows; a captured graph with octrees replays to the eager
+    octrees."""
+    sx = gpu
+    from shapeextraction_amd import learned as L
+
+    model = L.RadiusModel.load(L.default_model_path(K))
+    clouds = _mixed_clouds(sx, 21)
+    b, off = _batch_oct(sx, clouds)
+    tab = b.table()
+    feats = b.radius_features().numpy()
+    r2 = b.radius_predict(model)
+    r2_h = r2.numpy()
+    nn_r = b.knn_radius(K, r2).numpy()
+    nn_p = b.knn(K).numpy()
+    for c, cl in enumerate(clouds):
+        base, U = int(tab["base"][c]), int(tab["size"][c])
+        if U == 0:
+            continue
+        ix = sx.Index(len(cl), 63).build(cl[:, 0].copy(), cl[:, 1].copy(), cl[:, 2].copy())
+        sl = slice(base, base + U)
+        assert np.array_equal(feats[sl].view(np.uint32), L.features(ix).numpy().view(np.uint32)), c
+        assert np.array_equal(r2_h[sl].view(np.uint32), model.predict_index_radius2(ix).numpy().view(np.uint32)), c
+        assert np.array_equal(nn_r[sl], nn_p[sl]), c
+    # graph replays rebuild the octrees (and their feature links) in place
+    D = sx.DeviceArray
+    allp = np.concatenate(clouds, 0)
+    dx, dy, dz = (D.from_numpy(allp[:, i].copy()) for i in range(3))
+    S = tab["n_slots"]
+    nn_g, nrm_g = D((S, K), np.int32), D((S, 3), np.float32)
+    eager = b.fetch()["oct_nodes"].tobytes()
+    st = sx.Stream()
+    graph = b.graph(dx, dy, dz, K, nn_g, nrm_g, st.handle)
+    for rep in range(2):
+        graph.launch(st.handle)
+        st.sync()
+        b._view = None
+        assert b.fetch()["oct_nodes"].tobytes() == eager, rep
+        assert np.array_equal(b.radius_features().numpy().view(np.uint32), feats.view(np.uint32)), rep
+        sx.synchronize()
+    graph.close()
