@@ -480,6 +480,45 @@ def bench_batched(sx, args, dev) -> int:
     assert np.isfinite(nrm[valid]).all()
     log(f"eager {eager_ms:.2f} ms (wall {eager_wall:.2f}), graph {graph_ms:.2f} ms (wall {graph_wall:.2f}), "
         f"deferred {deferred}")
+    graph.close()
+
+    # the same batch with every cloud's octree: plain pipeline, and the learned
+    # radius (features + MLP, radius-seeded kNN) on top of it
+    from shapeextraction_amd import learned as L
+
+    model = L.RadiusModel.load(L.default_model_path(k))
+    r2 = sx.DeviceArray((S,), np.float32)
+    nn_l = sx.DeviceArray((S, k), np.int32)
+
+    def build_oct():
+        sx._lib.call("sx_batch_build", b.handle, dx.ptr, dy.ptr, dz.ptr, st.handle)
+
+    build_ms, _ = timed(build_oct, args.steps, args.warmup)  # octrees still off
+    b.set_octree(3.0)
+
+    def plain_oct():
+        build_oct()
+        sx._lib.call("sx_batch_knn", b.handle, k, nn.ptr, S, st.handle)
+        sx._lib.call("sx_batch_normals", b.handle, nn.ptr, k, sx.SX_PREC_FP32, 0.0, 0.0, 0.0, normals.ptr, None, S,
+                     st.handle)
+
+    def learned_oct():
+        build_oct()
+        sx._lib.call("sx_batch_radius_predict", b.handle, model.handle, r2.ptr, S, st.handle)
+        sx._lib.call("sx_batch_knn_radius", b.handle, k, r2.ptr, nn_l.ptr, S, st.handle)
+        sx._lib.call("sx_batch_normals", b.handle, nn_l.ptr, k, sx.SX_PREC_FP32, 0.0, 0.0, 0.0, normals.ptr, None, S,
+                     st.handle)
+
+    build_oct()
+    b.view(stream=st.handle)  # capacity check of the octrees
+    oct_nodes = int(b._view.n_oct)
+    oct_build_ms, _ = timed(build_oct, args.steps, args.warmup)
+    oct_plain_ms, _ = timed(plain_oct, args.steps, args.warmup)
+    oct_learned_ms, _ = timed(learned_oct, args.steps, args.warmup)
+    same = np.array_equal(nn.numpy(st.handle)[valid], nn_l.numpy(st.handle)[valid])
+    assert same, "learned rows differ from plain rows"
+    log(f"build alone {build_ms:.2f} ms; with octrees ({oct_nodes} nodes): build {oct_build_ms:.2f} ms, plain {oct_plain_ms:.2f} ms, "
+        f"learned {oct_learned_ms:.2f} ms (rows equal: {same})")
     line = {
         "metric": "points/sec batched normals, 512 clouds (log-uniform 1k-200k points), k=16",
         "value": n / (graph_wall / 1000.0),
@@ -506,9 +545,11 @@ def bench_batched(sx, args, dev) -> int:
         "autotune": {"bucket": tune["bucket"], "queue_blocks": tune["queue_blocks"],
                      "candidates_ms": tune["ms"].tolist()},
         "deferred_queries": deferred,
+        "build_ms": build_ms,
+        "with_octrees_ms": {"octree_nodes": oct_nodes, "build": oct_build_ms, "plain": oct_plain_ms,
+                            "learned": oct_learned_ms, "learned_rows_equal_plain": bool(same)},
     }
     print(json.dumps(line), flush=True)
-    graph.close()
     return 0
 
 
