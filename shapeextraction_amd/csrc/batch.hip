@@ -282,6 +282,12 @@ int check_batch(const sx_batch* b) {
     return SX_OK;
 }
 
+// Orders `stream` after the last eager build, whichever stream it ran on.
+int after_build(const sx_batch* b, void* stream) {
+    SX_HIP(hipStreamWaitEvent(sx::as_stream(stream), b->built_event, 0));
+    return SX_OK;
+}
+
 int cloud_bits(int n_clouds) {
     int bits = 1;
     while ((1ll << bits) < n_clouds) ++bits;
@@ -363,7 +369,7 @@ int batch_feature_arrays(sx_batch* b, BatchFeatures* out, void* stream) {
     if (!b->octree())
         return fail(SX_ESTATE, "batch features need the per-cloud octrees: call sx_batch_set_octree, then build");
     // ordered after the last build, whichever stream it ran on
-    SX_HIP(hipStreamWaitEvent(as_stream(stream), b->built_event, 0));
+    SX_TRY(after_build(b, stream));
     out->trav = b->trav;
     out->aux = b->oct_aux;
     out->pn = b->pn;
@@ -540,6 +546,7 @@ int sx_batch_knn(sx_batch* b, int k, int32_t* d_out, int64_t n_rows, void* strea
     if (!d_out || reinterpret_cast<uintptr_t>(d_out) % 16 != 0)
         return fail(SX_EINVAL, "sx_batch_knn: output must be a 16-byte aligned device buffer");
     if (n_rows < 0 || n_rows > b->slot_cap) return fail(SX_EINVAL, "sx_batch_knn: n_rows outside [0, slot capacity]");
+    SX_TRY(after_build(b, stream));
     return batch_knn(b, k, d_out, n_rows, sx::as_stream(stream));
 }
 
@@ -551,6 +558,7 @@ int sx_batch_knn_radius(sx_batch* b, int k, const float* d_radius2, int32_t* d_o
         return fail(SX_EINVAL, "sx_batch_knn_radius: output must be a 16-byte aligned device buffer");
     if (n_rows < 0 || n_rows > b->slot_cap)
         return fail(SX_EINVAL, "sx_batch_knn_radius: n_rows outside [0, slot capacity]");
+    SX_TRY(after_build(b, stream));
     return batch_knn(b, k, d_out, n_rows, sx::as_stream(stream), d_radius2);
 }
 
@@ -561,6 +569,7 @@ int sx_batch_normals(sx_batch* b, const int32_t* d_nn, int k, int precision, flo
     if (!d_nn || !d_normals) return fail(SX_EINVAL, "sx_batch_normals: null buffer");
     if (n_rows < 0 || n_rows > b->slot_cap)
         return fail(SX_EINVAL, "sx_batch_normals: n_rows outside [0, slot capacity]");
+    SX_TRY(after_build(b, stream));
     return batch_normals(b, d_nn, k, precision, vx, vy, vz, d_normals, d_flags, n_rows, sx::as_stream(stream));
 }
 
@@ -637,6 +646,7 @@ int sx_batch_autotune(sx_batch* b, int k, int32_t* d_scratch_nn, int64_t n_rows,
     if (!d_scratch_nn || !h_buckets || !h_queue_blocks || n_buckets < 1 || n_queue < 1 || reps < 1 || n_rows < 0 ||
         n_rows > b->slot_cap)
         return fail(SX_EINVAL, "sx_batch_autotune: bad argument");
+    SX_TRY(after_build(b, stream));
     hipStream_t st = sx::as_stream(stream);
     hipEvent_t e0, e1;
     SX_HIP(hipEventCreate(&e0));
