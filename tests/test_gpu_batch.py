@@ -272,11 +272,11 @@ def test_batch_knn_respects_row_capacity(gpu):
 
 # ---------------------------------------------------------------- per-cloud octrees
 
-def _batch_oct(sx, clouds, oct_factor=3.0):
+def _batch_oct(sx, clouds, oct_factor=3.0, bits=63):
     off = np.zeros(len(clouds) + 1, np.int64)
     off[1:] = np.cumsum([len(c) for c in clouds])
     allp = np.concatenate(clouds, 0)
-    b = sx.Batch(int(off[-1]), len(clouds))
+    b = sx.Batch(int(off[-1]), len(clouds), bits)
     b.set_clouds(off)
     b.set_octree(oct_factor)
     b.build(allp[:, 0].copy(), allp[:, 1].copy(), allp[:, 2].copy())
@@ -316,20 +316,21 @@ def test_batch_octrees_equal_standalone(gpu):
     assert np.array_equal(b.knn(K).numpy()[valid], plain.knn(K).numpy()[valid])
 
 
-def test_batch_octrees_vs_oracle(gpu, O):
+@pytest.mark.parametrize("bits", [63, 30])
+def test_batch_octrees_vs_oracle(gpu, O, bits):
     sx = gpu
     g = rng(12)
     clouds = [g.uniform(-5, 5, size=(n, 3)).astype(np.float32) for n in (2, 3, 40, 333, 1500, 64, 65)]
     clouds.append(g.integers(0, 4, size=(900, 3)).astype(np.float32))  # duplicates
-    b, off = _batch_oct(sx, clouds)
+    b, off = _batch_oct(sx, clouds, bits=bits)
     bf = b.fetch()
     for c, cl in enumerate(clouds):
         x, y, z = cl[:, 0], cl[:, 1], cl[:, 2]
         bounds, _ = O.bbox(x, y, z)
-        codes = O.encode(x, y, z, bounds[0], bounds[2], 63)
+        codes = O.encode(x, y, z, bounds[0], bounds[2], bits)
         ks, perm = O.sort_pairs(codes)
         uq, _, _ = O.unique(ks, perm)
-        nodes = O.octree(uq, 63, bounds[0], bounds[2], O.karras(uq, 63, 0))
+        nodes = O.octree(uq, bits, bounds[0], bounds[2], O.karras(uq, bits, 0))
         ob, osz = int(bf["oct_base"][c]), int(bf["oct_size"][c])
         assert osz == len(nodes), c
         assert bf["oct_nodes"][ob:ob + osz].tobytes() == nodes.tobytes(), c
