@@ -410,7 +410,8 @@ int sx_batch_set_tuning(sx_batch* batch, int bucket, int sort_algo, int queue_bl
 /* oct_factor > 0: later builds (and graphs created later) also build every
  * cloud's octree, in storage for oct_factor x slot-capacity nodes allocated
  * here (3 is ample for scans; the view reports SX_ERANGE when a build needs
- * more); 0 frees it.  Synchronises the device; the batch must be rebuilt. */
+ * more); 0 frees it.  Synchronises the device; the batch must be rebuilt.
+ * SX_ESTATE while graphs of the batch exist (they write into that storage). */
 int sx_batch_set_octree(sx_batch* batch, float oct_factor);
 /* Asynchronous, no host synchronisation, capturable. */
 int sx_batch_build(sx_batch* batch, const float* d_x, const float* d_y, const float* d_z, void* stream);
@@ -439,6 +440,7 @@ int sx_batch_deferred(sx_batch* batch, int64_t* deferred, int reset, void* strea
  * the given created stream; launching it reruns the whole batch. */
 int sx_batch_graph_create(sx_batch* batch, const float* d_x, const float* d_y, const float* d_z, int k,
                           int32_t* d_nn, float* d_normals, int64_t n_rows, void** graph, void* stream);
+/* SX_ESTATE when the graph's batch has been destroyed. */
 int sx_batch_graph_launch(void* graph, void* stream);
 int sx_batch_graph_destroy(void* graph);
 /* Times the built batch's kNN for every (bucket, queue_blocks) candidate and

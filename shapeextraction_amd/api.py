@@ -515,7 +515,7 @@ class Batch:
         call("sx_batch_graph_create", self.handle, x.ptr, y.ptr, z.ptr, k, nn.ptr, normals.ptr, rows, C.byref(g),
              stream)
         self._view = None
-        return BatchGraph(g.value, (x, y, z, nn, normals))
+        return BatchGraph(g.value, (self, x, y, z, nn, normals))  # the batch outlives its graph
 
     def autotune(self, k: int, scratch: DeviceArray, buckets=(8, 16, 32), queue_blocks=(0, 2048), reps: int = 2,
                  stream=None) -> dict:

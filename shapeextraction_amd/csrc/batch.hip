@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <string>
+#include <vector>
 
 #include "sx_internal.hpp"
 
@@ -172,6 +173,15 @@ __global__ void __launch_bounds__(256) compact_kernel(int64_t n, const uint64_t*
     rep[slot] = static_cast<uint32_t>(perm[i] - off[c]);
 }
 
+// A captured batch pipeline.  Its kernels write into the owner's buffers, so
+// the owner tracks its live graphs: replacing the octree storage waits for
+// them to go, and a destroyed batch leaves its graphs unusable.
+struct BatchGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    sx_batch* owner = nullptr;  // null once the batch is destroyed
+};
+
 }  // namespace sx
 
 using sx::fail;
@@ -219,6 +229,7 @@ struct sx_batch {
     int n_clouds = 0;
     int64_t n_input = 0;
     bool built = false;
+    std::vector<sx::BatchGraph*> graphs;  // captured graphs not yet destroyed
     bool octree() const { return oct_mem != nullptr; }
     sx::CloudMap map() const { return sx::CloudMap{tile_cloud, base, size}; }
 };
@@ -387,10 +398,6 @@ int batch_feature_arrays(sx_batch* b, BatchFeatures* out, void* stream) {
 
 namespace {
 
-struct BatchGraph {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-};
 }  // namespace
 
 extern "C" {
@@ -429,6 +436,7 @@ int sx_batch_create(sx_batch** out, int64_t max_points, int max_clouds, int code
 
 int sx_batch_destroy(sx_batch* b) {
     if (!b) return SX_OK;
+    for (sx::BatchGraph* g : b->graphs) g->owner = nullptr;  // left unusable (see sx_batch_graph_launch)
     if (b->built_event) SX_HIP(hipEventDestroy(b->built_event));
     if (b->oct_mem) SX_HIP(hipFree(b->oct_mem));
     if (b->mem) SX_HIP(hipFree(b->mem));
@@ -440,6 +448,9 @@ int sx_batch_set_octree(sx_batch* b, float oct_factor) {
     if (!b) return fail(SX_EINVAL, "sx_batch_set_octree: null batch");
     if (!(oct_factor >= 0.0f) || oct_factor > 64.0f)
         return fail(SX_EINVAL, "sx_batch_set_octree: oct_factor must be in [0, 64]");
+    if (!b->graphs.empty())
+        return fail(SX_ESTATE, "sx_batch_set_octree: destroy the batch's graphs first (they write into the storage "
+                               "this call replaces)");
     SX_HIP(hipDeviceSynchronize());  // no build or query may still use the old storage
     if (b->oct_mem) {
         SX_HIP(hipFree(b->oct_mem));
@@ -596,7 +607,7 @@ int sx_batch_graph_create(sx_batch* b, const float* d_x, const float* d_y, const
         return fail(SX_EINVAL, "sx_batch_graph_create: n_rows outside [0, slot capacity]");
     hipStream_t st = sx::as_stream(stream);
     if (!st) return fail(SX_EINVAL, "sx_batch_graph_create: capture needs a created stream, not the null stream");
-    auto* g = new BatchGraph();
+    auto* g = new sx::BatchGraph();
     SX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = batch_build(b, d_x, d_y, d_z, st);
     if (rc == SX_OK) rc = batch_knn(b, k, d_nn, n_rows, st);
@@ -615,19 +626,27 @@ int sx_batch_graph_create(sx_batch* b, const float* d_x, const float* d_y, const
         return rc;
     }
     b->built = true;
+    g->owner = b;
+    b->graphs.push_back(g);
     *graph = g;
     return SX_OK;
 }
 
 int sx_batch_graph_launch(void* graph, void* stream) {
     if (!graph) return fail(SX_EINVAL, "sx_batch_graph_launch: null graph");
-    SX_HIP(hipGraphLaunch(static_cast<BatchGraph*>(graph)->exec, sx::as_stream(stream)));
+    auto* g = static_cast<sx::BatchGraph*>(graph);
+    if (!g->owner) return fail(SX_ESTATE, "sx_batch_graph_launch: the graph's batch was destroyed");
+    SX_HIP(hipGraphLaunch(g->exec, sx::as_stream(stream)));
     return SX_OK;
 }
 
 int sx_batch_graph_destroy(void* graph) {
     if (!graph) return SX_OK;
-    auto* g = static_cast<BatchGraph*>(graph);
+    auto* g = static_cast<sx::BatchGraph*>(graph);
+    if (g->owner) {
+        auto& v = g->owner->graphs;
+        v.erase(std::remove(v.begin(), v.end(), g), v.end());
+    }
     if (g->exec) SX_HIP(hipGraphExecDestroy(g->exec));
     if (g->graph) SX_HIP(hipGraphDestroy(g->graph));
     delete g;

@@ -395,3 +395,37 @@ def test_batch_learned_radius_equals_standalone(gpu):
         assert np.array_equal(b.radius_features().numpy().view(np.uint32), feats.view(np.uint32)), rep
         sx.synchronize()
     graph.close()
+
+
+def test_batch_graphs_pin_their_storage(gpu):
+    """A captured graph writes into its batch's buffers: the octree storage
+    cannot be replaced while a graph exists, and a graph whose batch was
+    destroyed refuses to launch."""
+    sx = gpu
+    clouds = _mixed_clouds(sx, 17)
+    off = np.zeros(len(clouds) + 1, np.int64)
+    off[1:] = np.cumsum([len(c) for c in clouds])
+    allp = np.concatenate(clouds, 0)
+    D = sx.DeviceArray
+    dx, dy, dz = (D.from_numpy(allp[:, i].copy()) for i in range(3))
+    b = sx.Batch(int(off[-1]), len(clouds))
+    b.set_clouds(off).set_octree(2.0)
+    b.build(dx, dy, dz)
+    S = b.table()["n_slots"]
+    nn, nrm = D((S, K), np.int32), D((S, 3), np.float32)
+    st = sx.Stream()
+    graph = b.graph(dx, dy, dz, K, nn, nrm, st.handle)
+    with pytest.raises(sx.SxError) as e:
+        b.set_octree(0.0)
+    assert e.value.code == 6  # SX_ESTATE
+    graph.launch(st.handle)
+    st.sync()
+    graph.close()
+    b.set_octree(0.0)  # no graph left
+    graph = b.graph(dx, dy, dz, K, nn, nrm, st.handle)
+    sx._lib.call("sx_batch_destroy", b.handle)
+    b.handle = None
+    with pytest.raises(sx.SxError) as e:
+        graph.launch(st.handle)
+    assert e.value.code == 6
+    graph.close()
