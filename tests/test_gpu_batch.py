@@ -352,23 +352,24 @@ def test_batch_octree_capacity_overflow_reported(gpu):
     assert v.n_oct == 0 and not v.oct_nodes
 
 
-def test_batch_learned_radius_equals_standalone(gpu):
+@pytest.mark.parametrize("k", [16, 32])
+def test_batch_learned_radius_equals_standalone(gpu, k):
     """Features, fused radii and the radius-seeded kNN of a batch equal those
     of standalone indexes cloud by cloud (bit for bit), and the seeded rows
-    equal the plain rows; a captured graph with octrees replays to the eager
-    octrees."""
+    equal the plain rows (both committed models); a captured graph with
+    octrees replays to the eager octrees."""
     sx = gpu
     from shapeextraction_amd import learned as L
 
-    model = L.RadiusModel.load(L.default_model_path(K))
+    model = L.RadiusModel.load(L.default_model_path(k))
     clouds = _mixed_clouds(sx, 21)
     b, off = _batch_oct(sx, clouds)
     tab = b.table()
     feats = b.radius_features().numpy()
     r2 = b.radius_predict(model)
     r2_h = r2.numpy()
-    nn_r = b.knn_radius(K, r2).numpy()
-    nn_p = b.knn(K).numpy()
+    nn_r = b.knn_radius(k, r2).numpy()
+    nn_p = b.knn(k).numpy()
     for c, cl in enumerate(clouds):
         base, U = int(tab["base"][c]), int(tab["size"][c])
         if U == 0:
