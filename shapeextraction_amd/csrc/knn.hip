@@ -504,6 +504,19 @@ __device__ __forceinline__ int range_point(const int2* rng, const int* pre, int 
     return rng[lo].y - (pre[lo] - 1 - f);
 }
 
+// Inclusive wave sum whose lane-permute addresses are rebuilt at every call:
+// the compiler would otherwise hoist them out of the kNN's loops and keep six
+// registers live through the whole kernel.
+__device__ __forceinline__ int wave_incl_sum_fresh(int v) {
+    const int lane = opaque(static_cast<int>(lane_id()));
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __builtin_amdgcn_ds_bpermute((lane >= o ? lane - o : lane) << 2, v);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
 // In place: range i = (first, last) becomes (inclusive size prefix, last),
 // so the prefix needs no array of its own; returns the total (uniform).
 __device__ __forceinline__ int range_prefix_inplace(int2* rng, int nr) {
@@ -512,7 +525,7 @@ __device__ __forceinline__ int range_prefix_inplace(int2* rng, int nr) {
     for (int base = 0; base < nr; base += kWave) {
         const int idx = base + lane;
         const int2 rg = idx < nr ? rng[idx] : make_int2(0, -1);
-        const int incl = total + wave_incl_sum(rg.y - rg.x + 1);
+        const int incl = total + wave_incl_sum_fresh(rg.y - rg.x + 1);
         if (idx < nr) rng[idx].x = incl;
         total = __builtin_amdgcn_readlane(incl, kWave - 1);
     }
@@ -709,7 +722,7 @@ __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, co
 }
 
 // Waves per SIMD, measured on the 8M scene (K = 16): plain eight (64 VGPRs,
-// 16 bytes per lane spilled; kNN 2.72 ms against 2.76 at seven with 72 VGPRs
+// 8 bytes per lane spilled; kNN 2.72 ms against 2.76 at seven with 72 VGPRs
 // and 2.89 at six with 77), seeded seven (eight: no gain).  K = 24: seven
 // (4M degenerate scene: kNN 2.41 ms against 2.46 at six, 5.6 at eight);
 // K = 32: five (16M mixture: 11.02 ms against 12.16 at its natural ~120
