@@ -723,12 +723,12 @@ __device__ __forceinline__ void stream_ranges(const float4* __restrict__ pts, co
 
 // Waves per SIMD, measured on the 8M scene (K = 16): plain eight (64 VGPRs,
 // 8 bytes per lane spilled; kNN 2.72 ms against 2.76 at seven with 72 VGPRs
-// and 2.89 at six with 77), seeded seven (eight: no gain).  K = 24: seven
-// (4M degenerate scene: kNN 2.41 ms against 2.46 at six, 5.6 at eight);
-// K = 32: five (16M mixture: 11.02 ms against 12.16 at its natural ~120
-// VGPRs, four waves, and 11.49 at six).
+// and 2.89 at six with 77), seeded eight too (64 VGPRs, no spill: 2.69
+// against 2.73 ms at seven).  K = 24: seven (4M degenerate scene: kNN 2.41
+// ms against 2.46 at six, 6.3 at eight); K = 32: five (16M mixture: 11.0 ms
+// against 12.16 at its natural ~120 VGPRs, four waves, and 21.7 at six).
 template <int K, bool kLearned>
-__global__ void __launch_bounds__(kKnnBlock) __attribute__((amdgpu_waves_per_eu(K <= 16 ? (kLearned ? 7 : 8) : (K <= 24 ? 7 : 5), 8)))
+__global__ void __launch_bounds__(kKnnBlock) __attribute__((amdgpu_waves_per_eu(K <= 16 ? 8 : (K <= 24 ? 7 : 5), 8)))
 knn_tile_kernel(const float4* __restrict__ pts,
                                                              const TravNode* __restrict__ nodes,
                                                              const int32_t* __restrict__ d_counts, int64_t q_start,
