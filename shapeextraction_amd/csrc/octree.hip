@@ -441,7 +441,7 @@ __device__ __forceinline__ int chain_holder(const int32_t* __restrict__ parent, 
 }
 
 // One thread per point t (leaf link) and radix node t (its chain's links);
-// the loads of each round are independent, so a thread waits for three
+// the loads of each of the three steps are independent, so a thread waits for three
 // memory round trips when no walk is needed (most threads).  With a cloud
 // map, t is a slot and each cloud's tree is walked from its own root (the
 // links stay in slot and batch-octree numbering).
@@ -461,18 +461,18 @@ __global__ void __launch_bounds__(256) feature_links_kernel(const int32_t* d_cou
     if (U < 2 || li >= U) return;
     const int i = static_cast<int>(t), root = cloud.x;
     const bool node = li < U - 1;
-    // round 1
+    // first loads (independent of each other)
     const int lp = leaf_parent[i] >> 1;
     const int c = node ? counts[i] : 0;
     const int pa = node && li > 0 ? parent[i] : root;
     const int base = node ? offsets[i] : 0;
     const int e = node ? node_end[i] : i;
-    // round 2
+    // second loads
     const int c_lp = counts[lp];
     const int c_pa = counts[pa];
     const int hl = chain_holder(parent, counts, lp, c_lp, root);
     const int hp = chain_holder(parent, counts, pa, c_pa, root);
-    // round 3
+    // third loads
     leaf_oct[i] = offsets[hl];
     if (c <= 0) return;
     const int top_parent = li == 0 ? -1 : offsets[hp];

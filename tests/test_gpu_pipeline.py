@@ -3,7 +3,7 @@
 Bit-exact: cube bounds, Morton codes, sort permutation, unique codes and
 representatives, radix tree arrays, octree nodes (byte for byte), decoded
 points, kNN index rows.  Normals: angular error against FP64 PCA on the same
-neighbour rows, median <= 0.1 deg and p99 <= 1 deg (the north-star gate),
+neighbour rows, median <= 0.1 deg and p99 <= 1 deg (the metric's precision gate),
 plus the orientation rule.
 """
 import numpy as np
