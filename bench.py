@@ -108,7 +108,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--k", type=int, default=16)
     ap.add_argument("--n", type=int, default=8_388_608)
-    ap.add_argument("--bucket", type=int, default=16)
+    ap.add_argument("--bucket", type=int, default=32)
     ap.add_argument("--knn", choices=["plain", "learned", "both"], default="both",
                     help="learned/both use shapeextraction_amd/models/radius_k{k}.bin (trained here if missing)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -253,7 +253,7 @@ int sx_index_get_view(sx_index* index, sx_index_view* view, void* stream);
  * (src/main.cu:119-120); this is the scatter back to the caller's order. */
 int sx_index_to_input_order(sx_index* index, const void* d_rows, int64_t row_bytes, void* d_out, void* stream);
 /* Bucket size of the kNN descent: a child holding at most `bucket` points is
- * scanned linearly instead of descended (default 16; tuning knob, no effect
+ * scanned linearly instead of descended (default 32; tuning knob, no effect
  * on results).  No reference counterpart (the reference descends to single
  * points, src/Octree.hpp:160-178). */
 int sx_index_set_bucket(sx_index* index, int bucket);
