@@ -1302,8 +1302,13 @@ int knn_tiles_k(const float4* pts, const TravNode* nodes, ParentRef par, const i
     int64_t gq = (n_q + kGroups - 1) / kGroups;  // one group per query at most
     if (gq > 8192) gq = 8192;
     if (queue_blocks > 0 && queue_blocks < gq) gq = queue_blocks;
+    // The 16-lane groups stream a leaf range one point per lane, so they stop
+    // descending at ranges of at most 16 points whatever the tiles use (8M
+    // scene with tile ranges of 32: kNN 2.69 -> 2.60 ms; 8 or 12: 3.4 / 3.0 ms).
+    const int group_bucket = bucket < kGroup ? bucket : kGroup;
     knn_group_kernel<K><<<static_cast<unsigned>(gq), kKnnBlock, 0, st>>>(pts, nodes, d_counts, q_start, queue,
-                                                                       queue_count, bucket, out, deferred_total, cm);
+                                                                       queue_count, group_bucket, out, deferred_total,
+                                                                       cm);
     SX_LAUNCH_CHECK("knn_group_kernel");
     knn_overflow_kernel<K><<<64, kKnnBlock, 0, st>>>(pts, nodes, par, d_counts, q_start, queue, queue_count, bucket, out,
                                                      cm);
