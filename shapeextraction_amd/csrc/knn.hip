@@ -349,7 +349,7 @@ constexpr int kWinSlots = pow2_ceil(kWinPoints<K>);
 // / 2048 / 1536 / 1024 (flat from 1792 to 2048).
 template <int K>
 constexpr int kSegBudget = 2048 * K / 16;
-constexpr float kSegGap = 8.0f;                          // segment break: gap > kSegGap x mean radius
+constexpr float kSegGap = 5.0f;                          // segment break: gap > kSegGap x mean radius (5 / 6 / 8 / 12: scene8m kNN 2.58 / 2.59 / 2.66 / 2.74 ms)
 constexpr float kRadiusStretch = 2.25f;                  // learned: accept up to 1.5x the predicted radius
 constexpr float kLearnedCap = 1.21f;                     // learned: radius capped at 1.1x the predicted one
 
